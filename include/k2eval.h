@@ -1,0 +1,139 @@
+/*
+ * k2eval.h -- C ABI of the MI355X (gfx950) K2 evaluator, libk2eval.so.
+ *
+ *   K2(v; T, k) = 2 sum_{j < n_zeros} e^{-k g_j^2} Re(w_j e^{i g_j v}),  w_j = 1/(rho_j zeta'(rho_j))
+ *
+ * plus dK2/dv and d2K2/dv2, for lists of candidate points v against a zero table.
+ *
+ * Encodings (little-endian uint64 word pairs):
+ *   G[2j], G[2j+1]  lo, hi words of gamma_j / 2pi in Q18.110 fixed point; gamma_j is defined
+ *                   as 2 pi G_j / 2^110, so the phase gamma_j v / 2pi is an exact integer product
+ *   gamma[j]        float64 gamma_j (used for the Gaussian weight and the derivative factors)
+ *   w[2j], w[2j+1]  real, imaginary part of w_j
+ *   V[2c], V[2c+1]  lo (fraction), hi (integer) words of v_c in 64.64 fixed point
+ *
+ * Conventions: every entry point returns 0 on success or a K2_E* code, and k2_last_error()
+ * describes the last failure of the calling thread.  Host buffers are borrowed for the duration
+ * of the call.  Calls on one table handle are serialised by the caller; distinct handles may be
+ * used from distinct threads.  *_device entry points take device pointers and a hipStream_t
+ * (as void*, NULL = the table's own stream) and are asynchronous with respect to the host.
+ *
+ * Reference interfaces replaced (reference repository bzyzz/mertens_bound):
+ *   search/path1_eval_fast.cpp:15-31   compute_peak_AB            (same signature, below)
+ *   search/path1_eval_fast.cpp:58-70   polish_peaks_with_weights  (same signature, below)
+ *   search/path1_eval_fast.cpp:15-53 + :77-81 fused -> k2_eval (zero table + candidate list ->
+ *   K2 values, without the n_peaks x n_zeros A/B matrices)
+ */
+#ifndef K2EVAL_H
+#define K2EVAL_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* flags */
+#define K2_VALUE 1u  /* K2 (always computed) */
+#define K2_D1 2u     /* dK2/dv */
+#define K2_D2 4u     /* d2K2/dv2 */
+#define K2_MIXED 8u  /* FP32 sin/cos + weights after the exact phase, compensated FP32 partial
+                        sums, FP64 head zeros and FP64 cross-block sum */
+
+/* error codes */
+#define K2_OK 0
+#define K2_E_ARG 1      /* invalid argument (null pointer, size out of range) */
+#define K2_E_HIP 2      /* HIP runtime error */
+#define K2_E_NOMEM 3    /* device allocation failed */
+
+typedef struct k2_table k2_table;
+
+/* one request of a batched call: the first n_zeros zeros of the table (gamma < T cutoff),
+ * Gaussian constant k, candidates V[2*cand_offset .. 2*(cand_offset+n_cand)) */
+typedef struct k2_request {
+    uint32_t n_zeros;
+    uint32_t n_cand;
+    uint64_t cand_offset;
+    double k;
+} k2_request;
+
+/* evaluation tiling (chosen by the autotuner; zero fields mean "library default") */
+typedef struct k2_tiling {
+    int32_t nc;            /* candidates per lane: 1, 2, 4 or 8 */
+    int32_t target_blocks; /* workgroups per launch the zero splits aim at */
+    int32_t mixed_head;    /* zeros evaluated in FP64 before the FP32 form takes over */
+} k2_tiling;
+
+const char *k2_last_error(void);
+int k2_version(void);
+
+/* Upload a zero table (host arrays).  The table keeps its own device copy. Replaces the
+ * reference's per-call table arguments of compute_peak_AB (path1_eval_fast.cpp:20-28). */
+int k2_table_create(k2_table **out, uint32_t n_zeros, const uint64_t *G, const double *gamma,
+                    const double *w);
+int k2_table_destroy(k2_table *t);
+uint32_t k2_table_size(const k2_table *t);
+int k2_set_tiling(k2_table *t, const k2_tiling *tiling);
+int k2_get_tiling(const k2_table *t, k2_tiling *tiling);
+
+/* K2 (+ derivatives) of n_cand candidates over the first n_zeros zeros (0 = all).
+ * Host buffers.  Replaces compute_peak_AB + the weighted sum h0 = sum w_i A_i
+ * (path1_eval_fast.cpp:15-53,77-81) with the reduction fused on the device. */
+int k2_eval(k2_table *t, uint32_t n_cand, const uint64_t *V, double k, uint32_t n_zeros,
+            uint32_t flags, double *K, double *dK, double *d2K);
+
+/* Same with device buffers on a stream; dK / d2K may be NULL when not requested. */
+int k2_eval_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double k,
+                   uint32_t n_zeros, uint32_t flags, double *K_dev, double *dK_dev,
+                   double *d2K_dev, void *stream);
+
+/* Uneven requests in one call (device buffers): requests share the table, each has its own
+ * zero cutoff, k and candidate slice.  Outputs are indexed like V (cand_offset + i). */
+int k2_eval_batched_device(k2_table *t, uint32_t n_req, const k2_request *req_host,
+                           const uint64_t *V_dev, uint64_t n_cand_total, uint32_t flags,
+                           double *K_dev, double *dK_dev, double *d2K_dev, void *stream);
+
+/* Newton polish: n_steps Newton iterations on K2' = 0 from each V (device buffers, V updated in
+ * place).  Step rule of path1_eval_fast.cpp:120-128 (|K''| > 1e-12 guard, step = K'/K''
+ * clamped to +-clamp, stop once |step| < 1e-13); the FP64 step is applied exactly to the
+ * 64.64 word.  K/dK/d2K receive the values at the final point; steps_dev the iterations taken. */
+int k2_polish_device(k2_table *t, uint32_t n_cand, uint64_t *V_dev, double k, uint32_t n_zeros,
+                     int n_steps, double clamp, uint32_t flags, double *K_dev, double *dK_dev,
+                     double *d2K_dev, int32_t *steps_dev, void *stream);
+
+/* Debug: phase words floor(G_j V_c / 2^110) mod 2^64 for zeros [z0, z0+nz) (host buffers,
+ * out[c*nz + j]).  mode 0 = evaluator path, 1 = schoolbook product, 2 = the FP32 form's top-32
+ * word in the high half. */
+int k2_phase_words(k2_table *t, uint32_t n_cand, const uint64_t *V, uint32_t z0, uint32_t nz,
+                   int mode, uint64_t *out);
+
+/* Debug: the evaluator's sin/cos of 2 pi W / 2^64 (mode 0, FP64) or of the top 32 bits
+ * (mode 1, FP32 widened); out[2i] = sin, out[2i+1] = cos.  Host buffers. */
+int k2_sincos_words(uint32_t n, const uint64_t *W, int mode, double *out);
+
+/* Timing of the evaluation kernels on the stream they run on (HIP events around each launch).
+ * enable != 0 starts collecting; k2_kernel_time returns the summed milliseconds and the launch
+ * count since the last reset. */
+int k2_profile(k2_table *t, int enable);
+int k2_kernel_time(k2_table *t, double *ms, int64_t *launches);
+
+/* ---- reference signatures (search/path1_eval_fast.cpp) -------------------------------------- */
+
+/* search/path1_eval_fast.cpp:15-31.  Same arguments and output layout ([n_peaks][n_zeros],
+ * row-major); evaluated on the GPU.  Returns void like the reference; failures are reported
+ * through k2_last_error() and leave the outputs NaN. */
+void compute_peak_AB(int n_peaks, const uint64_t *x_lo_arr, const uint64_t *x_hi_arr,
+                     const double *dy0_arr, int n_zeros, const double *alpha, const double *gamma,
+                     const uint64_t *ghi_lo, const uint64_t *ghi_hi, const uint64_t *glo_lo,
+                     const uint64_t *glo_hi, const uint64_t *psi_lo, const uint64_t *psi_hi,
+                     double *out_A, double *out_B);
+
+/* search/path1_eval_fast.cpp:58-70 */
+void polish_peaks_with_weights(int n_peaks, int n_zeros, const double *A, const double *B,
+                               const double *gamma, const double *w, double grid_half_width,
+                               int grid_steps, double *out_h_unpol, double *out_h_pol,
+                               double *out_ddy_pol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* K2EVAL_H */

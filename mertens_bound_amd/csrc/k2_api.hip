@@ -1,0 +1,574 @@
+// C ABI of libk2eval.so (declared in include/k2eval.h): table management, launch planning
+// (candidate tiles x zero splits), per-k weight caching, profiling events and the
+// reference-signature entry points.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "k2_internal.h"
+
+using namespace k2impl;
+
+#define K2_VERSION 1
+#define K2_PAD 4096            // zero arrays are padded to a multiple of this
+#define K2_MAX_WEIGHT_SETS 4   // cached per-k weight arrays per table
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(K2_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+struct ProfPair {
+    hipEvent_t a, b;
+};
+
+struct k2_table {
+    int device = 0;
+    uint32_t n = 0, n_pad = 0;
+    uint4* g = nullptr;
+    double* gamma = nullptr;
+    double* w = nullptr;
+    std::vector<WeightArrays> wsets;
+    uint64_t use_clock = 0;
+    double* partial = nullptr;
+    size_t partial_bytes = 0;
+    void* host_io = nullptr;   // device staging for host-buffer entry points
+    size_t host_io_bytes = 0;
+    int* polish_active = nullptr;
+    size_t polish_cap = 0;
+    hipStream_t stream = nullptr;
+    k2_tiling tiling{4, 2048, 8192};
+    bool profile = false;
+    std::vector<ProfPair> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+    std::mutex mu;
+};
+
+extern "C" const char* k2_last_error(void) { return g_err.c_str(); }
+extern "C" int k2_version(void) { return K2_VERSION; }
+
+static int ensure_bytes(void** p, size_t* cap, size_t need) {
+    if (*cap >= need) return K2_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t want = std::max(need, (size_t)1 << 20);
+    hipError_t e = hipMalloc(p, want);
+    if (e != hipSuccess) return fail(K2_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    *cap = want;
+    return K2_OK;
+}
+
+extern "C" int k2_table_create(k2_table** out, uint32_t n_zeros, const uint64_t* G,
+                               const double* gamma, const double* w) {
+    if (!out || !G || !gamma || !w || n_zeros == 0 || n_zeros > 0x7fff0000u)
+        return fail(K2_E_ARG, "k2_table_create: null pointer or n_zeros out of range");
+    k2_table* t = new k2_table();
+    HIPCHK(hipGetDevice(&t->device));
+    t->n = n_zeros;
+    t->n_pad = (n_zeros + K2_PAD - 1) / K2_PAD * K2_PAD;
+    const size_t np = t->n_pad;
+    hipError_t e = hipMalloc((void**)&t->g, np * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->gamma, np * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->w, np * 2 * sizeof(double));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemsetAsync(t->g, 0, np * sizeof(uint4), t->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(t->gamma, 0, np * sizeof(double), t->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(t->w, 0, np * 2 * sizeof(double), t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->g, G, (size_t)n_zeros * 16, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->gamma, gamma, (size_t)n_zeros * 8, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->w, w, (size_t)n_zeros * 16, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    if (e != hipSuccess) {
+        k2_table_destroy(t);
+        return fail(K2_E_HIP, std::string("k2_table_create: ") + hipGetErrorString(e));
+    }
+    *out = t;
+    return K2_OK;
+}
+
+extern "C" int k2_table_destroy(k2_table* t) {
+    if (!t) return K2_OK;
+    (void)hipSetDevice(t->device);
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    for (auto& ws : t->wsets) {
+        (void)hipFree(ws.a); (void)hipFree(ws.b); (void)hipFree(ws.e);
+        (void)hipFree(ws.af); (void)hipFree(ws.bf); (void)hipFree(ws.ef);
+    }
+    for (auto& pp : t->prof_pending) { (void)hipEventDestroy(pp.a); (void)hipEventDestroy(pp.b); }
+    for (auto ev : t->ev_pool) (void)hipEventDestroy(ev);
+    (void)hipFree(t->g); (void)hipFree(t->gamma); (void)hipFree(t->w);
+    (void)hipFree(t->partial); (void)hipFree(t->host_io); (void)hipFree(t->polish_active);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+    return K2_OK;
+}
+
+extern "C" uint32_t k2_table_size(const k2_table* t) { return t ? t->n : 0; }
+
+extern "C" int k2_set_tiling(k2_table* t, const k2_tiling* ti) {
+    if (!t || !ti) return fail(K2_E_ARG, "k2_set_tiling: null pointer");
+    k2_tiling v = *ti;
+    if (v.nc == 0) v.nc = 4;
+    if (v.target_blocks == 0) v.target_blocks = 2048;
+    if (v.nc != 1 && v.nc != 2 && v.nc != 4 && v.nc != 8)
+        return fail(K2_E_ARG, "k2_set_tiling: nc must be 1, 2, 4 or 8");
+    if (v.target_blocks < 1 || v.mixed_head < 0)
+        return fail(K2_E_ARG, "k2_set_tiling: target_blocks >= 1 and mixed_head >= 0 required");
+    std::lock_guard<std::mutex> lk(t->mu);
+    t->tiling = v;
+    return K2_OK;
+}
+
+extern "C" int k2_get_tiling(const k2_table* t, k2_tiling* ti) {
+    if (!t || !ti) return fail(K2_E_ARG, "k2_get_tiling: null pointer");
+    *ti = t->tiling;
+    return K2_OK;
+}
+
+// weight arrays for k, computed on the table stream on first use (LRU of K2_MAX_WEIGHT_SETS)
+static int get_weights(k2_table* t, double k, WeightArrays** out) {
+    t->use_clock++;
+    for (auto& ws : t->wsets)
+        if (ws.k == k) { ws.last_use = t->use_clock; *out = &ws; return K2_OK; }
+    WeightArrays* slot = nullptr;
+    if ((int)t->wsets.size() < K2_MAX_WEIGHT_SETS) {
+        t->wsets.emplace_back();
+        slot = &t->wsets.back();
+        const size_t np = t->n_pad;
+        hipError_t e = hipMalloc((void**)&slot->a, np * sizeof(double2));
+        if (e == hipSuccess) e = hipMalloc((void**)&slot->b, np * sizeof(double2));
+        if (e == hipSuccess) e = hipMalloc((void**)&slot->e, np * sizeof(double2));
+        if (e == hipSuccess) e = hipMalloc((void**)&slot->af, np * sizeof(float2));
+        if (e == hipSuccess) e = hipMalloc((void**)&slot->bf, np * sizeof(float2));
+        if (e == hipSuccess) e = hipMalloc((void**)&slot->ef, np * sizeof(float2));
+        if (e != hipSuccess) {
+            WeightArrays bad = *slot;
+            t->wsets.pop_back();
+            (void)hipFree(bad.a); (void)hipFree(bad.b); (void)hipFree(bad.e);
+            (void)hipFree(bad.af); (void)hipFree(bad.bf); (void)hipFree(bad.ef);
+            return fail(K2_E_NOMEM, "weight arrays: hipMalloc failed");
+        }
+    } else {
+        slot = &*std::min_element(t->wsets.begin(), t->wsets.end(),
+                                  [](const WeightArrays& x, const WeightArrays& y) {
+                                      return x.last_use < y.last_use;
+                                  });
+    }
+    slot->k = k;
+    slot->last_use = t->use_clock;
+    HIPCHK(launch_weights(t->gamma, t->w, (int)t->n, (int)t->n_pad, k, *slot, t->stream));
+    *out = slot;
+    return K2_OK;
+}
+
+static hipEvent_t pool_event(k2_table* t) {
+    if (!t->ev_pool.empty()) {
+        hipEvent_t e = t->ev_pool.back();
+        t->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct Plan {
+    int n_ctiles = 0, n_cand_pad = 0;
+    int sp_head = 0, zps_head = 0;   // FP64 part
+    int sp_tail = 0, zps_tail = 0;   // FP32 part (mixed only)
+    int head_end = 0;
+};
+
+static void split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps) {
+    if (nz <= 0) { *n_splits = 0; *zps = 0; return; }
+    const int max_sp = (nz + 255) / 256;
+    int sp = (target + n_ctiles - 1) / n_ctiles;
+    sp = std::max(1, std::min(sp, max_sp));
+    int z = (nz + sp - 1) / sp;
+    z = (z + 255) / 256 * 256;
+    *zps = z;
+    *n_splits = (nz + z - 1) / z;
+}
+
+static Plan make_plan(const k2_table* t, int n_cand, int nz, bool mixed) {
+    Plan p;
+    const int nc = t->tiling.nc;
+    p.n_ctiles = (n_cand + 256 * nc - 1) / (256 * nc);
+    p.n_cand_pad = p.n_ctiles * 256 * nc;
+    p.head_end = mixed ? std::min(nz, t->tiling.mixed_head) : nz;
+    const int tb = t->tiling.target_blocks;
+    if (mixed) {
+        // share the workgroup budget in proportion to the zero counts
+        const int nt = nz - p.head_end;
+        const int tb_head = std::max(1, (int)((double)tb * p.head_end / std::max(1, nz)));
+        split_range(p.head_end, p.n_ctiles, tb_head, &p.sp_head, &p.zps_head);
+        split_range(nt, p.n_ctiles, std::max(1, tb - tb_head), &p.sp_tail, &p.zps_tail);
+    } else {
+        split_range(nz, p.n_ctiles, tb, &p.sp_head, &p.zps_head);
+    }
+    return p;
+}
+
+// evaluation of one candidate list over zeros [0, nz) with weights ws; outputs on device
+static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* ws, int nz,
+                     unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
+    const bool mixed = (flags & K2_MIXED) != 0;
+    Plan p = make_plan(t, n_cand, nz, mixed);
+    const int n_rows = std::max(1, p.sp_head + p.sp_tail);
+    int rc = ensure_bytes((void**)&t->partial, &t->partial_bytes,
+                          (size_t)n_rows * 3 * p.n_cand_pad * sizeof(double));
+    if (rc) return rc;
+    LaunchArgs a;
+    a.nc = t->tiling.nc;
+    a.flags = flags;
+    a.zg = t->g;
+    a.za = ws->a; a.zb = ws->b; a.ze = ws->e;
+    a.zaf = ws->af; a.zbf = ws->bf; a.zef = ws->ef;
+    a.cand = cand;
+    a.n_cand = n_cand;
+    a.n_ctiles = p.n_ctiles;
+    a.partial = t->partial;
+    a.n_cand_pad = p.n_cand_pad;
+    ProfPair pp{nullptr, nullptr};
+    if (t->profile) {
+        pp.a = pool_event(t);
+        pp.b = pool_event(t);
+        HIPCHK(hipEventRecord(pp.a, st));
+    }
+    if (p.sp_head > 0) {
+        a.z_begin = 0; a.z_end = p.head_end; a.zeros_per_split = p.zps_head;
+        a.split_base = 0;
+        a.n_blocks = p.n_ctiles * p.sp_head;
+        HIPCHK(launch_eval(a, false, st));
+    }
+    if (p.sp_tail > 0) {
+        a.z_begin = p.head_end; a.z_end = nz; a.zeros_per_split = p.zps_tail;
+        a.split_base = p.sp_head;
+        a.n_blocks = p.n_ctiles * p.sp_tail;
+        HIPCHK(launch_eval(a, true, st));
+    }
+    if (t->profile) {
+        HIPCHK(hipEventRecord(pp.b, st));
+        t->prof_pending.push_back(pp);
+    }
+    if (p.sp_head + p.sp_tail == 0) {
+        // empty zero range: the sums are zero
+        HIPCHK(hipMemsetAsync(t->partial, 0, (size_t)3 * p.n_cand_pad * sizeof(double), st));
+    }
+    HIPCHK(launch_reduce(t->partial, n_rows, n_cand, p.n_cand_pad, flags, K, dK, d2K, st));
+    return K2_OK;
+}
+
+static int check_eval_args(k2_table* t, uint32_t n_cand, const void* V, uint32_t* n_zeros,
+                           const double* K, const char* who) {
+    if (!t) return fail(K2_E_ARG, std::string(who) + ": null table");
+    if (n_cand > 0 && (!V || !K)) return fail(K2_E_ARG, std::string(who) + ": null buffer");
+    if (n_cand > 0x40000000u) return fail(K2_E_ARG, std::string(who) + ": n_cand too large");
+    if (*n_zeros == 0) *n_zeros = t->n;
+    if (*n_zeros > t->n) return fail(K2_E_ARG, std::string(who) + ": n_zeros exceeds table size");
+    return K2_OK;
+}
+
+extern "C" int k2_eval_device(k2_table* t, uint32_t n_cand, const uint64_t* V_dev, double k,
+                              uint32_t n_zeros, uint32_t flags, double* K_dev, double* dK_dev,
+                              double* d2K_dev, void* stream) {
+    int rc = check_eval_args(t, n_cand, V_dev, &n_zeros, K_dev, "k2_eval_device");
+    if (rc) return rc;
+    if (((uintptr_t)V_dev & 15) != 0) return fail(K2_E_ARG, "k2_eval_device: V not 16-byte aligned");
+    if (((flags & K2_D1) && !dK_dev) || ((flags & K2_D2) && !d2K_dev))
+        return fail(K2_E_ARG, "k2_eval_device: derivative requested without an output buffer");
+    if (!(k >= 0.0) || !isfinite(k)) return fail(K2_E_ARG, "k2_eval_device: k must be finite and >= 0");
+    if (n_cand == 0) return K2_OK;
+    std::lock_guard<std::mutex> lk(t->mu);
+    HIPCHK(hipSetDevice(t->device));
+    hipStream_t st = stream ? (hipStream_t)stream : t->stream;
+    WeightArrays* ws = nullptr;
+    if (st != t->stream) {
+        // weights are produced on the table stream; order the caller's stream after them
+        rc = get_weights(t, k, &ws);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(t->stream));
+    } else {
+        rc = get_weights(t, k, &ws);
+        if (rc) return rc;
+    }
+    return eval_core(t, (int)n_cand, (const uint4*)V_dev, ws, (int)n_zeros, flags, K_dev, dK_dev,
+                     d2K_dev, st);
+}
+
+extern "C" int k2_eval(k2_table* t, uint32_t n_cand, const uint64_t* V, double k, uint32_t n_zeros,
+                       uint32_t flags, double* K, double* dK, double* d2K) {
+    int rc = check_eval_args(t, n_cand, V, &n_zeros, K, "k2_eval");
+    if (rc) return rc;
+    if (((flags & K2_D1) && !dK) || ((flags & K2_D2) && !d2K))
+        return fail(K2_E_ARG, "k2_eval: derivative requested without an output buffer");
+    if (!(k >= 0.0) || !isfinite(k)) return fail(K2_E_ARG, "k2_eval: k must be finite and >= 0");
+    if (n_cand == 0) return K2_OK;
+    std::lock_guard<std::mutex> lk(t->mu);
+    HIPCHK(hipSetDevice(t->device));
+    const size_t nb_v = (size_t)n_cand * 16, nb_o = (size_t)n_cand * 8;
+    rc = ensure_bytes(&t->host_io, &t->host_io_bytes, nb_v + 3 * nb_o + 64);
+    if (rc) return rc;
+    char* base = (char*)t->host_io;
+    uint4* dV = (uint4*)base;
+    double* dK0 = (double*)(base + nb_v);
+    double* dK1 = dK0 + n_cand;
+    double* dK2 = dK1 + n_cand;
+    HIPCHK(hipMemcpyAsync(dV, V, nb_v, hipMemcpyHostToDevice, t->stream));
+    WeightArrays* ws = nullptr;
+    rc = get_weights(t, k, &ws);
+    if (rc) return rc;
+    rc = eval_core(t, (int)n_cand, dV, ws, (int)n_zeros, flags, dK0, dK1, dK2, t->stream);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(K, dK0, nb_o, hipMemcpyDeviceToHost, t->stream));
+    if (flags & K2_D1) HIPCHK(hipMemcpyAsync(dK, dK1, nb_o, hipMemcpyDeviceToHost, t->stream));
+    if (flags & K2_D2) HIPCHK(hipMemcpyAsync(d2K, dK2, nb_o, hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    return K2_OK;
+}
+
+extern "C" int k2_eval_batched_device(k2_table* t, uint32_t n_req, const k2_request* req,
+                                      const uint64_t* V_dev, uint64_t n_cand_total, uint32_t flags,
+                                      double* K_dev, double* dK_dev, double* d2K_dev, void* stream) {
+    if (!t || (n_req > 0 && (!req || !V_dev || !K_dev)))
+        return fail(K2_E_ARG, "k2_eval_batched_device: null pointer");
+    for (uint32_t r = 0; r < n_req; ++r) {
+        if (req[r].n_zeros > t->n || req[r].cand_offset + req[r].n_cand > n_cand_total)
+            return fail(K2_E_ARG, "k2_eval_batched_device: request " + std::to_string(r) +
+                                      " out of range");
+    }
+    for (uint32_t r = 0; r < n_req; ++r) {
+        if (req[r].n_cand == 0) continue;
+        const uint64_t o = req[r].cand_offset;
+        int rc = k2_eval_device(t, req[r].n_cand, V_dev + 2 * o, req[r].k,
+                                req[r].n_zeros ? req[r].n_zeros : t->n, flags, K_dev + o,
+                                dK_dev ? dK_dev + o : nullptr, d2K_dev ? d2K_dev + o : nullptr,
+                                stream);
+        if (rc) return rc;
+    }
+    return K2_OK;
+}
+
+extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, double k,
+                                uint32_t n_zeros, int n_steps, double clamp, uint32_t flags,
+                                double* K_dev, double* dK_dev, double* d2K_dev, int32_t* steps_dev,
+                                void* stream) {
+    if (!t || (n_cand > 0 && (!V_dev || !K_dev || !dK_dev || !d2K_dev || !steps_dev)))
+        return fail(K2_E_ARG, "k2_polish_device: null pointer");
+    if (n_steps < 0 || !(clamp > 0.0) || clamp > 0.25)
+        return fail(K2_E_ARG, "k2_polish_device: n_steps >= 0 and 0 < clamp <= 0.25 required");
+    if (n_cand == 0) return K2_OK;
+    hipStream_t st = stream ? (hipStream_t)stream : t->stream;
+    const unsigned fl = (flags & K2_MIXED) | K2_VALUE | K2_D1 | K2_D2;
+    {
+        std::lock_guard<std::mutex> lk(t->mu);
+        HIPCHK(hipSetDevice(t->device));
+        int rc = ensure_bytes((void**)&t->polish_active, &t->polish_cap, (size_t)n_cand * 4);
+        if (rc) return rc;
+        std::vector<int> ones(n_cand, 1);
+        HIPCHK(hipMemcpyAsync(t->polish_active, ones.data(), (size_t)n_cand * 4,
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(steps_dev, 0, (size_t)n_cand * 4, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    for (int it = 0; it < n_steps; ++it) {
+        int rc = k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
+        if (rc) return rc;
+        std::lock_guard<std::mutex> lk(t->mu);
+        HIPCHK(launch_newton((uint4*)V_dev, dK_dev, d2K_dev, (int)n_cand, clamp, t->polish_active,
+                             steps_dev, st));
+    }
+    return k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
+}
+
+extern "C" int k2_phase_words(k2_table* t, uint32_t n_cand, const uint64_t* V, uint32_t z0,
+                              uint32_t nz, int mode, uint64_t* out) {
+    if (!t || (n_cand && nz && (!V || !out))) return fail(K2_E_ARG, "k2_phase_words: null pointer");
+    if ((uint64_t)z0 + nz > t->n) return fail(K2_E_ARG, "k2_phase_words: zero range out of table");
+    if (mode < 0 || mode > 2) return fail(K2_E_ARG, "k2_phase_words: mode must be 0, 1 or 2");
+    if ((uint64_t)n_cand * nz > ((uint64_t)1 << 31))
+        return fail(K2_E_ARG, "k2_phase_words: n_cand * nz too large");
+    if (n_cand == 0 || nz == 0) return K2_OK;
+    std::lock_guard<std::mutex> lk(t->mu);
+    HIPCHK(hipSetDevice(t->device));
+    const size_t nb_v = (size_t)n_cand * 16, nb_o = (size_t)n_cand * nz * 8;
+    int rc = ensure_bytes(&t->host_io, &t->host_io_bytes, nb_v + nb_o + 64);
+    if (rc) return rc;
+    uint4* dV = (uint4*)t->host_io;
+    uint64_t* dO = (uint64_t*)((char*)t->host_io + nb_v);
+    HIPCHK(hipMemcpyAsync(dV, V, nb_v, hipMemcpyHostToDevice, t->stream));
+    HIPCHK(launch_phase_words(t->g, (int)z0, (int)nz, dV, (int)n_cand, mode, dO, t->stream));
+    HIPCHK(hipMemcpyAsync(out, dO, nb_o, hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    return K2_OK;
+}
+
+extern "C" int k2_sincos_words(uint32_t n, const uint64_t* W, int mode, double* out) {
+    if (n && (!W || !out)) return fail(K2_E_ARG, "k2_sincos_words: null pointer");
+    if (n == 0) return K2_OK;
+    if (n > (1u << 28)) return fail(K2_E_ARG, "k2_sincos_words: n too large");
+    uint64_t* dW = nullptr;
+    double* dO = nullptr;
+    HIPCHK(hipMalloc((void**)&dW, (size_t)n * 8));
+    hipError_t e = hipMalloc((void**)&dO, (size_t)n * 16);
+    if (e == hipSuccess) e = hipMemcpy(dW, W, (size_t)n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_sincos_words(dW, (int)n, mode, dO, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, dO, (size_t)n * 16, hipMemcpyDeviceToHost);
+    (void)hipFree(dW);
+    (void)hipFree(dO);
+    if (e != hipSuccess) return fail(K2_E_HIP, std::string("k2_sincos_words: ") + hipGetErrorString(e));
+    return K2_OK;
+}
+
+extern "C" int k2_profile(k2_table* t, int enable) {
+    if (!t) return fail(K2_E_ARG, "k2_profile: null table");
+    std::lock_guard<std::mutex> lk(t->mu);
+    HIPCHK(hipSetDevice(t->device));
+    for (auto& pp : t->prof_pending) {
+        HIPCHK(hipEventSynchronize(pp.b));
+        t->ev_pool.push_back(pp.a);
+        t->ev_pool.push_back(pp.b);
+    }
+    t->prof_pending.clear();
+    t->prof_ms = 0.0;
+    t->prof_launches = 0;
+    t->profile = enable != 0;
+    return K2_OK;
+}
+
+extern "C" int k2_kernel_time(k2_table* t, double* ms, int64_t* launches) {
+    if (!t || !ms || !launches) return fail(K2_E_ARG, "k2_kernel_time: null pointer");
+    std::lock_guard<std::mutex> lk(t->mu);
+    HIPCHK(hipSetDevice(t->device));
+    for (auto& pp : t->prof_pending) {
+        HIPCHK(hipEventSynchronize(pp.b));
+        float e = 0.f;
+        HIPCHK(hipEventElapsedTime(&e, pp.a, pp.b));
+        t->prof_ms += e;
+        t->prof_launches += 1;
+        t->ev_pool.push_back(pp.a);
+        t->ev_pool.push_back(pp.b);
+    }
+    t->prof_pending.clear();
+    *ms = t->prof_ms;
+    *launches = t->prof_launches;
+    return K2_OK;
+}
+
+// ---- reference signatures ---------------------------------------------------------------------
+
+static void fill_nan(double* p, size_t n) {
+    if (!p) return;
+    for (size_t i = 0; i < n; ++i) p[i] = NAN;
+}
+
+extern "C" void compute_peak_AB(int n_peaks, const uint64_t* x_lo_arr, const uint64_t* x_hi_arr,
+                                const double* dy0_arr, int n_zeros, const double* alpha,
+                                const double* gamma, const uint64_t* ghi_lo, const uint64_t* ghi_hi,
+                                const uint64_t* glo_lo, const uint64_t* glo_hi,
+                                const uint64_t* psi_lo, const uint64_t* psi_hi, double* out_A,
+                                double* out_B) {
+    if (n_peaks <= 0 || n_zeros <= 0) return;
+    const size_t np = (size_t)n_peaks, nz = (size_t)n_zeros, nab = np * nz;
+    if (n_peaks > 65535) {
+        fail(K2_E_ARG, "compute_peak_AB: n_peaks > 65535");
+        fill_nan(out_A, nab); fill_nan(out_B, nab);
+        return;
+    }
+    // device layout: x_lo x_hi (np) | 6 zero words (nz each) | dy0 (np) | alpha gamma (nz) | A B
+    const size_t bytes = np * 8 * 3 + nz * 8 * 8 + nab * 16;
+    char* d = nullptr;
+    hipStream_t st = nullptr;
+    hipError_t e = hipMalloc((void**)&d, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        uint64_t* xl = (uint64_t*)d;
+        uint64_t* xh = xl + np;
+        uint64_t* zw = xh + np;
+        double* ddy = (double*)(zw + 6 * nz);
+        double* dal = ddy + np;
+        double* dga = dal + nz;
+        double* dA = dga + nz;
+        double* dB = dA + nab;
+        const uint64_t* hs[6] = {ghi_lo, ghi_hi, glo_lo, glo_hi, psi_lo, psi_hi};
+        e = hipMemcpyAsync(xl, x_lo_arr, np * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(xh, x_hi_arr, np * 8, hipMemcpyHostToDevice, st);
+        for (int q = 0; q < 6 && e == hipSuccess; ++q)
+            e = hipMemcpyAsync(zw + q * nz, hs[q], nz * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(ddy, dy0_arr, np * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dal, alpha, nz * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dga, gamma, nz * 8, hipMemcpyHostToDevice, st);
+        const uint64_t* ptrs[8] = {xl, xh, zw, zw + nz, zw + 2 * nz, zw + 3 * nz, zw + 4 * nz,
+                                   zw + 5 * nz};
+        if (e == hipSuccess) e = launch_ref_peak_ab(n_peaks, ptrs, ddy, n_zeros, dal, dga, dA, dB, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(out_A, dA, nab * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(out_B, dB, nab * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        fail(K2_E_HIP, std::string("compute_peak_AB: ") + hipGetErrorString(e));
+        fill_nan(out_A, nab); fill_nan(out_B, nab);
+    }
+}
+
+extern "C" void polish_peaks_with_weights(int n_peaks, int n_zeros, const double* A,
+                                          const double* B, const double* gamma, const double* w,
+                                          double grid_half_width, int grid_steps,
+                                          double* out_h_unpol, double* out_h_pol,
+                                          double* out_ddy_pol) {
+    if (n_peaks <= 0) return;
+    const size_t np = (size_t)n_peaks, nz = (size_t)std::max(n_zeros, 0), nab = np * nz;
+    const size_t bytes = nab * 16 + nz * 16 + np * 24 + 64;
+    char* d = nullptr;
+    hipStream_t st = nullptr;
+    hipError_t e = hipMalloc((void**)&d, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        double* dA = (double*)d;
+        double* dB = dA + nab;
+        double* dg = dB + nab;
+        double* dw = dg + nz;
+        double* o = dw + nz;
+        if (nab) e = hipMemcpyAsync(dA, A, nab * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && nab) e = hipMemcpyAsync(dB, B, nab * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && nz) e = hipMemcpyAsync(dg, gamma, nz * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && nz) e = hipMemcpyAsync(dw, w, nz * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = launch_ref_polish(n_peaks, (int)nz, dA, dB, dg, dw, grid_half_width, grid_steps, o,
+                                  o + np, o + 2 * np, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(out_h_unpol, o, np * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(out_h_pol, o + np, np * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out_ddy_pol, o + 2 * np, np * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        fail(K2_E_HIP, std::string("polish_peaks_with_weights: ") + hipGetErrorString(e));
+        fill_nan(out_h_unpol, np); fill_nan(out_h_pol, np); fill_nan(out_ddy_pol, np);
+    }
+}

@@ -1,0 +1,213 @@
+// Device-side arithmetic shared by the K2 kernels (gfx950 / CDNA4).
+//
+// Phase: W = floor(G * V / 2^110) mod 2^64 for G (gamma/2pi, Q18.110) and V (v, 64.64), both
+// 128-bit words split into 32-bit limbs g0..g3 / v0..v3.  The product is formed with 32x32->64
+// multiply-adds on the VALU (v_mad_u64_u32).  Limb products in columns 0 and 1 (bit weights
+// 2^0 and 2^32) sum to less than 2^98 and can only move W by a carry into bit 110, which needs
+// bits [98,110) of the rest of the product to be all ones; that case (probability 2^-12) is
+// detected and recomputed exactly, so W is bit-exact for every input.
+//
+// sincos: the phase is in turns, so the quadrant comes from the top two bits of W and the
+// reduced argument |x| <= pi/4 from the remaining 62 bits, with no floating-point range
+// reduction.  sin and cos are Taylor polynomials on [-pi/4, pi/4] (truncation error < 2e-18).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace k2dev {
+
+struct Limbs4 { uint32_t x0, x1, x2, x3; };
+
+__device__ __forceinline__ uint64_t mul32(uint32_t a, uint32_t b) { return (uint64_t)a * b; }
+
+// (a mod 2^24)(b mod 2^24) + c mod 2^32 (v_mad_u32_u24, full rate).  Written as asm: the
+// compiler otherwise drops the operand masks when only low result bits are used and falls back
+// to the quarter-rate v_mul_lo_u32.
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// a*b + c over 64 bits with the carry out of bit 64 added to k (v_mad_u64_u32's carry-out,
+// then v_addc_co_u32).  The s_nop gives the VCC write two wait states before it is read.
+__device__ __forceinline__ uint64_t mad_carry(uint32_t a, uint32_t b, uint64_t c, uint32_t& k) {
+    uint64_t r;
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "=&v"(r), "+v"(k)
+        : "v"(a), "v"(b), "v"(c)
+        : "vcc");
+    return r;
+}
+
+// exact W via a straightforward 256-bit schoolbook product (rare fix-up path and debug use)
+__device__ __noinline__ uint64_t phase_word_exact(Limbs4 g, Limbs4 v) {
+    const uint32_t G[4] = {g.x0, g.x1, g.x2, g.x3};
+    const uint32_t V[4] = {v.x0, v.x1, v.x2, v.x3};
+    uint32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; ++i) {
+        uint64_t carry = 0;
+        for (int j = 0; j < 4; ++j) {
+            uint64_t t = mul32(G[i], V[j]) + r[i + j] + carry;
+            r[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        r[i + 4] = (uint32_t)carry;
+    }
+    // bits [110,174): r[3] >> 14 | r[4] << 18 | r[5] << 50
+    uint64_t lo = ((uint64_t)r[4] << 18) | (r[3] >> 14);
+    uint64_t hi = (uint64_t)r[5] << 50;
+    return lo | hi;
+}
+
+// W with columns 0-1 skipped and the carry case fixed up exactly
+__device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
+    // column 2 (weight 2^64): g0 v2 + g1 v1 + g2 v0; a carry out of a 64-bit add shows as the
+    // new sum being below the old one
+    uint32_t k2 = 0;
+    uint64_t t = mul32(g.x0, v.x2);
+    t = mad_carry(g.x1, v.x1, t, k2);
+    t = mad_carry(g.x2, v.x0, t, k2);
+    // column 3 (2^96): carry-in hi32(t) + k2 2^32
+    uint64_t s3 = (t >> 32) | ((uint64_t)k2 << 32);
+    uint32_t k3 = 0;
+    s3 = mad_carry(g.x0, v.x3, s3, k3);
+    s3 = mad_carry(g.x1, v.x2, s3, k3);
+    s3 = mad_carry(g.x2, v.x1, s3, k3);
+    s3 = mad_carry(g.x3, v.x0, s3, k3);
+    // column 4 (2^128): only bits [0,46) are needed, so wrap mod 2^64
+    uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
+    s4 = mul32(g.x1, v.x3) + s4;
+    s4 = mul32(g.x2, v.x2) + s4;
+    s4 = mul32(g.x3, v.x1) + s4;
+    // column 5 (2^160): only bits [0,14) are needed; 24x24-bit products are exact mod 2^24
+    uint32_t s5 = (uint32_t)(s4 >> 32);
+    s5 = mad24(g.x2, v.x3, s5);
+    s5 = mad24(g.x3, v.x2, s5);
+    const uint32_t s3lo = (uint32_t)s3, s4lo = (uint32_t)s4;
+    uint32_t wlo = __builtin_amdgcn_alignbit(s4lo, s3lo, 14);
+    uint32_t whi = __builtin_amdgcn_alignbit(s5, s4lo, 14);
+    uint64_t W = ((uint64_t)whi << 32) | wlo;
+    if (__builtin_expect(((s3lo >> 2) & 0xFFFu) == 0xFFFu, 0)) W = phase_word_exact(g, v);
+    return W;
+}
+
+// top-32-bit phase word floor(G V / 2^142) mod 2^32 (for the FP32 form), exact.
+// Columns <= 2 sum to < 2^130 and only matter through a carry into bit 142, which needs bits
+// [130,142) of the rest to be all ones.
+__device__ __forceinline__ uint32_t phase_word_hi32(Limbs4 g, Limbs4 v) {
+    uint64_t s3 = mul32(g.x0, v.x3);
+    uint32_t k3 = 0;
+    s3 = mad_carry(g.x1, v.x2, s3, k3);
+    s3 = mad_carry(g.x2, v.x1, s3, k3);
+    s3 = mad_carry(g.x3, v.x0, s3, k3);
+    uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
+    s4 = mul32(g.x1, v.x3) + s4;
+    s4 = mul32(g.x2, v.x2) + s4;
+    s4 = mul32(g.x3, v.x1) + s4;
+    uint32_t s5 = (uint32_t)(s4 >> 32);
+    s5 = mad24(g.x2, v.x3, s5);
+    s5 = mad24(g.x3, v.x2, s5);
+    const uint32_t s4lo = (uint32_t)s4;
+    uint32_t whi = __builtin_amdgcn_alignbit(s5, s4lo, 14);
+    // bits [130,142) of the partial sum are bits [2,14) of s4lo
+    if (__builtin_expect(((s4lo >> 2) & 0xFFFu) == 0xFFFu, 0))
+        whi = (uint32_t)(phase_word_exact(g, v) >> 32);
+    return whi;
+}
+
+// ---- FP64 sin/cos of 2 pi W / 2^64 ---------------------------------------------------------
+#define K2_TWO_PI_2M64 3.4061215800865545e-19   // 2 pi / 2^64 (correctly rounded)
+
+__device__ __forceinline__ void sincos_poly(double x, double& s, double& c) {
+    const double z = x * x;
+    // sin x = x + x z P(z), P = sum (-1)^n z^(n-1)/(2n+1)!, n = 1..8
+    double ps = 2.8114572543455206e-15;              //  1/17!
+    ps = fma(ps, z, -7.6471637318198164e-13);        // -1/15!
+    ps = fma(ps, z, 1.6059043836821613e-10);         //  1/13!
+    ps = fma(ps, z, -2.5052108385441720e-08);        // -1/11!
+    ps = fma(ps, z, 2.7557319223985893e-06);         //  1/9!
+    ps = fma(ps, z, -1.9841269841269841e-04);        // -1/7!
+    ps = fma(ps, z, 8.3333333333333333e-03);         //  1/5!
+    ps = fma(ps, z, -1.6666666666666666e-01);        // -1/3!
+    s = fma(x * z, ps, x);
+    // cos x = 1 + z Q(z), Q = sum (-1)^n z^(n-1)/(2n)!, n = 1..8
+    double pc = 4.7794773323873853e-14;              //  1/16!
+    pc = fma(pc, z, -1.1470745597729725e-11);        // -1/14!
+    pc = fma(pc, z, 2.0876756987868099e-09);         //  1/12!
+    pc = fma(pc, z, -2.7557319223985888e-07);        // -1/10!
+    pc = fma(pc, z, 2.4801587301587302e-05);         //  1/8!
+    pc = fma(pc, z, -1.3888888888888889e-03);        // -1/6!
+    pc = fma(pc, z, 4.1666666666666664e-02);         //  1/4!
+    pc = fma(pc, z, -0.5);
+    c = fma(z, pc, 1.0);
+}
+
+// sin and cos of the angle 2 pi W / 2^64
+__device__ __forceinline__ void sincos_turns64(uint64_t W, double& s, double& c) {
+    const uint32_t wlo = (uint32_t)W, whi = (uint32_t)(W >> 32);
+    const uint32_t q = (whi + 0x20000000u) >> 30;                  // nearest quarter turn
+    const int32_t rhi = (int32_t)(whi - (q << 30));                // r in [-2^61, 2^61)
+    const double r = fma((double)rhi, 4294967296.0, (double)wlo);
+    const double x = r * K2_TWO_PI_2M64;
+    double sp, cp;
+    sincos_poly(x, sp, cp);
+    // e^{i(q pi/2 + x)} = i^q (cp + i sp)
+    const double cs = (q & 1) ? -sp : cp;
+    const double sn = (q & 1) ? cp : sp;
+    const bool neg = (q & 2) != 0;
+    c = neg ? -cs : cs;
+    s = neg ? -sn : sn;
+}
+
+// ---- FP32 sin/cos of 2 pi W32 / 2^32 (precision form) ---------------------------------------
+__device__ __forceinline__ void sincos_turns32(uint32_t W, float& s, float& c) {
+    const uint32_t q = (W + 0x20000000u) >> 30;
+    const int32_t r = (int32_t)(W - (q << 30));                    // in [-2^29, 2^29)
+    const float x = (float)r * 1.4629181e-09f;            // 2 pi / 2^32
+    const float z = x * x;
+    // Taylor on [-pi/4, pi/4]: sin to x^11 (trunc < 3e-10), cos to x^10 (trunc < 2e-9 rel 1)
+    float ps = -2.5052108e-08f;
+    ps = fmaf(ps, z, 2.7557319e-06f);
+    ps = fmaf(ps, z, -1.9841270e-04f);
+    ps = fmaf(ps, z, 8.3333333e-03f);
+    ps = fmaf(ps, z, -1.6666667e-01f);
+    const float sp = fmaf(x * z, ps, x);
+    float pc = -2.7557319e-07f;
+    pc = fmaf(pc, z, 2.4801587e-05f);
+    pc = fmaf(pc, z, -1.3888889e-03f);
+    pc = fmaf(pc, z, 4.1666668e-02f);
+    pc = fmaf(pc, z, -0.5f);
+    const float cp = fmaf(z, pc, 1.0f);
+    const float cs = (q & 1) ? -sp : cp;
+    const float sn = (q & 1) ? cp : sp;
+    const bool neg = (q & 2) != 0;
+    c = neg ? -cs : cs;
+    s = neg ? -sn : sn;
+}
+
+// ---- general FP64 sincos (|x| < 2^20 pi/2) for the reference-format entry points -----------
+// Cody-Waite reduction by pi/2 with a three-part constant (W. J. Cody and W. Waite,
+// "Software Manual for the Elementary Functions", 1980).  PIO2_1 and PIO2_2 carry 33 bits each,
+// so n * PIO2_1 and n * PIO2_2 are exact for |n| < 2^20.
+__device__ __forceinline__ void sincos_general(double x, double& s, double& c) {
+    const double PIO2_1 = 1.5707963267341256e+00;   // first 33 bits of pi/2
+    const double PIO2_2 = 6.0771005063039660e-11;   // next 33 bits
+    const double PIO2_3 = 2.0222662487959506e-21;   // remainder
+    const double n = rint(x * 6.3661977236758138e-01);
+    double r = fma(-n, PIO2_1, x);
+    r = fma(-n, PIO2_2, r);
+    r = fma(-n, PIO2_3, r);
+    const int q = ((int)(long long)n) & 3;
+    double sp, cp;
+    sincos_poly(r, sp, cp);
+    const double cs = (q & 1) ? -sp : cp;
+    const double sn = (q & 1) ? cp : sp;
+    const bool neg = (q & 2) != 0;
+    c = neg ? -cs : cs;
+    s = neg ? -sn : sn;
+}
+
+}  // namespace k2dev

@@ -1,0 +1,443 @@
+// HIP kernels of the K2 evaluator for gfx950 (MI355X, CDNA4, wave64).
+//
+// K2(v) = 2 sum_j e^{-k g_j^2} Re(w_j e^{i g_j v}) and its first two v-derivatives, for a list of
+// candidates v against a zero table.  This is a streamed trig sum (no contraction), so it runs on
+// the VALU: exact integer phase (k2_device.h), FP64 (or FP32) sin/cos, FP64 accumulation.
+//
+// Work decomposition of the evaluation kernels:
+//   * a workgroup (256 lanes = 4 waves) owns a tile of 256*NC candidates, NC per lane in VGPRs;
+//   * it walks one contiguous zero range ("split") in tiles of 256 zeros staged in LDS
+//     (double-buffered, one barrier per tile); every lane reads the same zero record, an LDS
+//     broadcast, so one staged zero feeds 256*NC candidate evaluations;
+//   * each workgroup writes its per-candidate partial sums to partial[split][3][n_cand_pad];
+//     k2_reduce_kernel adds the splits in a fixed order, so results are deterministic;
+//   * blockIdx -> (split, candidate tile) is XCD-aware: workgroups are dealt round-robin to the 8
+//     XCDs, and consecutive workgroups of one XCD get the same split, so a zero range is pulled
+//     into one XCD's L2 once and re-read from there.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "k2_device.h"
+#include "k2_internal.h"
+
+using namespace k2dev;
+
+#define K2_ZT 256   // zeros per LDS tile = threads per workgroup
+
+__device__ __forceinline__ int xcd_linear_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    return (nb % 8 == 0) ? ((b & 7) * (nb >> 3) + (b >> 3)) : b;
+}
+
+// ---- per-k weighted zero arrays ---------------------------------------------------------------
+// a = 2 e^{-k g^2} w, b = g a, e = g^2 a (FP64) and the same in FP32; padded entries are zero.
+__global__ __launch_bounds__(256) void k2_weights_kernel(
+    const double* __restrict__ gamma, const double2* __restrict__ w, int n, int n_pad, double k,
+    double2* __restrict__ a, double2* __restrict__ b, double2* __restrict__ e,
+    float2* __restrict__ af, float2* __restrict__ bf, float2* __restrict__ ef) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_pad) return;
+    double2 va = make_double2(0.0, 0.0), vb = va, ve = va;
+    if (j < n) {
+        const double g = gamma[j];
+        const double f = 2.0 * exp(-k * g * g);
+        const double2 ww = w[j];
+        va = make_double2(f * ww.x, f * ww.y);
+        vb = make_double2(g * va.x, g * va.y);
+        ve = make_double2(g * vb.x, g * vb.y);
+    }
+    a[j] = va; b[j] = vb; e[j] = ve;
+    af[j] = make_float2((float)va.x, (float)va.y);
+    bf[j] = make_float2((float)vb.x, (float)vb.y);
+    ef[j] = make_float2((float)ve.x, (float)ve.y);
+}
+
+// ---- plain FP64 evaluation ---------------------------------------------------------------------
+template <int NC, int FL>
+__global__ __launch_bounds__(256) void k2_plain_kernel(
+    const uint4* __restrict__ zg, const double2* __restrict__ za, const double2* __restrict__ zb,
+    const double2* __restrict__ ze, int z_begin, int z_end, int zeros_per_split,
+    const uint4* __restrict__ cand, int n_cand, int n_ctiles,
+    double* __restrict__ partial, int split_base, int n_cand_pad) {
+    constexpr bool D1 = (FL & K2_D1) != 0;
+    constexpr bool D2 = (FL & K2_D2) != 0;
+    __shared__ uint4 s_g[2][K2_ZT];
+    __shared__ double2 s_a[2][K2_ZT];
+    __shared__ double2 s_b[2][D1 ? K2_ZT : 1];
+    __shared__ double2 s_e[2][D2 ? K2_ZT : 1];
+
+    const int L = xcd_linear_block();
+    const int split = L / n_ctiles, ct = L - split * n_ctiles;
+    const int z0 = z_begin + split * zeros_per_split;
+    const int z1 = min(z_end, z0 + zeros_per_split);
+    const int tid = threadIdx.x;
+
+    Limbs4 v[NC];
+    double acc0[NC], acc1[NC], acc2[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+        uint4 cv = make_uint4(0, 0, 0, 0);
+        if (c < n_cand) cv = cand[c];
+        v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
+        acc0[i] = 0.0; acc1[i] = 0.0; acc2[i] = 0.0;
+    }
+
+    const int ntiles = z1 > z0 ? (z1 - z0 + K2_ZT - 1) / K2_ZT : 0;
+    uint4 rg = make_uint4(0, 0, 0, 0);
+    double2 ra = make_double2(0.0, 0.0), rb = ra, re = ra;
+    auto fetch = [&](int t) {
+        const int j = z0 + t * K2_ZT + tid;
+        if (j < z1) {
+            rg = zg[j]; ra = za[j];
+            if (D1) rb = zb[j];
+            if (D2) re = ze[j];
+        } else {
+            rg = make_uint4(0, 0, 0, 0); ra = make_double2(0.0, 0.0); rb = ra; re = ra;
+        }
+    };
+    auto stage = [&](int buf) {
+        s_g[buf][tid] = rg; s_a[buf][tid] = ra;
+        if (D1) s_b[buf][tid] = rb;
+        if (D2) s_e[buf][tid] = re;
+    };
+    if (ntiles > 0) { fetch(0); stage(0); }
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) fetch(t + 1);
+#pragma unroll 2
+        for (int z = 0; z < K2_ZT; ++z) {
+            const uint4 gg = s_g[buf][z];
+            const double2 aa = s_a[buf][z];
+            double2 bb, ee;
+            if (D1) bb = s_b[buf][z];
+            if (D2) ee = s_e[buf][z];
+            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const uint64_t W = phase_word(g, v[i]);
+                double s, c;
+                sincos_turns64(W, s, c);
+                acc0[i] = fma(aa.x, c, acc0[i]);
+                acc0[i] = fma(-aa.y, s, acc0[i]);
+                if (D1) {
+                    acc1[i] = fma(-bb.x, s, acc1[i]);
+                    acc1[i] = fma(-bb.y, c, acc1[i]);
+                }
+                if (D2) {
+                    acc2[i] = fma(-ee.x, c, acc2[i]);
+                    acc2[i] = fma(ee.y, s, acc2[i]);
+                }
+            }
+        }
+        if (t + 1 < ntiles) stage(buf ^ 1);
+        __syncthreads();
+    }
+    double* prow = partial + (size_t)(split_base + split) * 3 * n_cand_pad;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+        if (c < n_cand_pad) {
+            prow[c] = acc0[i];
+            if (D1) prow[n_cand_pad + c] = acc1[i];
+            if (D2) prow[2 * n_cand_pad + c] = acc2[i];
+        }
+    }
+}
+
+// ---- mixed-precision evaluation: exact top-32-bit phase, FP32 sin/cos and weights, -----------
+// ---- Kahan-compensated FP32 accumulation; partials leave in FP64 ------------------------------
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
+    const float y = x - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+template <int NC, int FL>
+__global__ __launch_bounds__(256) void k2_mixed_kernel(
+    const uint4* __restrict__ zg, const float2* __restrict__ za, const float2* __restrict__ zb,
+    const float2* __restrict__ ze, int z_begin, int z_end, int zeros_per_split,
+    const uint4* __restrict__ cand, int n_cand, int n_ctiles,
+    double* __restrict__ partial, int split_base, int n_cand_pad) {
+    constexpr bool D1 = (FL & K2_D1) != 0;
+    constexpr bool D2 = (FL & K2_D2) != 0;
+    __shared__ uint4 s_g[2][K2_ZT];
+    __shared__ float4 s_ab[2][K2_ZT];
+    __shared__ float2 s_e[2][D2 ? K2_ZT : 1];
+
+    const int L = xcd_linear_block();
+    const int split = L / n_ctiles, ct = L - split * n_ctiles;
+    const int z0 = z_begin + split * zeros_per_split;
+    const int z1 = min(z_end, z0 + zeros_per_split);
+    const int tid = threadIdx.x;
+
+    Limbs4 v[NC];
+    float s0[NC], c0[NC], s1[NC], c1[NC], s2[NC], c2[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+        uint4 cv = make_uint4(0, 0, 0, 0);
+        if (c < n_cand) cv = cand[c];
+        v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
+        s0[i] = c0[i] = s1[i] = c1[i] = s2[i] = c2[i] = 0.0f;
+    }
+    const int ntiles = z1 > z0 ? (z1 - z0 + K2_ZT - 1) / K2_ZT : 0;
+    uint4 rg = make_uint4(0, 0, 0, 0);
+    float4 rab = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 re = make_float2(0.f, 0.f);
+    auto fetch = [&](int t) {
+        const int j = z0 + t * K2_ZT + tid;
+        if (j < z1) {
+            rg = zg[j];
+            const float2 a = za[j];
+            float2 b = make_float2(0.f, 0.f);
+            if (D1) b = zb[j];
+            rab = make_float4(a.x, a.y, b.x, b.y);
+            if (D2) re = ze[j];
+        } else {
+            rg = make_uint4(0, 0, 0, 0); rab = make_float4(0.f, 0.f, 0.f, 0.f);
+            re = make_float2(0.f, 0.f);
+        }
+    };
+    auto stage = [&](int buf) {
+        s_g[buf][tid] = rg; s_ab[buf][tid] = rab;
+        if (D2) s_e[buf][tid] = re;
+    };
+    if (ntiles > 0) { fetch(0); stage(0); }
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) fetch(t + 1);
+#pragma unroll 2
+        for (int z = 0; z < K2_ZT; ++z) {
+            const uint4 gg = s_g[buf][z];
+            const float4 ab = s_ab[buf][z];
+            float2 ee;
+            if (D2) ee = s_e[buf][z];
+            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const uint32_t W = phase_word_hi32(g, v[i]);
+                float s, c;
+                sincos_turns32(W, s, c);
+                kahan_add(s0[i], c0[i], fmaf(ab.x, c, -ab.y * s));
+                if (D1) kahan_add(s1[i], c1[i], -fmaf(ab.z, s, ab.w * c));
+                if (D2) kahan_add(s2[i], c2[i], fmaf(ee.y, s, -ee.x * c));
+            }
+        }
+        if (t + 1 < ntiles) stage(buf ^ 1);
+        __syncthreads();
+    }
+    double* prow = partial + (size_t)(split_base + split) * 3 * n_cand_pad;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+        if (c < n_cand_pad) {
+            prow[c] = (double)s0[i] - (double)c0[i];
+            if (D1) prow[n_cand_pad + c] = (double)s1[i] - (double)c1[i];
+            if (D2) prow[2 * n_cand_pad + c] = (double)s2[i] - (double)c2[i];
+        }
+    }
+}
+
+// ---- deterministic split reduction ---------------------------------------------------------------
+__global__ __launch_bounds__(256) void k2_reduce_kernel(
+    const double* __restrict__ partial, int n_splits, int n_cand, int n_cand_pad, unsigned flags,
+    double* __restrict__ K, double* __restrict__ dK, double* __restrict__ d2K) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cand) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int s = 0; s < n_splits; ++s) {
+        const double* row = partial + (size_t)s * 3 * n_cand_pad;
+        s0 += row[c];
+        if (flags & K2_D1) s1 += row[n_cand_pad + c];
+        if (flags & K2_D2) s2 += row[2 * n_cand_pad + c];
+    }
+    if (K) K[c] = s0;
+    if (dK) dK[c] = (flags & K2_D1) ? s1 : 0.0;
+    if (d2K) d2K[c] = (flags & K2_D2) ? s2 : 0.0;
+}
+
+// ---- debug: phase words --------------------------------------------------------------------------
+// mode 0: fast W (skip + fix-up), 1: schoolbook W, 2: fast top-32 word (zero-extended)
+__global__ __launch_bounds__(256) void k2_phase_words_kernel(
+    const uint4* __restrict__ zg, int z0, int nz, const uint4* __restrict__ cand, int n_cand,
+    int mode, uint64_t* __restrict__ out) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n_cand * nz) return;
+    const int c = (int)(idx / nz), j = (int)(idx - (long long)c * nz);
+    const uint4 gg = zg[z0 + j], cv = cand[c];
+    const Limbs4 g{gg.x, gg.y, gg.z, gg.w}, v{cv.x, cv.y, cv.z, cv.w};
+    uint64_t W;
+    if (mode == 0) W = phase_word(g, v);
+    else if (mode == 1) W = phase_word_exact(g, v);
+    else W = phase_word_hi32(g, v);
+    out[idx] = W;
+}
+
+// ---- debug: FP64 / FP32 sin,cos of phase words ------------------------------------------------
+__global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __restrict__ W,
+                                                              int n, int mode,
+                                                              double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mode == 0) {
+        double s, c;
+        sincos_turns64(W[i], s, c);
+        out[2 * i] = s; out[2 * i + 1] = c;
+    } else {
+        float s, c;
+        sincos_turns32((uint32_t)(W[i] >> 32), s, c);
+        out[2 * i] = s; out[2 * i + 1] = c;
+    }
+}
+
+// ---- Newton update on 64.64 points (polish) ---------------------------------------------------
+// Rule of search/path1_eval_fast.cpp:120-128: require |K''| > 1e-12, step = K'/K'' clamped to
+// +-clamp, v -= step, stop once |step| < 1e-13.  A candidate whose flag is 0 has stopped.
+__global__ __launch_bounds__(256) void k2_newton_kernel(
+    uint4* __restrict__ cand, const double* __restrict__ dK, const double* __restrict__ d2K,
+    int n, double clamp, int* __restrict__ active, int* __restrict__ steps) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || !active[c]) return;
+    const double fpp = d2K[c];
+    if (!(fabs(fpp) > 1e-12)) { active[c] = 0; return; }
+    double step = dK[c] / fpp;
+    step = fmin(fmax(step, -clamp), clamp);
+    const long long d = llrint(ldexp(step, 64));          // |step| <= clamp < 2^-1
+    uint4 v = cand[c];
+    unsigned __int128 x = ((unsigned __int128)(((uint64_t)v.w << 32) | v.z) << 64) |
+                          (((uint64_t)v.y << 32) | v.x);
+    x -= (unsigned __int128)(__int128)d;
+    const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+    cand[c] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    steps[c] += 1;
+    if (fabs(step) < 1e-13) active[c] = 0;
+}
+
+// ---- reference-format compute_peak_AB (search/path1_eval_fast.cpp:15-53) ---------------------
+// one lane per (peak, zero); ph = xh*ghi + xl*glo - psi mod 2^128; th = top64 * 2pi/2^64 + g dy0
+__global__ __launch_bounds__(256) void k2_ref_peak_ab_kernel(
+    int n_peaks, const uint64_t* __restrict__ x_lo, const uint64_t* __restrict__ x_hi,
+    const double* __restrict__ dy0, int n_zeros, const double* __restrict__ alpha,
+    const double* __restrict__ gamma, const uint64_t* __restrict__ ghi_lo,
+    const uint64_t* __restrict__ ghi_hi, const uint64_t* __restrict__ glo_lo,
+    const uint64_t* __restrict__ glo_hi, const uint64_t* __restrict__ psi_lo,
+    const uint64_t* __restrict__ psi_hi, double* __restrict__ A, double* __restrict__ B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (i >= n_zeros || p >= n_peaks) return;
+    typedef unsigned __int128 u128;
+    const u128 x = ((u128)x_hi[p] << 64) | x_lo[p];
+    const uint64_t xh = (uint64_t)(x >> 48), xl = (uint64_t)(x & ((1ull << 48) - 1));
+    const u128 gh = ((u128)ghi_hi[i] << 64) | ghi_lo[i];
+    const u128 gl = ((u128)glo_hi[i] << 64) | glo_lo[i];
+    const u128 ps = ((u128)psi_hi[i] << 64) | psi_lo[i];
+    const u128 ph = (u128)xh * gh + (u128)xl * gl - ps;
+    const double th = (double)(uint64_t)(ph >> 64) * K2_TWO_PI_2M64 + gamma[i] * dy0[p];
+    double s, c;
+    sincos_general(th, s, c);
+    const double ta = 2.0 * alpha[i];
+    A[(size_t)p * n_zeros + i] = ta * c;
+    B[(size_t)p * n_zeros + i] = ta * s;
+}
+
+// ---- launchers -------------------------------------------------------------------------------------
+namespace k2impl {
+
+template <int NC, int FL>
+static hipError_t launch_plain_t(const LaunchArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((k2_plain_kernel<NC, FL>), dim3(a.n_blocks), dim3(K2_ZT), 0, st,
+                       a.zg, a.za, a.zb, a.ze, a.z_begin, a.z_end, a.zeros_per_split, a.cand,
+                       a.n_cand, a.n_ctiles, a.partial, a.split_base, a.n_cand_pad);
+    return hipGetLastError();
+}
+
+template <int NC, int FL>
+static hipError_t launch_mixed_t(const LaunchArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((k2_mixed_kernel<NC, FL>), dim3(a.n_blocks), dim3(K2_ZT), 0, st,
+                       a.zg, a.zaf, a.zbf, a.zef, a.z_begin, a.z_end, a.zeros_per_split, a.cand,
+                       a.n_cand, a.n_ctiles, a.partial, a.split_base, a.n_cand_pad);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t dispatch_fl(const LaunchArgs& a, bool mixed, hipStream_t st) {
+    const unsigned fl = a.flags & (K2_D1 | K2_D2);
+    if (mixed) {
+        switch (fl) {
+            case 0: return launch_mixed_t<NC, 0>(a, st);
+            case K2_D1: return launch_mixed_t<NC, K2_D1>(a, st);
+            case K2_D2: return launch_mixed_t<NC, K2_D2>(a, st);
+            default: return launch_mixed_t<NC, K2_D1 | K2_D2>(a, st);
+        }
+    }
+    switch (fl) {
+        case 0: return launch_plain_t<NC, 0>(a, st);
+        case K2_D1: return launch_plain_t<NC, K2_D1>(a, st);
+        case K2_D2: return launch_plain_t<NC, K2_D2>(a, st);
+        default: return launch_plain_t<NC, K2_D1 | K2_D2>(a, st);
+    }
+}
+
+hipError_t launch_eval(const LaunchArgs& a, bool mixed, hipStream_t st) {
+    switch (a.nc) {
+        case 1: return dispatch_fl<1>(a, mixed, st);
+        case 2: return dispatch_fl<2>(a, mixed, st);
+        case 4: return dispatch_fl<4>(a, mixed, st);
+        case 8: return dispatch_fl<8>(a, mixed, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,
+                          WeightArrays& out, hipStream_t st) {
+    const int nb = (n_pad + 255) / 256;
+    hipLaunchKernelGGL(k2_weights_kernel, dim3(nb), dim3(256), 0, st, gamma, (const double2*)w, n,
+                       n_pad, k, out.a, out.b, out.e, out.af, out.bf, out.ef);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_cand_pad,
+                         unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
+    const int nb = (n_cand + 255) / 256;
+    hipLaunchKernelGGL(k2_reduce_kernel, dim3(nb), dim3(256), 0, st, partial, n_splits, n_cand,
+                       n_cand_pad, flags, K, dK, d2K);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand, int n_cand,
+                              int mode, uint64_t* out, hipStream_t st) {
+    const long long n = (long long)n_cand * nz;
+    const long long nb = (n + 255) / 256;
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k2_phase_words_kernel, dim3((unsigned)nb), dim3(256), 0, st, zg, z0, nz,
+                       cand, n_cand, mode, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k2_sincos_words_kernel, dim3((n + 255) / 256), dim3(256), 0, st, W, n,
+                       mode, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_newton(uint4* cand, const double* dK, const double* d2K, int n, double clamp,
+                         int* active, int* steps, hipStream_t st) {
+    hipLaunchKernelGGL(k2_newton_kernel, dim3((n + 255) / 256), dim3(256), 0, st, cand, dK, d2K, n,
+                       clamp, active, steps);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_peak_ab(int n_peaks, const uint64_t* const* p, const double* dy0,
+                              int n_zeros, const double* alpha, const double* gamma, double* A,
+                              double* B, hipStream_t st) {
+    // p: x_lo, x_hi, ghi_lo, ghi_hi, glo_lo, glo_hi, psi_lo, psi_hi
+    dim3 grid((n_zeros + 255) / 256, n_peaks);
+    hipLaunchKernelGGL(k2_ref_peak_ab_kernel, grid, dim3(256), 0, st, n_peaks, p[0], p[1], dy0,
+                       n_zeros, alpha, gamma, p[2], p[3], p[4], p[5], p[6], p[7], A, B);
+    return hipGetLastError();
+}
+
+}  // namespace k2impl

@@ -1,0 +1,183 @@
+"""The reference's table and candidate formats, and conversion to the canonical encoding.
+
+* ``zeros.bin`` (search/path1_build_zeros_100k.py:34-47): int32 n, then planar arrays
+  two_astar[n] (2 |w| e^{-3e-9 g^2}), gamma[n], ghi_lo, ghi_hi, glo_lo, glo_hi, psi_lo, psi_hi.
+* ``zeros_600k.bin`` (legacy/superseded/certify_peak2_600k.py:119-143, read by
+  search/eval_tech1_600k.py:38-50): uint64 n, gamma[n], alpha[n] (= |w|), then n x 6 uint64
+  rows (ghi_lo, ghi_hi, glo_lo, glo_hi, psi_lo, psi_hi).
+* words: glo = rint(g 2^98), ghi = rint(g 2^146) mod 2^128, psi = rint(frac(-arg w / 2pi) 2^128)
+  with g = gamma / 2pi (search/path1_build_zeros_100k.py:56-58,102-104).
+* candidates: ``CLUSTER`` log lines carry x (Q98.30 integer) and dy, y = x / 2^30 + dy
+  (search/eval_sub35_candidates.py:26-46, search/eval_tech1_600k.py:27-29,216-218).
+
+Canonical words: G = floor(g 2^110) (Q18.110) and V = v 2^64 (64.64).  A canonical table maps
+exactly onto reference words (ghi = G 2^36 mod 2^128, glo = round(G / 2^12)); a reference table
+maps back with :func:`table_from_reference`.
+"""
+from __future__ import annotations
+
+import re
+import struct
+
+import numpy as np
+
+from . import synth
+
+M64 = (1 << 64) - 1
+M128 = (1 << 128) - 1
+
+
+def _words(x: int):
+    return x & M64, (x >> 64) & M64
+
+
+def _u128(lo, hi) -> int:
+    return (int(hi) << 64) | int(lo)
+
+
+def reference_words(table: synth.ZeroTable, weight_k: float | None = None):
+    """Reference-format arrays for a canonical table.
+
+    Returns dict(gamma, alpha, ghi_lo, ghi_hi, glo_lo, glo_hi, psi_lo, psi_hi) where alpha = |w|
+    (zeros_600k.bin convention) or, with weight_k, two_astar = 2|w| e^{-k g^2} (zeros.bin).
+    """
+    n = table.n
+    out = {k: np.zeros(n, dtype=np.uint64) for k in
+           ("ghi_lo", "ghi_hi", "glo_lo", "glo_hi", "psi_lo", "psi_hi")}
+    arg = np.angle(table.w)
+    frac = np.mod(-arg / (2 * np.pi), 1.0)
+    for j in range(n):
+        G = _u128(*table.G[j])
+        ghi = (G << 36) & M128
+        glo = (G + (1 << 11)) >> 12
+        # psi: 53 significant bits of the phase fraction, scaled to 2^128
+        psi = int(round(float(frac[j]) * 2.0 ** 53)) << 75 & M128
+        out["ghi_lo"][j], out["ghi_hi"][j] = _words(ghi)
+        out["glo_lo"][j], out["glo_hi"][j] = _words(glo)
+        out["psi_lo"][j], out["psi_hi"][j] = _words(psi)
+    alpha = np.abs(table.w)
+    out["gamma"] = table.gamma.copy()
+    if weight_k is None:
+        out["alpha"] = alpha
+    else:
+        out["two_astar"] = 2.0 * alpha * np.exp(-weight_k * table.gamma ** 2)
+    return out
+
+
+def table_from_reference(gamma, alpha, ghi_lo, ghi_hi, glo_lo, glo_hi, psi_lo, psi_hi):
+    """Canonical table from reference words.
+
+    ghi = rint(g 2^146) mod 2^128 keeps only g mod 2^-18 (the rest never changes the
+    reference's phase, path1_eval_fast.cpp:45); glo = rint(g 2^98) carries the leading bits.
+    g 2^146 = K 2^128 + ghi with K = round((glo 2^48 - ghi) / 2^128), then G = round(g 2^146 / 2^36).
+    w = alpha e^{-2 pi i psi / 2^128}.
+    """
+    n = len(gamma)
+    G = np.zeros((n, 2), dtype=np.uint64)
+    for j in range(n):
+        ghi = _u128(ghi_lo[j], ghi_hi[j])
+        glo = _u128(glo_lo[j], glo_hi[j])
+        K = ((glo << 48) - ghi + (1 << 127)) >> 128
+        full = (K << 128) + ghi
+        G[j, 0], G[j, 1] = _words((full + (1 << 35)) >> 36)
+    ang = np.array([-2 * np.pi * (_u128(l, h) / 2.0 ** 128) for l, h in zip(psi_lo, psi_hi)])
+    w = np.asarray(alpha, dtype=np.float64) * np.exp(1j * ang)
+    return synth.ZeroTable(G=G, gamma=np.asarray(gamma, dtype=np.float64), w=w)
+
+
+def candidates_to_x_dy(V):
+    """64.64 words -> reference (x in Q98.30, dy0 float) with V = x 2^34 + dy0 2^64 exactly
+    representable remainder (dy0 in [0, 2^-30))."""
+    xs, dys = [], []
+    for lo, hi in V:
+        v = _u128(lo, hi)
+        x = v >> 34
+        r = v - (x << 34)
+        xs.append(x)
+        dys.append(r / 2.0 ** 64)
+    return xs, np.array(dys)
+
+
+def x_words(xs):
+    lo = np.array([x & M64 for x in xs], dtype=np.uint64)
+    hi = np.array([(x >> 64) & M64 for x in xs], dtype=np.uint64)
+    return lo, hi
+
+
+# ---- binary table files --------------------------------------------------------------------
+
+def write_zeros_bin(path, ref: dict):
+    """search/path1_build_zeros_100k.py:34-47 layout (needs ref['two_astar'])."""
+    n = len(ref["gamma"])
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", n))
+        for key in ("two_astar", "gamma", "ghi_lo", "ghi_hi", "glo_lo", "glo_hi", "psi_lo",
+                    "psi_hi"):
+            f.write(np.ascontiguousarray(ref[key][:n]).tobytes())
+
+
+def read_zeros_bin(path) -> dict:
+    with open(path, "rb") as f:
+        n = struct.unpack("<i", f.read(4))[0]
+        out = {"two_astar": np.frombuffer(f.read(8 * n), dtype=np.float64).copy(),
+               "gamma": np.frombuffer(f.read(8 * n), dtype=np.float64).copy()}
+        for key in ("ghi_lo", "ghi_hi", "glo_lo", "glo_hi", "psi_lo", "psi_hi"):
+            out[key] = np.frombuffer(f.read(8 * n), dtype=np.uint64).copy()
+    return out
+
+
+def write_zeros_600k_bin(path, ref: dict):
+    """legacy/superseded/certify_peak2_600k.py:119-143 layout (needs ref['alpha'])."""
+    n = len(ref["gamma"])
+    rows = np.stack([ref[k] for k in ("ghi_lo", "ghi_hi", "glo_lo", "glo_hi", "psi_lo",
+                                      "psi_hi")], axis=1).astype(np.uint64)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<Q", n))
+        f.write(np.ascontiguousarray(ref["gamma"], dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(ref["alpha"], dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(rows).tobytes())
+
+
+def read_zeros_600k_bin(path) -> dict:
+    """search/eval_tech1_600k.py:38-50."""
+    with open(path, "rb") as f:
+        n = struct.unpack("<Q", f.read(8))[0]
+        gamma = np.frombuffer(f.read(n * 8), dtype=np.float64).copy()
+        alpha = np.frombuffer(f.read(n * 8), dtype=np.float64).copy()
+        raw = np.frombuffer(f.read(n * 48), dtype=np.uint64).reshape(n, 6)
+    out = {"gamma": gamma, "alpha": alpha}
+    for i, key in enumerate(("ghi_lo", "ghi_hi", "glo_lo", "glo_hi", "psi_lo", "psi_hi")):
+        out[key] = np.ascontiguousarray(raw[:, i])
+    return out
+
+
+# ---- CLUSTER log lines ------------------------------------------------------------------
+
+# the fields the reference's parsers read: |h|, h, y and the (x, dy) pair
+# (search/eval_sub35_candidates.py:26-46; search/eval_tech1_600k.py:27-29)
+_CLUSTER = re.compile(
+    r"CLUSTER #(?P<n>\d+)(?:\s+\[coset=(?P<coset>[^\]]+)\])?\s+\|h\|=(?P<absh>[0-9.]+)\s+"
+    r"h=(?P<h>[+\-0-9.]+).*?\(x=(?P<x>\d+),\s*dy=(?P<dy>[+\-0-9.eE]+)\)")
+
+
+def parse_cluster_lines(lines):
+    """Candidates from CLUSTER lines: list of dict(n, coset, absh, h, x, dy)."""
+    out = []
+    for line in lines:
+        m = _CLUSTER.search(line)
+        if not m:
+            continue
+        out.append({"n": int(m["n"]), "coset": m["coset"], "absh": float(m["absh"]),
+                    "h": float(m["h"]), "x": int(m["x"]), "dy": float(m["dy"])})
+    return out
+
+
+def dedupe_candidates(cands, shift=32):
+    """Keep the largest |h| per bucket x >> shift (search/eval_sub35_candidates.py dedupes on
+    x >> 32; search/path1_kernel_opt.py:224-264 on x >> 18)."""
+    best = {}
+    for c in cands:
+        key = c["x"] >> shift
+        if key not in best or c["absh"] > best[key]["absh"]:
+            best[key] = c
+    return sorted(best.values(), key=lambda c: -c["absh"])

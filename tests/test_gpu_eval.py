@@ -1,0 +1,133 @@
+"""K2 / K2' / K2'' of the HIP evaluator against the oracle (golden fixtures and live CPU runs).
+
+Tolerances: K2 within 1e-9 absolute (the accuracy bar of the metric; FP64 lands near 1e-13);
+derivatives in a scale-aware relative form |dK'| / sum_j g_j |a_j| <= 1e-12 (FP64) and
+|dK''| / sum_j g_j^2 |a_j| <= 1e-12, since K' ~ 0 near peaks makes plain relative error
+meaningless.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden
+from mertens_bound_amd import evaluator as ev
+from mertens_bound_amd import synth
+
+pytestmark = pytest.mark.gpu
+ALL = ev.VALUE | ev.D1 | ev.D2
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    c = synth.CONFIG1
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    V = synth.make_candidates(c["n_cand"], c["v_lo"], c["v_hi"], c["seed"])
+    return tb, ev.K2Table(tb), V, c["k"]
+
+
+def _scales(tb, k, n=None):
+    n = tb.n if n is None else n
+    a = 2 * np.exp(-k * tb.gamma[:n] ** 2) * np.abs(tb.w[:n])
+    return np.sum(a), np.sum(tb.gamma[:n] * a), np.sum(tb.gamma[:n] ** 2 * a)
+
+
+def test_config1_plain_vs_golden(cfg1):
+    tb, t, V, k = cfg1
+    g = golden("config1_oracle.npz")
+    K, dK, d2K = t.eval(V, k, flags=ALL)
+    s0, s1, s2 = _scales(tb, k)
+    errK = np.max(np.abs(K - g["K"]))
+    print(f"config1 plain max|dK2|={errK:.3e}")
+    assert errK <= 1e-12
+    assert np.max(np.abs(dK - g["dK"])) / s1 <= 1e-12
+    assert np.max(np.abs(d2K - g["d2K"])) / s2 <= 1e-12
+
+
+def test_config1_mixed_vs_golden(cfg1):
+    tb, t, V, k = cfg1
+    g = golden("config1_oracle.npz")
+    K, dK, d2K = t.eval(V, k, flags=ALL | ev.MIXED)
+    errK = np.max(np.abs(K - g["K"]))
+    s0, s1, s2 = _scales(tb, k)
+    print(f"config1 mixed max|dK2|={errK:.3e} max|dK2'|/scale={np.max(np.abs(dK - g['dK'])) / s1:.3e}")
+    assert errK <= 1e-9
+    assert np.max(np.abs(dK - g["dK"])) / s1 <= 1e-6
+
+
+def test_deterministic(cfg1):
+    tb, t, V, k = cfg1
+    a = t.eval(V, k, flags=ALL)
+    b = t.eval(V, k, flags=ALL)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 4, 8])
+@pytest.mark.parametrize("target", [37, 4096])
+def test_tilings_agree(cfg1, nc, target):
+    tb, t, V, k = cfg1
+    g = golden("config1_oracle.npz")
+    t.set_tiling(nc=nc, target_blocks=target)
+    try:
+        K, dK, _ = t.eval(V, k, flags=ev.VALUE | ev.D1)
+    finally:
+        t.set_tiling(nc=4, target_blocks=2048)
+    assert np.max(np.abs(K - g["K"])) <= 1e-12
+
+
+@pytest.mark.parametrize("n_cand,n_zeros", [(1, 1), (1, 10_000), (255, 257), (257, 3333),
+                                            (1000, 4096), (300, 9999)])
+def test_ragged_sizes_vs_port(cfg1, oracle_lib, n_cand, n_zeros):
+    tb, t, V, k = cfg1
+    Vs = V[:n_cand]
+    K, dK, d2K = t.eval(Vs, k, n_zeros=n_zeros, flags=ALL)
+    Ko, dKo, d2Ko = oracle_lib.eval_f64(tb, Vs, k, n_zeros=n_zeros)
+    s0, s1, s2 = _scales(tb, k, n_zeros)
+    assert np.max(np.abs(K - Ko)) <= 1e-12
+    assert np.max(np.abs(dK - dKo)) / s1 <= 1e-12
+    assert np.max(np.abs(d2K - d2Ko)) / s2 <= 1e-12
+
+
+def test_value_only_and_device_path(cfg1):
+    import torch
+    tb, t, V, k = cfg1
+    K, _, _ = t.eval(V, k, flags=ev.VALUE)
+    Kd, dKd, _ = t.eval_device(ev.v_tensor(V), k, flags=ev.VALUE | ev.D1)
+    torch.cuda.synchronize()
+    Kh, dKh, _ = t.eval(V, k, flags=ev.VALUE | ev.D1)
+    assert np.array_equal(Kd.cpu().numpy(), Kh) and np.array_equal(dKd.cpu().numpy(), dKh)
+    assert np.array_equal(K, Kh)
+
+
+def test_argument_errors(cfg1):
+    tb, t, V, k = cfg1
+    with pytest.raises(ev.K2Error):
+        t.eval(V[:4], k, n_zeros=tb.n + 1)
+    with pytest.raises(ev.K2Error):
+        t.eval(V[:4], -1.0)
+    with pytest.raises(ev.K2Error):
+        t.set_tiling(nc=3)
+    K, _, _ = t.eval(V[:0], k)
+    assert K.shape == (0,)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "config2_sub256.npz")),
+                    reason="config2 fixture not generated")
+def test_config2_subsample_vs_golden():
+    """256 candidates of the full-size workload against all 2,325,572 zeros."""
+    c = synth.CONFIG2
+    g = golden("config2_sub256.npz")
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    assert tb.sha256() == str(g["table_sha256"])
+    t = ev.K2Table(tb)
+    K, dK, d2K = t.eval(g["V"], c["k"], flags=ALL)
+    s0, s1, s2 = _scales(tb, c["k"])
+    e = np.max(np.abs(K - g["K"]))
+    print(f"config2 sub256 plain max|dK2|={e:.3e}")
+    assert e <= 1e-9
+    assert np.max(np.abs(dK - g["dK"])) / s1 <= 1e-11
+    Km, dKm, _ = t.eval(g["V"], c["k"], flags=ev.VALUE | ev.D1 | ev.MIXED)
+    em = np.max(np.abs(Km - g["K"]))
+    print(f"config2 sub256 mixed max|dK2|={em:.3e}")
+    assert em <= 1e-9

@@ -1,0 +1,80 @@
+"""Bit-exact phase words and the sin/cos of the HIP path against the CPU oracle."""
+import numpy as np
+import pytest
+
+from mertens_bound_amd import evaluator as ev
+from mertens_bound_amd import synth
+
+pytestmark = pytest.mark.gpu
+M64 = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def small():
+    tb = synth.make_zero_table(2000, 1)
+    return tb, ev.K2Table(tb)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_phase_words_bit_exact(small, oracle_lib, mode):
+    tb, t = small
+    V = synth.make_candidates(256, 1e18, 4e18, 1)
+    W = t.phase_words(V, mode=mode)
+    Wo = oracle_lib.phase_words(tb.G, V)
+    assert np.array_equal(W, Wo)
+
+
+def test_phase_fixup_cases_hit_and_exact(small, oracle_lib):
+    """Pairs whose partial product has bits [98,110) all ones take the exact recomputation;
+    they occur at rate 2^-12, so 512 x 2000 pairs contain ~250 of them."""
+    tb, t = small
+    V = synth.make_candidates(512, 3.4e18, 3.6e18, 2)
+    W = t.phase_words(V, mode=0)
+    Wo = oracle_lib.phase_words(tb.G, V)
+    assert np.array_equal(W, Wo)
+    hits = 0
+    for c, v in enumerate(synth.v_to_int(V[:64])):
+        for j, g in enumerate(synth.v_to_int(tb.G)):
+            gl = [(g >> (32 * i)) & 0xFFFFFFFF for i in range(4)]
+            vl = [(v >> (32 * i)) & 0xFFFFFFFF for i in range(4)]
+            U = sum(gl[i] * vl[j2] << (32 * (i + j2)) for i in range(4) for j2 in range(4)
+                    if i + j2 >= 2)
+            hits += ((U >> 98) & 0xFFF) == 0xFFF
+    assert hits > 0
+
+
+def test_phase_words_edges(oracle_lib):
+    edge = [0, 1, (1 << 128) - 1, 1 << 110, (1 << 127) + 12345, M64, 1 << 64, (1 << 98) - 1]
+    G = synth.v_from_int(edge)
+    tb = synth.ZeroTable(G=G, gamma=np.ones(len(edge)), w=np.zeros(len(edge), complex))
+    t = ev.K2Table(tb)
+    V = synth.v_from_int(edge + [(3 << 64) + (1 << 62)])
+    for mode in (0, 1):
+        assert np.array_equal(t.phase_words(V, mode=mode), oracle_lib.phase_words(G, V))
+    top = t.phase_words(V, mode=2)
+    assert np.array_equal(top >> np.uint64(32), oracle_lib.phase_words(G, V) >> np.uint64(32))
+
+
+def test_top32_phase_word_exact(small, oracle_lib):
+    tb, t = small
+    V = synth.make_candidates(512, 3.4e18, 3.6e18, 3)
+    top = t.phase_words(V, mode=2) >> np.uint64(32)
+    assert np.array_equal(top, oracle_lib.phase_words(tb.G, V) >> np.uint64(32))
+
+
+def test_sincos_of_words():
+    rng = np.random.default_rng(0)
+    W = rng.integers(0, 2**64, size=100_000, dtype=np.uint64)
+    W[:8] = np.array([0, 1 << 62, 1 << 63, 3 << 62, (1 << 61), (1 << 61) - 1, M64, 1 << 60],
+                     dtype=np.uint64)
+    s, c = ev.sincos_words(W, mode=0)
+    th = 2 * np.pi * (W.astype(np.float64) / 2.0 ** 64)
+    import mpmath
+    # exact reference on a sample
+    for i in list(range(8)) + list(range(8, 100_000, 997)):
+        t = 2 * mpmath.pi * mpmath.mpf(int(W[i])) / mpmath.mpf(2) ** 64
+        assert abs(s[i] - float(mpmath.sin(t))) <= 2.3e-16
+        assert abs(c[i] - float(mpmath.cos(t))) <= 2.3e-16
+    assert np.max(np.abs(s - np.sin(th))) < 1e-15
+    s32, c32 = ev.sincos_words(W, mode=1)
+    assert np.max(np.abs(s32 - np.sin(th))) < 3e-7 and np.max(np.abs(c32 - np.cos(th))) < 3e-7

@@ -1,0 +1,199 @@
+"""Pins the CPU oracle (oracle/) before anything is checked against it.
+
+Each oracle tier is checked against something that does not retype its own formula:
+Python big-integer arithmetic, mpmath, closed forms, structural identities between the value
+and derivative formulas, finite differences in binary128, and the reference evaluator compiled
+from its own source (oracle/_ref, where built).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+from mertens_bound_amd import formats, synth
+
+M64 = (1 << 64) - 1
+M128 = (1 << 128) - 1
+
+
+def _table(n, seed=1):
+    return synth.make_zero_table(n, seed)
+
+
+def _one_zero(G: int, w: complex):
+    Gw = np.array([[G & M64, G >> 64]], dtype=np.uint64)
+    g = synth.gamma_from_words(Gw)
+    return synth.ZeroTable(G=Gw, gamma=g, w=np.array([w], dtype=np.complex128))
+
+
+def test_phase_words_match_big_integers(oracle_lib):
+    rng = np.random.default_rng(7)
+    G = rng.integers(0, 2**64, size=(64, 2), dtype=np.uint64)
+    V = rng.integers(0, 2**64, size=(16, 2), dtype=np.uint64)
+    edge = [0, 1, M128, 1 << 110, (1 << 127) + 12345, M64, 1 << 64]
+    G = np.concatenate([G, synth.v_from_int(edge)])
+    V = np.concatenate([V, synth.v_from_int(edge)])
+    W = oracle_lib.phase_words(G, V)
+    for c, v in enumerate(synth.v_to_int(V)):
+        for j, g in enumerate(synth.v_to_int(G)):
+            assert int(W[c, j]) == ((g * v) >> 110) & M64
+
+
+def test_phase_word_quarter_turn_closed_form(oracle_lib):
+    # G = 2^110 is gamma = 2 pi; v = n + 1/4 turns the phase by exactly a quarter
+    for n in (0, 1, 3 * 10**18, (1 << 63) - 1):
+        V = synth.v_from_int([(n << 64) + (1 << 62)])
+        W = oracle_lib.phase_words(synth.v_from_int([1 << 110]), V)
+        assert int(W[0, 0]) == 1 << 62
+
+
+@pytest.mark.parametrize("tier", ["f64", "quad"])
+def test_single_zero_closed_forms(oracle_lib, tier):
+    k = 1.5e-10
+    w = complex(0.03, -0.07)
+    tb = _one_zero(1 << 110, w)                  # gamma = 2 pi exactly
+    n = 3 * 10**18
+    V = synth.v_from_int([n << 64, (n << 64) + (1 << 62), (n << 64) + (1 << 63)])
+    if tier == "f64":
+        K, dK, d2K = oracle_lib.eval_f64(tb, V, k)
+    else:
+        K, dK, d2K, _ = oracle_lib.eval_quad(tb, V, k)
+    a = 2 * np.exp(-k * (2 * np.pi) ** 2)
+    g = 2 * np.pi
+    # e^{i theta} = 1, i, -1 at v = n, n + 1/4, n + 1/2
+    np.testing.assert_allclose(K, [a * w.real, -a * w.imag, -a * w.real], rtol=0, atol=1e-15)
+    # K' = a g Re(i w e^{i theta}), K'' = -g^2 K
+    np.testing.assert_allclose(dK, [-a * g * w.imag, -a * g * w.real, a * g * w.imag],
+                               rtol=0, atol=1e-14)
+    np.testing.assert_allclose(d2K, -g * g * K, rtol=1e-14, atol=1e-14)
+
+
+def test_v_zero_is_weight_sum(oracle_lib):
+    tb = _table(300)
+    k = 3e-9
+    V = synth.v_from_int([0])
+    K, _, _, _ = oracle_lib.eval_quad(tb, V, k)
+    ref = 2 * np.sum(np.exp(-k * tb.gamma ** 2) * tb.w.real)
+    assert abs(K[0] - ref) < 1e-15
+
+
+def test_linearity_and_derivative_identities(oracle_lib):
+    """K is linear in w; K'[w] = K[i g w] and K''[w] = -K[g^2 w] tie the derivative formulas
+    to the value formula."""
+    tb = _table(400, seed=3)
+    V = synth.make_candidates(6, 1e18, 4e18, 3)
+    k = 1e-9
+    K1, dK1, d2K1, _ = oracle_lib.eval_quad(tb, V, k)
+    w2 = np.random.default_rng(1).normal(size=tb.n) + 1j * np.random.default_rng(2).normal(size=tb.n)
+    tb2 = synth.ZeroTable(tb.G, tb.gamma, w2 * 1e-2)
+    tbs = synth.ZeroTable(tb.G, tb.gamma, tb.w + w2 * 1e-2)
+    K2, _, _, _ = oracle_lib.eval_quad(tb2, V, k)
+    Ks, _, _, _ = oracle_lib.eval_quad(tbs, V, k)
+    np.testing.assert_allclose(Ks, K1 + K2, rtol=0, atol=1e-14)
+    tbi = synth.ZeroTable(tb.G, tb.gamma, 1j * tb.gamma * tb.w)
+    Ki, _, _, _ = oracle_lib.eval_quad(tbi, V, k)
+    np.testing.assert_allclose(dK1, Ki, rtol=1e-12, atol=1e-12)
+    tbg = synth.ZeroTable(tb.G, tb.gamma, tb.gamma ** 2 * tb.w)
+    Kg, _, _, _ = oracle_lib.eval_quad(tbg, V, k)
+    np.testing.assert_allclose(d2K1, -Kg, rtol=1e-12, atol=1e-9)
+
+
+def test_quad_matches_mpmath(oracle_lib):
+    tb = _table(60, seed=4)
+    V = synth.make_candidates(3, 3.4e18, 3.6e18, 4)
+    k = 1.5e-10
+    K, dK, d2K, _ = oracle_lib.eval_quad(tb, V, k)
+    mK, mdK, md2K = oracle_lib.mp_k2(tb.G, tb.w, V, k, dps=40)
+    np.testing.assert_allclose(K, [float(x) for x in mK], rtol=0, atol=1e-16)
+    np.testing.assert_allclose(dK, [float(x) for x in mdK], rtol=1e-15, atol=1e-14)
+    np.testing.assert_allclose(d2K, [float(x) for x in md2K], rtol=1e-15, atol=1e-10)
+
+
+def test_derivatives_match_binary128_differences(oracle_lib):
+    tb = _table(200, seed=5)
+    k = 1.5e-10
+    v0 = synth.v_to_int(synth.make_candidates(4, 3.4e18, 3.6e18, 5))
+    _, dK, d2K, _ = oracle_lib.eval_quad(tb, synth.v_from_int(v0), k)
+    # first derivative: h = 2^-30 (truncation (g h)^2/6 < 1e-13 relative)
+    h = 1 << 34
+    hp, lp = oracle_lib.eval_quad_pair(tb, synth.v_from_int([v + h for v in v0]), k)
+    hm, lm = oracle_lib.eval_quad_pair(tb, synth.v_from_int([v - h for v in v0]), k)
+    fd1 = ((hp - hm) + (lp - lm)) / (2 * 2.0 ** -30)
+    np.testing.assert_allclose(fd1, dK, rtol=1e-10, atol=1e-9)
+    # second derivative: h = 2^-16
+    h2 = 1 << 48
+    h0, l0 = oracle_lib.eval_quad_pair(tb, synth.v_from_int(v0), k)
+    hp, lp = oracle_lib.eval_quad_pair(tb, synth.v_from_int([v + h2 for v in v0]), k)
+    hm, lm = oracle_lib.eval_quad_pair(tb, synth.v_from_int([v - h2 for v in v0]), k)
+    fd2 = ((hp - 2 * h0 + hm) + (lp - 2 * l0 + lm)) / (2.0 ** -16) ** 2
+    np.testing.assert_allclose(fd2, d2K, rtol=1e-6, atol=1e-6 * np.max(np.abs(d2K)))
+
+
+def test_f64_tier_tracks_quad(oracle_lib):
+    tb = synth.make_zero_table(10_000, 0)
+    V = synth.make_candidates(32, 1e18, 4e18, 0)
+    K, dK, d2K = oracle_lib.eval_f64(tb, V, 1.5e-10)
+    Kq, dKq, d2Kq, aS = oracle_lib.eval_quad(tb, V, 1.5e-10)
+    assert np.max(np.abs(K - Kq)) < 1e-13
+    assert np.max(np.abs(dK - dKq)) < 1e-9
+
+
+def test_golden_config1_reproduces(oracle_lib):
+    g = golden("config1_oracle.npz")
+    c = synth.CONFIG1
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    assert str(g["table_sha256"]) == tb.sha256()
+    V = synth.make_candidates(c["n_cand"], c["v_lo"], c["v_hi"], c["seed"])
+    assert np.array_equal(V, g["V"])
+    K, dK, d2K, _ = oracle_lib.eval_quad(tb, V[:16], c["k"])
+    assert np.array_equal(K, g["K"][:16]) and np.array_equal(dK, g["dK"][:16])
+
+
+def test_polish_oracle_reaches_stationary_point(oracle_lib):
+    g = golden("polish_small.npz")
+    tb = synth.make_zero_table(3000, 11)
+    assert str(g["table_sha256"]) == tb.sha256()
+    Vout, K0, Kf, dKf, st = oracle_lib.polish_f64(tb, g["V0"], float(g["k"]))
+    assert np.array_equal(Vout, g["V"]) and np.array_equal(Kf, g["K"])
+    assert np.all(np.abs(Kf) >= np.abs(K0))
+    # scale-aware stationarity: |K'| small against sum g |a|
+    scale = np.sum(2 * np.exp(-float(g["k"]) * tb.gamma ** 2) * tb.gamma * np.abs(tb.w))
+    # candidates whose Newton loop stopped on |step| < 1e-13 sit on a stationary point
+    done = (st < 8) & np.any(Vout != g["V0"], axis=1)
+    assert done.sum() >= 4
+    assert np.all(np.abs(dKf[done]) < 1e-9 * scale)
+
+
+def test_restated_reference_matches_compiled_reference(oracle_lib):
+    L = oracle_lib.load_ref()
+    if L is None:
+        pytest.skip("oracle/_ref not built (reference source not mounted where it was built)")
+    from golden.make_golden import ref_inputs
+    tb, rw, args = ref_inputs()
+    A0, B0 = oracle_lib.ref_lib_compute_peak_AB(L, *args)
+    A1, B1 = oracle_lib.ref_compute_peak_AB(*args)
+    assert np.array_equal(A0, A1) and np.array_equal(B0, B1)
+    w = np.exp(-3e-9 * rw["gamma"] ** 2)
+    r0 = oracle_lib.ref_lib_polish(L, A0, B0, rw["gamma"], w)
+    r1 = oracle_lib.ref_polish_peaks_with_weights(A0, B0, rw["gamma"], w)
+    for a, b in zip(r0, r1):
+        assert np.array_equal(a, b)
+    gold = golden("ref_compat.npz")
+    assert np.array_equal(gold["h_pol"], r0[1]) and np.array_equal(gold["A_head"], A0[:, :8])
+
+
+def test_reference_format_matches_canonical(oracle_lib):
+    """The reference's two-word phase (ghi/glo split, psi) on canonical-derived words gives the
+    canonical K2: 2 Re(w e^{i g y}) = A with y = x/2^30 + dy0 (path1_eval_fast.cpp:7-14)."""
+    tb = synth.make_zero_table(500, 9)
+    rw = formats.reference_words(tb)
+    V = synth.make_candidates(8, 3.4e18, 3.6e18, 9)
+    xs, dy = formats.candidates_to_x_dy(V)
+    xlo, xhi = formats.x_words(xs)
+    A, B = oracle_lib.ref_compute_peak_AB(xlo, xhi, dy, rw["alpha"], rw["gamma"], rw["ghi_lo"],
+                                          rw["ghi_hi"], rw["glo_lo"], rw["glo_hi"], rw["psi_lo"],
+                                          rw["psi_hi"])
+    k = 1.5e-10
+    h0 = A @ np.exp(-k * tb.gamma ** 2)
+    K, _, _, _ = oracle_lib.eval_quad(tb, V, k)
+    # dy0 < 2^-30 is applied in FP64 as g * dy0 (<= 1e-5 rad), psi carries 53 bits
+    np.testing.assert_allclose(h0, K, rtol=0, atol=5e-12)
