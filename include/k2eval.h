@@ -16,7 +16,8 @@
  * describes the last failure of the calling thread.  Host buffers are borrowed for the duration
  * of the call.  Calls on one table handle are serialised by the caller; distinct handles may be
  * used from distinct threads.  *_device entry points take device pointers and a hipStream_t
- * (as void*, NULL = the table's own stream) and are asynchronous with respect to the host.
+ * (as void*; NULL = the default stream) and are asynchronous with respect to the host, except
+ * the first use of a new k, which computes that k's weight arrays and waits for them.
  *
  * Reference interfaces replaced (reference repository bzyzz/mertens_bound):
  *   search/path1_eval_fast.cpp:15-31   compute_peak_AB            (same signature, below)

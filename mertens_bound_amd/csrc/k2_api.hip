@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "k2_device.h"
 #include "k2_internal.h"
 
 using namespace k2impl;
@@ -18,6 +19,33 @@ using namespace k2impl;
 #define K2_VERSION 1
 #define K2_PAD 4096            // zero arrays are padded to a multiple of this
 #define K2_MAX_WEIGHT_SETS 4   // cached per-k weight arrays per table
+#define K2_TAB_N K2_TAB        // entries of the (cos, sin) table the kernels stage in LDS
+
+static int fail(int code, const std::string& msg);
+
+void k2impl::sincos_tables(int n, double2* t64, float2* t32) {
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < n; ++k) {
+        const long double a = two_pi * (long double)k / (long double)n;
+        const long double c = cosl(a), s = sinl(a);
+        t64[k] = make_double2((double)c, (double)s);
+        t32[k] = make_float2((float)c, (float)s);
+    }
+}
+
+static int upload_tables(double2** t64, float2** t32, hipStream_t st) {
+    std::vector<double2> h64(K2_TAB_N);
+    std::vector<float2> h32(K2_TAB_N);
+    sincos_tables(K2_TAB_N, h64.data(), h32.data());
+    hipError_t e = hipMalloc((void**)t64, K2_TAB_N * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc((void**)t32, K2_TAB_N * sizeof(float2));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(*t64, h64.data(), K2_TAB_N * sizeof(double2), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(*t32, h32.data(), K2_TAB_N * sizeof(float2), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? K2_OK : fail(K2_E_HIP, std::string("sin/cos table: ") + hipGetErrorString(e));
+}
 
 static thread_local std::string g_err;
 
@@ -43,6 +71,8 @@ struct k2_table {
     uint4* g = nullptr;
     double* gamma = nullptr;
     double* w = nullptr;
+    double2* tab64 = nullptr;   // (cos, sin)(2 pi k / K2_TAB_N)
+    float2* tab32 = nullptr;
     std::vector<WeightArrays> wsets;
     uint64_t use_clock = 0;
     double* partial = nullptr;
@@ -103,6 +133,11 @@ extern "C" int k2_table_create(k2_table** out, uint32_t n_zeros, const uint64_t*
         k2_table_destroy(t);
         return fail(K2_E_HIP, std::string("k2_table_create: ") + hipGetErrorString(e));
     }
+    if (upload_tables(&t->tab64, &t->tab32, t->stream) != K2_OK) {
+        std::string msg = g_err;
+        k2_table_destroy(t);
+        return fail(K2_E_HIP, msg);
+    }
     *out = t;
     return K2_OK;
 }
@@ -118,6 +153,7 @@ extern "C" int k2_table_destroy(k2_table* t) {
     for (auto& pp : t->prof_pending) { (void)hipEventDestroy(pp.a); (void)hipEventDestroy(pp.b); }
     for (auto ev : t->ev_pool) (void)hipEventDestroy(ev);
     (void)hipFree(t->g); (void)hipFree(t->gamma); (void)hipFree(t->w);
+    (void)hipFree(t->tab64); (void)hipFree(t->tab32);
     (void)hipFree(t->partial); (void)hipFree(t->host_io); (void)hipFree(t->polish_active);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
@@ -146,8 +182,9 @@ extern "C" int k2_get_tiling(const k2_table* t, k2_tiling* ti) {
     return K2_OK;
 }
 
-// weight arrays for k, computed on the table stream on first use (LRU of K2_MAX_WEIGHT_SETS)
-static int get_weights(k2_table* t, double k, WeightArrays** out) {
+// weight arrays for k, computed on stream st on first use and waited for, so that work on any
+// stream may read them afterwards (LRU of K2_MAX_WEIGHT_SETS)
+static int get_weights(k2_table* t, double k, hipStream_t st, WeightArrays** out) {
     t->use_clock++;
     for (auto& ws : t->wsets)
         if (ws.k == k) { ws.last_use = t->use_clock; *out = &ws; return K2_OK; }
@@ -177,7 +214,8 @@ static int get_weights(k2_table* t, double k, WeightArrays** out) {
     }
     slot->k = k;
     slot->last_use = t->use_clock;
-    HIPCHK(launch_weights(t->gamma, t->w, (int)t->n, (int)t->n_pad, k, *slot, t->stream));
+    HIPCHK(launch_weights(t->gamma, t->w, (int)t->n, (int)t->n_pad, k, *slot, st));
+    HIPCHK(hipStreamSynchronize(st));
     *out = slot;
     return K2_OK;
 }
@@ -250,6 +288,8 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
     a.n_ctiles = p.n_ctiles;
     a.partial = t->partial;
     a.n_cand_pad = p.n_cand_pad;
+    a.tab64 = t->tab64;
+    a.tab32 = t->tab32;
     ProfPair pp{nullptr, nullptr};
     if (t->profile) {
         pp.a = pool_event(t);
@@ -302,17 +342,10 @@ extern "C" int k2_eval_device(k2_table* t, uint32_t n_cand, const uint64_t* V_de
     if (n_cand == 0) return K2_OK;
     std::lock_guard<std::mutex> lk(t->mu);
     HIPCHK(hipSetDevice(t->device));
-    hipStream_t st = stream ? (hipStream_t)stream : t->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL is the default stream
     WeightArrays* ws = nullptr;
-    if (st != t->stream) {
-        // weights are produced on the table stream; order the caller's stream after them
-        rc = get_weights(t, k, &ws);
-        if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(t->stream));
-    } else {
-        rc = get_weights(t, k, &ws);
-        if (rc) return rc;
-    }
+    rc = get_weights(t, k, st, &ws);
+    if (rc) return rc;
     return eval_core(t, (int)n_cand, (const uint4*)V_dev, ws, (int)n_zeros, flags, K_dev, dK_dev,
                      d2K_dev, st);
 }
@@ -337,7 +370,7 @@ extern "C" int k2_eval(k2_table* t, uint32_t n_cand, const uint64_t* V, double k
     double* dK2 = dK1 + n_cand;
     HIPCHK(hipMemcpyAsync(dV, V, nb_v, hipMemcpyHostToDevice, t->stream));
     WeightArrays* ws = nullptr;
-    rc = get_weights(t, k, &ws);
+    rc = get_weights(t, k, t->stream, &ws);
     if (rc) return rc;
     rc = eval_core(t, (int)n_cand, dV, ws, (int)n_zeros, flags, dK0, dK1, dK2, t->stream);
     if (rc) return rc;
@@ -379,7 +412,7 @@ extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, d
     if (n_steps < 0 || !(clamp > 0.0) || clamp > 0.25)
         return fail(K2_E_ARG, "k2_polish_device: n_steps >= 0 and 0 < clamp <= 0.25 required");
     if (n_cand == 0) return K2_OK;
-    hipStream_t st = stream ? (hipStream_t)stream : t->stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL is the default stream
     const unsigned fl = (flags & K2_MIXED) | K2_VALUE | K2_D1 | K2_D2;
     {
         std::lock_guard<std::mutex> lk(t->mu);
@@ -428,15 +461,22 @@ extern "C" int k2_sincos_words(uint32_t n, const uint64_t* W, int mode, double* 
     if (n && (!W || !out)) return fail(K2_E_ARG, "k2_sincos_words: null pointer");
     if (n == 0) return K2_OK;
     if (n > (1u << 28)) return fail(K2_E_ARG, "k2_sincos_words: n too large");
+    if (mode < 0 || mode > 1) return fail(K2_E_ARG, "k2_sincos_words: mode must be 0 or 1");
     uint64_t* dW = nullptr;
     double* dO = nullptr;
-    HIPCHK(hipMalloc((void**)&dW, (size_t)n * 8));
-    hipError_t e = hipMalloc((void**)&dO, (size_t)n * 16);
+    double2* t64 = nullptr;
+    float2* t32 = nullptr;
+    int rc = upload_tables(&t64, &t32, nullptr);
+    if (rc) return rc;
+    hipError_t e = hipMalloc((void**)&dW, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&dO, (size_t)n * 16);
     if (e == hipSuccess) e = hipMemcpy(dW, W, (size_t)n * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = launch_sincos_words(dW, (int)n, mode, dO, nullptr);
+    if (e == hipSuccess) e = launch_sincos_words(dW, (int)n, mode, t64, t32, dO, nullptr);
     if (e == hipSuccess) e = hipMemcpy(out, dO, (size_t)n * 16, hipMemcpyDeviceToHost);
     (void)hipFree(dW);
     (void)hipFree(dO);
+    (void)hipFree(t64);
+    (void)hipFree(t32);
     if (e != hipSuccess) return fail(K2_E_HIP, std::string("k2_sincos_words: ") + hipGetErrorString(e));
     return K2_OK;
 }

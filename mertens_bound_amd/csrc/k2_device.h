@@ -7,9 +7,8 @@
 // bits [98,110) of the rest of the product to be all ones; that case (probability 2^-12) is
 // detected and recomputed exactly, so W is bit-exact for every input.
 //
-// sincos: the phase is in turns, so the quadrant comes from the top two bits of W and the
-// reduced argument |x| <= pi/4 from the remaining 62 bits, with no floating-point range
-// reduction.  sin and cos are Taylor polynomials on [-pi/4, pi/4] (truncation error < 2e-18).
+// sincos: the phase is in turns, so the split into a table point and a small remainder comes
+// straight from the integer word, with no floating-point range reduction.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -145,47 +144,41 @@ __device__ __forceinline__ void sincos_poly(double x, double& s, double& c) {
     c = fma(z, pc, 1.0);
 }
 
-// sin and cos of the angle 2 pi W / 2^64
-__device__ __forceinline__ void sincos_turns64(uint64_t W, double& s, double& c) {
-    const uint32_t wlo = (uint32_t)W, whi = (uint32_t)(W >> 32);
-    const uint32_t q = (whi + 0x20000000u) >> 30;                  // nearest quarter turn
-    const int32_t rhi = (int32_t)(whi - (q << 30));                // r in [-2^61, 2^61)
+// ---- table-driven sin/cos used by the evaluation kernels ------------------------------------
+// The angle 2 pi W / 2^64 is split into a table point 2 pi k / 512 (k = W rounded to 9 bits) and
+// a remainder |x| <= pi/512 = 6.2e-3.  cos/sin of the table points sit in LDS (K2_TAB entries);
+// the remainder needs only x - x^3/6 + x^5/120 (truncation < 1.1e-17 relative) and
+// 1 - x^2/2 + x^4/24 (truncation < 7.5e-17), and one complex rotation combines the two.
+// This replaces the quadrant reduction (no selects) and halves the polynomial work.
+#define K2_TAB 512
+#define K2_TAB_SHIFT 23   // 64-bit word: index bits 63..55 -> high word bits 31..23
+
+__device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const double2* tab,
+                                             double& S, double& C) {
+    const uint32_t k = ((whi + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT) & (K2_TAB - 1);
+    const int32_t rhi = (int32_t)(whi - (k << K2_TAB_SHIFT));      // r in [-2^54, 2^54)
     const double r = fma((double)rhi, 4294967296.0, (double)wlo);
     const double x = r * K2_TWO_PI_2M64;
-    double sp, cp;
-    sincos_poly(x, sp, cp);
-    // e^{i(q pi/2 + x)} = i^q (cp + i sp)
-    const double cs = (q & 1) ? -sp : cp;
-    const double sn = (q & 1) ? cp : sp;
-    const bool neg = (q & 2) != 0;
-    c = neg ? -cs : cs;
-    s = neg ? -sn : sn;
+    const double z = x * x;
+    const double sp = fma(x * z, fma(z, 8.3333333333333333e-03, -1.6666666666666666e-01), x);
+    const double cp = fma(z, fma(z, 4.1666666666666664e-02, -0.5), 1.0);
+    const double2 t = tab[k];                                        // (cos, sin)(2 pi k / 512)
+    C = fma(t.x, cp, -t.y * sp);
+    S = fma(t.y, cp, t.x * sp);
 }
 
-// ---- FP32 sin/cos of 2 pi W32 / 2^32 (precision form) ---------------------------------------
-__device__ __forceinline__ void sincos_turns32(uint32_t W, float& s, float& c) {
-    const uint32_t q = (W + 0x20000000u) >> 30;
-    const int32_t r = (int32_t)(W - (q << 30));                    // in [-2^29, 2^29)
-    const float x = (float)r * 1.4629181e-09f;            // 2 pi / 2^32
+// FP32 form: W32 = top 32 bits of the phase word; |x| <= pi/512, x - x^3/6 and 1 - x^2/2 are
+// exact to FP32 rounding (next terms < 2e-11 relative)
+__device__ __forceinline__ void sincos_tab32(uint32_t W, const float2* tab, float& S, float& C) {
+    const uint32_t k = ((W + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT) & (K2_TAB - 1);
+    const int32_t r = (int32_t)(W - (k << K2_TAB_SHIFT));           // |r| <= 2^22: exact in FP32
+    const float x = (float)r * 1.4629181e-09f;                       // 2 pi / 2^32
     const float z = x * x;
-    // Taylor on [-pi/4, pi/4]: sin to x^11 (trunc < 3e-10), cos to x^10 (trunc < 2e-9 rel 1)
-    float ps = -2.5052108e-08f;
-    ps = fmaf(ps, z, 2.7557319e-06f);
-    ps = fmaf(ps, z, -1.9841270e-04f);
-    ps = fmaf(ps, z, 8.3333333e-03f);
-    ps = fmaf(ps, z, -1.6666667e-01f);
-    const float sp = fmaf(x * z, ps, x);
-    float pc = -2.7557319e-07f;
-    pc = fmaf(pc, z, 2.4801587e-05f);
-    pc = fmaf(pc, z, -1.3888889e-03f);
-    pc = fmaf(pc, z, 4.1666668e-02f);
-    pc = fmaf(pc, z, -0.5f);
-    const float cp = fmaf(z, pc, 1.0f);
-    const float cs = (q & 1) ? -sp : cp;
-    const float sn = (q & 1) ? cp : sp;
-    const bool neg = (q & 2) != 0;
-    c = neg ? -cs : cs;
-    s = neg ? -sn : sn;
+    const float sp = fmaf(x * z, -1.6666667e-01f, x);
+    const float cp = fmaf(z, -0.5f, 1.0f);
+    const float2 t = tab[k];
+    C = fmaf(t.x, cp, -t.y * sp);
+    S = fmaf(t.y, cp, t.x * sp);
 }
 
 // ---- general FP64 sincos (|x| < 2^20 pi/2) for the reference-format entry points -----------

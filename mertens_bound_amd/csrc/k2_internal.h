@@ -34,7 +34,12 @@ struct LaunchArgs {
     int n_cand = 0, n_ctiles = 0;
     double* partial = nullptr;
     int split_base = 0, n_cand_pad = 0;
+    const double2* tab64 = nullptr;   // (cos, sin)(2 pi k / K2_TAB)
+    const float2* tab32 = nullptr;
 };
+
+// host-side (cos, sin)(2 pi k / n) tables, rounded from long double
+void sincos_tables(int n, double2* t64, float2* t32);
 
 hipError_t launch_eval(const LaunchArgs& a, bool mixed, hipStream_t st);
 hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,
@@ -43,7 +48,8 @@ hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_
                          unsigned flags, double* K, double* dK, double* d2K, hipStream_t st);
 hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand, int n_cand,
                               int mode, uint64_t* out, hipStream_t st);
-hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, double* out, hipStream_t st);
+hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const double2* tab64,
+                               const float2* tab32, double* out, hipStream_t st);
 hipError_t launch_newton(uint4* cand, const double* dK, const double* d2K, int n, double clamp,
                          int* active, int* steps, hipStream_t st);
 hipError_t launch_ref_peak_ab(int n_peaks, const uint64_t* const* p, const double* dy0,

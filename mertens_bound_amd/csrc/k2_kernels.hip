@@ -57,9 +57,11 @@ __global__ __launch_bounds__(256) void k2_plain_kernel(
     const uint4* __restrict__ zg, const double2* __restrict__ za, const double2* __restrict__ zb,
     const double2* __restrict__ ze, int z_begin, int z_end, int zeros_per_split,
     const uint4* __restrict__ cand, int n_cand, int n_ctiles,
-    double* __restrict__ partial, int split_base, int n_cand_pad) {
+    double* __restrict__ partial, int split_base, int n_cand_pad,
+    const double2* __restrict__ tab) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
+    __shared__ double2 s_tab[K2_TAB];
     __shared__ uint4 s_g[2][K2_ZT];
     __shared__ double2 s_a[2][K2_ZT];
     __shared__ double2 s_b[2][D1 ? K2_ZT : 1];
@@ -71,6 +73,7 @@ __global__ __launch_bounds__(256) void k2_plain_kernel(
     const int z1 = min(z_end, z0 + zeros_per_split);
     const int tid = threadIdx.x;
 
+    for (int i = tid; i < K2_TAB; i += K2_ZT) s_tab[i] = tab[i];
     Limbs4 v[NC];
     double acc0[NC], acc1[NC], acc2[NC];
 #pragma unroll
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(256) void k2_plain_kernel(
             for (int i = 0; i < NC; ++i) {
                 const uint64_t W = phase_word(g, v[i]);
                 double s, c;
-                sincos_turns64(W, s, c);
+                sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), s_tab, s, c);
                 acc0[i] = fma(aa.x, c, acc0[i]);
                 acc0[i] = fma(-aa.y, s, acc0[i]);
                 if (D1) {
@@ -159,9 +162,11 @@ __global__ __launch_bounds__(256) void k2_mixed_kernel(
     const uint4* __restrict__ zg, const float2* __restrict__ za, const float2* __restrict__ zb,
     const float2* __restrict__ ze, int z_begin, int z_end, int zeros_per_split,
     const uint4* __restrict__ cand, int n_cand, int n_ctiles,
-    double* __restrict__ partial, int split_base, int n_cand_pad) {
+    double* __restrict__ partial, int split_base, int n_cand_pad,
+    const float2* __restrict__ tab) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
+    __shared__ float2 s_tab[K2_TAB];
     __shared__ uint4 s_g[2][K2_ZT];
     __shared__ float4 s_ab[2][K2_ZT];
     __shared__ float2 s_e[2][D2 ? K2_ZT : 1];
@@ -172,6 +177,7 @@ __global__ __launch_bounds__(256) void k2_mixed_kernel(
     const int z1 = min(z_end, z0 + zeros_per_split);
     const int tid = threadIdx.x;
 
+    for (int i = tid; i < K2_TAB; i += K2_ZT) s_tab[i] = tab[i];
     Limbs4 v[NC];
     float s0[NC], c0[NC], s1[NC], c1[NC], s2[NC], c2[NC];
 #pragma unroll
@@ -220,7 +226,7 @@ __global__ __launch_bounds__(256) void k2_mixed_kernel(
             for (int i = 0; i < NC; ++i) {
                 const uint32_t W = phase_word_hi32(g, v[i]);
                 float s, c;
-                sincos_turns32(W, s, c);
+                sincos_tab32(W, s_tab, s, c);
                 kahan_add(s0[i], c0[i], fmaf(ab.x, c, -ab.y * s));
                 if (D1) kahan_add(s1[i], c1[i], -fmaf(ab.z, s, ab.w * c));
                 if (D2) kahan_add(s2[i], c2[i], fmaf(ee.y, s, -ee.x * c));
@@ -272,23 +278,26 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     uint64_t W;
     if (mode == 0) W = phase_word(g, v);
     else if (mode == 1) W = phase_word_exact(g, v);
-    else W = phase_word_hi32(g, v);
+    else W = (uint64_t)phase_word_hi32(g, v) << 32;
     out[idx] = W;
 }
 
-// ---- debug: FP64 / FP32 sin,cos of phase words ------------------------------------------------
+// ---- debug: the evaluators' sin/cos of phase words ---------------------------------------------
+// mode 0: FP64 table path of the plain kernel, 1: FP32 table path of the mixed kernel (top 32 bits)
 __global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __restrict__ W,
                                                               int n, int mode,
+                                                              const double2* __restrict__ tab64,
+                                                              const float2* __restrict__ tab32,
                                                               double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (mode == 0) {
         double s, c;
-        sincos_turns64(W[i], s, c);
+        sincos_tab64((uint32_t)W[i], (uint32_t)(W[i] >> 32), tab64, s, c);
         out[2 * i] = s; out[2 * i + 1] = c;
     } else {
         float s, c;
-        sincos_turns32((uint32_t)(W[i] >> 32), s, c);
+        sincos_tab32((uint32_t)(W[i] >> 32), tab32, s, c);
         out[2 * i] = s; out[2 * i + 1] = c;
     }
 }
@@ -350,7 +359,7 @@ template <int NC, int FL>
 static hipError_t launch_plain_t(const LaunchArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((k2_plain_kernel<NC, FL>), dim3(a.n_blocks), dim3(K2_ZT), 0, st,
                        a.zg, a.za, a.zb, a.ze, a.z_begin, a.z_end, a.zeros_per_split, a.cand,
-                       a.n_cand, a.n_ctiles, a.partial, a.split_base, a.n_cand_pad);
+                       a.n_cand, a.n_ctiles, a.partial, a.split_base, a.n_cand_pad, a.tab64);
     return hipGetLastError();
 }
 
@@ -358,7 +367,7 @@ template <int NC, int FL>
 static hipError_t launch_mixed_t(const LaunchArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((k2_mixed_kernel<NC, FL>), dim3(a.n_blocks), dim3(K2_ZT), 0, st,
                        a.zg, a.zaf, a.zbf, a.zef, a.z_begin, a.z_end, a.zeros_per_split, a.cand,
-                       a.n_cand, a.n_ctiles, a.partial, a.split_base, a.n_cand_pad);
+                       a.n_cand, a.n_ctiles, a.partial, a.split_base, a.n_cand_pad, a.tab32);
     return hipGetLastError();
 }
 
@@ -417,9 +426,10 @@ hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand
     return hipGetLastError();
 }
 
-hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, double* out, hipStream_t st) {
+hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const double2* tab64,
+                               const float2* tab32, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k2_sincos_words_kernel, dim3((n + 255) / 256), dim3(256), 0, st, W, n,
-                       mode, out);
+                       mode, tab64, tab32, out);
     return hipGetLastError();
 }
 

@@ -25,10 +25,13 @@ def test_compute_peak_AB(oracle_lib):
     npk, nz = A0.shape
     A = np.empty_like(A0); B = np.empty_like(B0)
     ev.compute_peak_AB(npk, *args[:3], nz, *args[3:], A, B)
-    tol = 6e-16 * 2 * rw["alpha"][None, :] + 1e-300
+    # the argument th = top64 2pi/2^64 + g dy0 is rounded once (|th| <= 2pi + g |dy0|), so the
+    # two builds may differ by one ulp of th on top of the sin/cos rounding
+    th_max = 2 * np.pi + rw["gamma"][None, :] * np.max(np.abs(args[2]))
+    tol = 2 * rw["alpha"][None, :] * (th_max * 2.3e-16 + 4.5e-16)
     assert np.all(np.abs(A - A0) <= tol) and np.all(np.abs(B - B0) <= tol)
     g = golden("ref_compat.npz")
-    np.testing.assert_allclose(A[:, :8], g["A_head"], rtol=0, atol=1e-16)
+    np.testing.assert_allclose(A[:, :8], g["A_head"], rtol=0, atol=3e-16)
     np.testing.assert_allclose(A.sum(axis=1), g["A_rowsum"], rtol=0, atol=1e-13)
 
 
