@@ -88,10 +88,45 @@ int k2_eval_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double k
                    double *d2K_dev, void *stream);
 
 /* Uneven requests in one call (device buffers): requests share the table, each has its own
- * zero cutoff, k and candidate slice.  Outputs are indexed like V (cand_offset + i). */
+ * zero cutoff, k and candidate slice (slices may not overlap).  Outputs are indexed like V
+ * (cand_offset + i); entries outside every request are left untouched.  One persistent launch
+ * (k2_batch_create + k2_batch_run_direct + k2_batch_destroy).  No counterpart in the reference,
+ * whose drivers loop over weight vectors and cutoffs one call at a time
+ * (search/eval_tech1_600k.py:252-267). */
 int k2_eval_batched_device(k2_table *t, uint32_t n_req, const k2_request *req_host,
                            const uint64_t *V_dev, uint64_t n_cand_total, uint32_t flags,
                            double *K_dev, double *dK_dev, double *d2K_dev, void *stream);
+
+/* Batched plans: the work items of a request set, bound to fixed device buffers, with the
+ * launches captured once in a HIP graph and replayed by k2_batch_run. */
+typedef struct k2_batch k2_batch;
+
+typedef struct k2_batch_tiling {
+    int32_t nc;          /* candidates per lane in C tiles (256*nc per tile): 2 or 4 */
+    int32_t cz;          /* candidates per Z tile: 8 or 16 */
+    int32_t chunk;       /* zeros per C item; Z items take chunk*256*nc/cz */
+    int32_t c_min_fill;  /* leftover candidates of a request get a C tile from this many on,
+                            else Z tiles */
+    int32_t wg_per_cu;   /* resident workgroups per CU of the persistent grid (0 = occupancy) */
+} k2_batch_tiling;
+
+typedef struct k2_batch_info {
+    int64_t n_items;
+    int32_t grid;
+    int32_t n_k;
+    double useful_evals;   /* sum over requests of n_cand * n_zeros */
+    double padded_evals;   /* evaluations issued, idle lanes of partial tiles included */
+    int64_t partial_bytes;
+} k2_batch_info;
+
+int k2_batch_create(k2_table *t, uint32_t n_req, const k2_request *req_host,
+                    const uint64_t *V_dev, uint64_t n_cand_total, uint32_t flags,
+                    const k2_batch_tiling *tiling, double *K_dev, double *dK_dev, double *d2K_dev,
+                    k2_batch **out);
+int k2_batch_run(k2_batch *b, void *stream);         /* HIP graph replay */
+int k2_batch_run_direct(k2_batch *b, void *stream);  /* the same launches, no graph */
+int k2_batch_get_info(const k2_batch *b, k2_batch_info *info);
+int k2_batch_destroy(k2_batch *b);
 
 /* Newton polish: n_steps Newton iterations on K2' = 0 from each V (device buffers, V updated in
  * place).  Step rule of path1_eval_fast.cpp:120-128 (|K''| > 1e-12 guard, step = K'/K''

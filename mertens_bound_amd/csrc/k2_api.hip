@@ -33,7 +33,7 @@ void k2impl::sincos_tables(int n, double2* t64, float2* t32) {
     }
 }
 
-static int upload_tables(double2** t64, float2** t32, hipStream_t st) {
+int k2impl::upload_tables(double2** t64, float2** t32, hipStream_t st) {
     std::vector<double2> h64(K2_TAB_N);
     std::vector<float2> h32(K2_TAB_N);
     sincos_tables(K2_TAB_N, h64.data(), h32.data());
@@ -49,10 +49,12 @@ static int upload_tables(double2** t64, float2** t32, hipStream_t st) {
 
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+int k2impl::set_error(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+
+static int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
 #define HIPCHK(expr)                                                                       \
     do {                                                                                   \
@@ -60,36 +62,6 @@ static int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                              \
             return fail(K2_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
     } while (0)
-
-struct ProfPair {
-    hipEvent_t a, b;
-};
-
-struct k2_table {
-    int device = 0;
-    uint32_t n = 0, n_pad = 0;
-    uint4* g = nullptr;
-    double* gamma = nullptr;
-    double* w = nullptr;
-    double2* tab64 = nullptr;   // (cos, sin)(2 pi k / K2_TAB_N)
-    float2* tab32 = nullptr;
-    std::vector<WeightArrays> wsets;
-    uint64_t use_clock = 0;
-    double* partial = nullptr;
-    size_t partial_bytes = 0;
-    void* host_io = nullptr;   // device staging for host-buffer entry points
-    size_t host_io_bytes = 0;
-    int* polish_active = nullptr;
-    size_t polish_cap = 0;
-    hipStream_t stream = nullptr;
-    k2_tiling tiling{4, 2048, 8192};
-    bool profile = false;
-    std::vector<ProfPair> prof_pending;
-    std::vector<hipEvent_t> ev_pool;
-    double prof_ms = 0.0;
-    int64_t prof_launches = 0;
-    std::mutex mu;
-};
 
 extern "C" const char* k2_last_error(void) { return g_err.c_str(); }
 extern "C" int k2_version(void) { return K2_VERSION; }
@@ -146,10 +118,7 @@ extern "C" int k2_table_destroy(k2_table* t) {
     if (!t) return K2_OK;
     (void)hipSetDevice(t->device);
     if (t->stream) (void)hipStreamSynchronize(t->stream);
-    for (auto& ws : t->wsets) {
-        (void)hipFree(ws.a); (void)hipFree(ws.b); (void)hipFree(ws.e);
-        (void)hipFree(ws.af); (void)hipFree(ws.bf); (void)hipFree(ws.ef);
-    }
+    for (auto& ws : t->wsets) free_weights(ws);
     for (auto& pp : t->prof_pending) { (void)hipEventDestroy(pp.a); (void)hipEventDestroy(pp.b); }
     for (auto ev : t->ev_pool) (void)hipEventDestroy(ev);
     (void)hipFree(t->g); (void)hipFree(t->gamma); (void)hipFree(t->w);
@@ -182,6 +151,27 @@ extern "C" int k2_get_tiling(const k2_table* t, k2_tiling* ti) {
     return K2_OK;
 }
 
+int k2impl::alloc_weights(WeightArrays& w, size_t np) {
+    hipError_t e = hipMalloc((void**)&w.a, np * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.b, np * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.e, np * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.af, np * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.bf, np * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.ef, np * sizeof(float2));
+    if (e != hipSuccess) {
+        free_weights(w);
+        return fail(K2_E_NOMEM, "weight arrays: hipMalloc failed");
+    }
+    return K2_OK;
+}
+
+void k2impl::free_weights(WeightArrays& w) {
+    (void)hipFree(w.a); (void)hipFree(w.b); (void)hipFree(w.e);
+    (void)hipFree(w.af); (void)hipFree(w.bf); (void)hipFree(w.ef);
+    w.a = w.b = w.e = nullptr;
+    w.af = w.bf = w.ef = nullptr;
+}
+
 // weight arrays for k, computed on stream st on first use and waited for, so that work on any
 // stream may read them afterwards (LRU of K2_MAX_WEIGHT_SETS)
 static int get_weights(k2_table* t, double k, hipStream_t st, WeightArrays** out) {
@@ -190,22 +180,11 @@ static int get_weights(k2_table* t, double k, hipStream_t st, WeightArrays** out
         if (ws.k == k) { ws.last_use = t->use_clock; *out = &ws; return K2_OK; }
     WeightArrays* slot = nullptr;
     if ((int)t->wsets.size() < K2_MAX_WEIGHT_SETS) {
-        t->wsets.emplace_back();
+        WeightArrays w;
+        int rc = alloc_weights(w, t->n_pad);
+        if (rc) return rc;
+        t->wsets.push_back(w);
         slot = &t->wsets.back();
-        const size_t np = t->n_pad;
-        hipError_t e = hipMalloc((void**)&slot->a, np * sizeof(double2));
-        if (e == hipSuccess) e = hipMalloc((void**)&slot->b, np * sizeof(double2));
-        if (e == hipSuccess) e = hipMalloc((void**)&slot->e, np * sizeof(double2));
-        if (e == hipSuccess) e = hipMalloc((void**)&slot->af, np * sizeof(float2));
-        if (e == hipSuccess) e = hipMalloc((void**)&slot->bf, np * sizeof(float2));
-        if (e == hipSuccess) e = hipMalloc((void**)&slot->ef, np * sizeof(float2));
-        if (e != hipSuccess) {
-            WeightArrays bad = *slot;
-            t->wsets.pop_back();
-            (void)hipFree(bad.a); (void)hipFree(bad.b); (void)hipFree(bad.e);
-            (void)hipFree(bad.af); (void)hipFree(bad.bf); (void)hipFree(bad.ef);
-            return fail(K2_E_NOMEM, "weight arrays: hipMalloc failed");
-        }
     } else {
         slot = &*std::min_element(t->wsets.begin(), t->wsets.end(),
                                   [](const WeightArrays& x, const WeightArrays& y) {
@@ -378,28 +357,6 @@ extern "C" int k2_eval(k2_table* t, uint32_t n_cand, const uint64_t* V, double k
     if (flags & K2_D1) HIPCHK(hipMemcpyAsync(dK, dK1, nb_o, hipMemcpyDeviceToHost, t->stream));
     if (flags & K2_D2) HIPCHK(hipMemcpyAsync(d2K, dK2, nb_o, hipMemcpyDeviceToHost, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
-    return K2_OK;
-}
-
-extern "C" int k2_eval_batched_device(k2_table* t, uint32_t n_req, const k2_request* req,
-                                      const uint64_t* V_dev, uint64_t n_cand_total, uint32_t flags,
-                                      double* K_dev, double* dK_dev, double* d2K_dev, void* stream) {
-    if (!t || (n_req > 0 && (!req || !V_dev || !K_dev)))
-        return fail(K2_E_ARG, "k2_eval_batched_device: null pointer");
-    for (uint32_t r = 0; r < n_req; ++r) {
-        if (req[r].n_zeros > t->n || req[r].cand_offset + req[r].n_cand > n_cand_total)
-            return fail(K2_E_ARG, "k2_eval_batched_device: request " + std::to_string(r) +
-                                      " out of range");
-    }
-    for (uint32_t r = 0; r < n_req; ++r) {
-        if (req[r].n_cand == 0) continue;
-        const uint64_t o = req[r].cand_offset;
-        int rc = k2_eval_device(t, req[r].n_cand, V_dev + 2 * o, req[r].k,
-                                req[r].n_zeros ? req[r].n_zeros : t->n, flags, K_dev + o,
-                                dK_dev ? dK_dev + o : nullptr, d2K_dev ? d2K_dev + o : nullptr,
-                                stream);
-        if (rc) return rc;
-    }
     return K2_OK;
 }
 

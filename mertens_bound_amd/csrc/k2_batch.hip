@@ -1,0 +1,476 @@
+// Batched evaluation of uneven requests in one persistent launch, captured in a HIP graph.
+//
+// A request r has its own zero cutoff (the first n_r zeros), Gaussian constant k_r and candidate
+// slice.  The host cuts every request into work items:
+//   * C items: a tile of 256*NC candidates (NC per lane) x a chunk of zeros, the same body as the
+//     one-launch kernel;
+//   * Z items: a tile of CZ candidates x a longer chunk of zeros, one zero per lane at a time,
+//     for the candidates left over once a request's full C tiles are taken (a C tile would leave
+//     most of its lanes idle);
+//   * with K2_MIXED, the zeros below the FP64 head and the FP32 tail are separate items.
+// Items are sorted longest first and pulled from an atomic counter by a grid of resident
+// workgroups; each item writes its partial sums to its own slot, and a reduction kernel adds a
+// candidate's slots in zero order, so results do not depend on the execution order.
+// The graph holds [counter reset kernel, persistent kernel, reduction kernel]: no memset nodes.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "k2_device.h"
+#include "k2_internal.h"
+#include "k2_tiles.h"
+
+using namespace k2dev;
+
+struct BatchItem {
+    uint32_t flags;     // bit0: Z tile, bit1: FP32, bits 8-9: k slot
+    uint32_t z0, z1;
+    uint32_t c_first;   // global index of the tile's first candidate
+    uint32_t c_count;
+    uint32_t pad;
+    uint64_t poff;      // partial[poff + q * width + pos]
+};
+
+struct CandDesc {
+    uint64_t base;      // first item slot of the candidate's tile
+    uint32_t n_items;   // items of the tile (zero chunks)
+    uint16_t width;     // candidates per slot row (256*NC or CZ)
+    uint16_t pos;
+};
+
+struct BatchViews {
+    ZeroView zv[4];
+};
+
+__global__ void k2_counter_reset_kernel(int* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+template <int NC, int CZ, int FL, bool ALLOW32>
+__global__ __launch_bounds__(256) void k2_batch_kernel(
+    BatchViews views, const BatchItem* __restrict__ items, int n_items, int* counter,
+    const uint4* __restrict__ cand, double* __restrict__ partial,
+    const double2* __restrict__ tab64, const float2* __restrict__ tab32) {
+    __shared__ CTileSmem sm;
+    __shared__ ZTileSmem<CZ> zsm;
+    __shared__ double2 s_tab64[K2_TAB];
+    __shared__ float2 s_tab32[ALLOW32 ? K2_TAB : 1];
+    __shared__ int s_item;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < K2_TAB; i += K2_WG) {
+        s_tab64[i] = tab64[i];
+        if (ALLOW32) s_tab32[i] = tab32[i];
+    }
+    __syncthreads();
+    constexpr int CW = K2_WG * NC;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const BatchItem I = items[it];
+        const ZeroView zv = views.zv[(I.flags >> 8) & 3];
+        const bool f32 = ALLOW32 && (I.flags & 2);
+        if (I.flags & 1) {
+            double* out = partial + I.poff;
+            if (f32) {
+                if constexpr (ALLOW32)
+                    ztile<CZ, FL, true>(zv, I.z0, I.z1, cand, I.c_first, I.c_count, out, zsm,
+                                        s_tab64, s_tab32);
+            } else {
+                ztile<CZ, FL, false>(zv, I.z0, I.z1, cand, I.c_first, I.c_count, out, zsm, s_tab64,
+                                     s_tab32);
+            }
+            continue;
+        }
+        Limbs4 v[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const uint32_t c = i * K2_WG + tid;
+            uint4 cv = make_uint4(0, 0, 0, 0);
+            if (c < I.c_count) cv = cand[I.c_first + c];
+            v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
+        }
+        double* out = partial + I.poff;
+        if (f32) {
+            if constexpr (ALLOW32) {
+                Acc<true> acc[NC];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) acc[i].zero();
+                ctile_f32<NC, FL>(zv, I.z0, I.z1, v, acc, sm, s_tab32);
+#pragma unroll
+                for (int i = 0; i < NC; ++i)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) out[q * CW + i * K2_WG + tid] = acc[i].get(q);
+            }
+        } else {
+            Acc<false> acc[NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) acc[i].zero();
+            ctile_f64<NC, FL>(zv, I.z0, I.z1, v, acc, sm, s_tab64);
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) out[q * CW + i * K2_WG + tid] = acc[i].get(q);
+        }
+    }
+}
+
+// K[c] = sum over the candidate's tile items of its slot, in zero order
+__global__ __launch_bounds__(256) void k2_batch_reduce_kernel(
+    const CandDesc* __restrict__ desc, uint64_t n, const double* __restrict__ partial,
+    double* __restrict__ K, double* __restrict__ dK, double* __restrict__ d2K) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const CandDesc d = desc[c];
+    if (d.n_items == 0) return;   // not part of any request
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const uint64_t stride = 3ull * d.width;
+    for (uint32_t h = 0; h < d.n_items; ++h) {
+        const double* row = partial + d.base + h * stride + d.pos;
+        s0 += row[0];
+        s1 += row[d.width];
+        s2 += row[2 * d.width];
+    }
+    K[c] = s0;
+    if (dK) dK[c] = s1;
+    if (d2K) d2K[c] = s2;
+}
+
+// ---- host side ------------------------------------------------------------------------------
+
+namespace k2impl {
+
+hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,
+                          WeightArrays& out, hipStream_t st);
+
+struct BatchPlan {
+    std::vector<BatchItem> items;
+    std::vector<CandDesc> desc;
+    uint64_t partial_doubles = 0;
+    double useful_evals = 0, padded_evals = 0;
+};
+
+// requests -> items and candidate descriptors (see the file comment)
+std::string plan_batch(uint32_t n_req, const k2_request* req, uint64_t n_cand_total,
+                       uint32_t table_n, const std::vector<double>& kslots, bool mixed,
+                       int mixed_head, const k2_batch_tiling& ti, BatchPlan& P) {
+    const int NC = ti.nc, CZ = ti.cz;
+    const uint32_t CW = 256u * NC;
+    P.desc.assign(n_cand_total, CandDesc{0, 0, 0, 0});
+    std::vector<uint8_t> seen(n_cand_total, 0);
+    struct Tile { uint32_t c_first, c_count, width; bool z; };
+    for (uint32_t r = 0; r < n_req; ++r) {
+        const k2_request& q = req[r];
+        if (q.n_cand == 0) continue;
+        const uint32_t nz = q.n_zeros ? q.n_zeros : table_n;
+        if (nz > table_n) return "request " + std::to_string(r) + ": n_zeros exceeds the table";
+        if (q.cand_offset + q.n_cand > n_cand_total)
+            return "request " + std::to_string(r) + ": candidates out of range";
+        for (uint64_t c = q.cand_offset; c < q.cand_offset + q.n_cand; ++c) {
+            if (seen[c]) return "request " + std::to_string(r) + ": candidates overlap another request";
+            seen[c] = 1;
+        }
+        const int ks = (int)(std::find(kslots.begin(), kslots.end(), q.k) - kslots.begin());
+        const uint32_t head = mixed ? std::min<uint32_t>(nz, (uint32_t)mixed_head) : nz;
+        std::vector<Tile> tiles;
+        const uint32_t n_full = q.n_cand / CW, rem = q.n_cand % CW;
+        for (uint32_t i = 0; i < n_full; ++i)
+            tiles.push_back({(uint32_t)q.cand_offset + i * CW, CW, CW, false});
+        if (rem >= (uint32_t)ti.c_min_fill && rem > 0) {
+            tiles.push_back({(uint32_t)q.cand_offset + n_full * CW, rem, CW, false});
+        } else {
+            for (uint32_t o = 0; o < rem; o += CZ)
+                tiles.push_back({(uint32_t)q.cand_offset + n_full * CW + o,
+                                 std::min<uint32_t>(CZ, rem - o), (uint32_t)CZ, true});
+        }
+        for (const Tile& tl : tiles) {
+            // zero chunks: same work per item for both tile kinds
+            const uint64_t chunk = tl.z ? (uint64_t)ti.chunk * CW / CZ : (uint64_t)ti.chunk;
+            struct Region { uint32_t a, b; bool f32; };
+            Region regs[2] = {{0, head, false}, {head, nz, true}};
+            std::vector<std::pair<uint32_t, uint32_t>> spans;
+            std::vector<bool> f32s;
+            for (const Region& rg : regs) {
+                for (uint64_t z = rg.a; z < rg.b; z += chunk) {
+                    spans.push_back({(uint32_t)z, (uint32_t)std::min<uint64_t>(rg.b, z + chunk)});
+                    f32s.push_back(rg.f32);
+                }
+            }
+            const uint64_t base = P.partial_doubles;
+            const uint32_t H = (uint32_t)spans.size();
+            for (uint32_t h = 0; h < H; ++h) {
+                BatchItem it{};
+                it.flags = (tl.z ? 1u : 0u) | (f32s[h] ? 2u : 0u) | ((uint32_t)ks << 8);
+                it.z0 = spans[h].first;
+                it.z1 = spans[h].second;
+                it.c_first = tl.c_first;
+                it.c_count = tl.c_count;
+                it.poff = base + (uint64_t)h * 3 * tl.width;
+                P.items.push_back(it);
+                P.padded_evals += (double)tl.width * (it.z1 - it.z0);
+            }
+            P.partial_doubles += (uint64_t)H * 3 * tl.width;
+            for (uint32_t i = 0; i < tl.c_count; ++i)
+                P.desc[tl.c_first + i] = CandDesc{base, H, (uint16_t)tl.width, (uint16_t)i};
+            P.useful_evals += (double)tl.c_count * nz;
+        }
+    }
+    // longest first (FP32 items run faster than FP64 ones)
+    auto cost = [&](const BatchItem& it) {
+        const double w = (it.flags & 1) ? (double)CZ : (double)CW;
+        return w * (it.z1 - it.z0) * ((it.flags & 2) ? 0.7 : 1.0);
+    };
+    std::stable_sort(P.items.begin(), P.items.end(),
+                     [&](const BatchItem& x, const BatchItem& y) { return cost(x) > cost(y); });
+    if (P.items.size() > 0x7fffffff) return "too many work items";
+    return "";
+}
+
+template <int NC, int CZ, int FL, bool A32>
+static const void* batch_kernel_ptr() {
+    return reinterpret_cast<const void*>(&k2_batch_kernel<NC, CZ, FL, A32>);
+}
+
+const void* batch_kernel(int nc, int cz, unsigned flags, bool allow32) {
+    const bool d2 = (flags & K2_D2) != 0;
+#define K2_SEL(NC_, CZ_)                                                                         \
+    if (nc == NC_ && cz == CZ_) {                                                                \
+        if (d2) return allow32 ? batch_kernel_ptr<NC_, CZ_, K2_D1 | K2_D2, true>()               \
+                               : batch_kernel_ptr<NC_, CZ_, K2_D1 | K2_D2, false>();             \
+        return allow32 ? batch_kernel_ptr<NC_, CZ_, K2_D1, true>()                               \
+                       : batch_kernel_ptr<NC_, CZ_, K2_D1, false>();                             \
+    }
+    K2_SEL(2, 8) K2_SEL(2, 16) K2_SEL(4, 8) K2_SEL(4, 16)
+#undef K2_SEL
+    return nullptr;
+}
+
+hipError_t launch_batch(const void* kernel, int grid, const BatchViews* views,
+                        const BatchItem* items, int n_items, int* counter, const uint4* cand,
+                        double* partial, const double2* tab64, const float2* tab32,
+                        const CandDesc* desc, uint64_t n_cand, double* K, double* dK, double* d2K,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k2_counter_reset_kernel, dim3(1), dim3(64), 0, st, counter);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (n_items > 0 && grid > 0) {
+        BatchViews v = *views;
+        void* args[] = {&v, &items, &n_items, &counter, &cand, &partial, &tab64, &tab32};
+        e = hipLaunchKernel(kernel, dim3(grid), dim3(256), args, 0, st);
+        if (e != hipSuccess) return e;
+    }
+    const unsigned nb = (unsigned)((n_cand + 255) / 256);
+    if (nb > 0) {
+        hipLaunchKernelGGL(k2_batch_reduce_kernel, dim3(nb), dim3(256), 0, st, desc, n_cand,
+                           (const double*)partial, K, dK, d2K);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+}  // namespace k2impl
+
+// ---- C ABI: batched plans --------------------------------------------------------------------
+
+using namespace k2impl;
+
+struct k2_batch {
+    k2_table* t = nullptr;
+    std::vector<WeightArrays> w;   // one per distinct k, owned (the table's cache may evict)
+    BatchItem* d_items = nullptr;
+    CandDesc* d_desc = nullptr;
+    double* d_partial = nullptr;
+    int* d_counter = nullptr;
+    int n_items = 0;
+    uint64_t n_cand = 0;
+    int grid = 0;
+    const void* kernel = nullptr;
+    BatchViews views{};
+    const uint4* cand = nullptr;
+    double *K = nullptr, *dK = nullptr, *d2K = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    k2_batch_info info{};
+};
+
+static int berr(int code, const std::string& m) { return set_error(code, m); }
+
+#define BCHK(expr)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            k2_batch_destroy(b);                                                           \
+            return berr(K2_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+        }                                                                                  \
+    } while (0)
+
+extern "C" int k2_batch_destroy(k2_batch* b) {
+    if (!b) return K2_OK;
+    if (b->t) (void)hipSetDevice(b->t->device);
+    if (b->cap) (void)hipStreamSynchronize(b->cap);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto& w : b->w) free_weights(w);
+    (void)hipFree(b->d_items);
+    (void)hipFree(b->d_desc);
+    (void)hipFree(b->d_partial);
+    (void)hipFree(b->d_counter);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    delete b;
+    return K2_OK;
+}
+
+extern "C" int k2_batch_create(k2_table* t, uint32_t n_req, const k2_request* req,
+                               const uint64_t* V_dev, uint64_t n_cand_total, uint32_t flags,
+                               const k2_batch_tiling* tiling, double* K_dev, double* dK_dev,
+                               double* d2K_dev, k2_batch** out) {
+    if (!t || !out || (n_req > 0 && (!req || !V_dev || !K_dev)))
+        return berr(K2_E_ARG, "k2_batch_create: null pointer");
+    if (((flags & K2_D1) && !dK_dev) || ((flags & K2_D2) && !d2K_dev))
+        return berr(K2_E_ARG, "k2_batch_create: derivative requested without an output buffer");
+    if (n_cand_total >= (1ull << 32)) return berr(K2_E_ARG, "k2_batch_create: n_cand_total >= 2^32");
+    if (((uintptr_t)V_dev & 15) != 0) return berr(K2_E_ARG, "k2_batch_create: V not 16-byte aligned");
+    k2_batch_tiling ti{4, 8, 8192, 512, 0};
+    if (tiling) {
+        ti = *tiling;
+        if (ti.nc == 0) ti.nc = 4;
+        if (ti.cz == 0) ti.cz = 8;
+        if (ti.chunk == 0) ti.chunk = 8192;
+        if (ti.c_min_fill == 0) ti.c_min_fill = 256 * ti.nc / 2;
+    }
+    if ((ti.nc != 2 && ti.nc != 4) || (ti.cz != 8 && ti.cz != 16) || ti.chunk < 256 ||
+        ti.c_min_fill < 1 || ti.wg_per_cu < 0)
+        return berr(K2_E_ARG, "k2_batch_create: tiling out of range (nc 2|4, cz 8|16, chunk >= 256)");
+    std::vector<double> ks;
+    for (uint32_t r = 0; r < n_req; ++r) {
+        if (!(req[r].k >= 0.0) || !std::isfinite(req[r].k))
+            return berr(K2_E_ARG, "k2_batch_create: request k must be finite and >= 0");
+        if (std::find(ks.begin(), ks.end(), req[r].k) == ks.end()) ks.push_back(req[r].k);
+    }
+    if (ks.size() > 4) return berr(K2_E_ARG, "k2_batch_create: more than 4 distinct k values");
+    const bool mixed = (flags & K2_MIXED) != 0;
+    BatchPlan P;
+    std::string msg = plan_batch(n_req, req, n_cand_total, t->n, ks, mixed, t->tiling.mixed_head,
+                                 ti, P);
+    if (!msg.empty()) return berr(K2_E_ARG, "k2_batch_create: " + msg);
+
+    k2_batch* b = new k2_batch();
+    b->t = t;
+    b->n_items = (int)P.items.size();
+    b->n_cand = n_cand_total;
+    b->cand = (const uint4*)V_dev;
+    b->K = K_dev;
+    b->dK = dK_dev;
+    b->d2K = d2K_dev;
+    BCHK(hipSetDevice(t->device));
+    BCHK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    for (double k : ks) {
+        WeightArrays w;
+        if (alloc_weights(w, t->n_pad) != K2_OK) {
+            k2_batch_destroy(b);
+            return K2_E_NOMEM;
+        }
+        w.k = k;
+        b->w.push_back(w);
+        BCHK(launch_weights(t->gamma, t->w, (int)t->n, (int)t->n_pad, k, b->w.back(), b->cap));
+    }
+    for (size_t i = 0; i < 4; ++i) {
+        const WeightArrays& w = b->w.empty() ? WeightArrays{} : b->w[std::min(i, b->w.size() - 1)];
+        b->views.zv[i] = ZeroView{t->g, w.a, w.b, w.e, w.af, w.bf, w.ef};
+    }
+    BCHK(hipMalloc((void**)&b->d_counter, 256));
+    BCHK(hipMalloc((void**)&b->d_items, std::max<size_t>(1, P.items.size()) * sizeof(BatchItem)));
+    BCHK(hipMalloc((void**)&b->d_desc, std::max<size_t>(1, P.desc.size()) * sizeof(CandDesc)));
+    BCHK(hipMalloc((void**)&b->d_partial, std::max<uint64_t>(1, P.partial_doubles) * sizeof(double)));
+    if (!P.items.empty())
+        BCHK(hipMemcpyAsync(b->d_items, P.items.data(), P.items.size() * sizeof(BatchItem),
+                            hipMemcpyHostToDevice, b->cap));
+    if (!P.desc.empty())
+        BCHK(hipMemcpyAsync(b->d_desc, P.desc.data(), P.desc.size() * sizeof(CandDesc),
+                            hipMemcpyHostToDevice, b->cap));
+    BCHK(hipStreamSynchronize(b->cap));
+
+    b->kernel = batch_kernel(ti.nc, ti.cz, flags, mixed);
+    if (!b->kernel) {
+        k2_batch_destroy(b);
+        return berr(K2_E_ARG, "k2_batch_create: no kernel for this tiling");
+    }
+    int per_cu = 0, n_cu = 0;
+    BCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->kernel, 256, 0));
+    BCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, t->device));
+    if (ti.wg_per_cu > 0) per_cu = std::min(per_cu, ti.wg_per_cu);
+    b->grid = std::max(1, std::min(b->n_items, std::max(1, per_cu) * n_cu));
+
+    b->info.n_items = b->n_items;
+    b->info.grid = b->grid;
+    b->info.n_k = (int)ks.size();
+    b->info.useful_evals = P.useful_evals;
+    b->info.padded_evals = P.padded_evals;
+    b->info.partial_bytes = (int64_t)(P.partial_doubles * sizeof(double));
+    *out = b;
+    return K2_OK;
+}
+
+static hipError_t batch_launches(k2_batch* b, hipStream_t st) {
+    k2_table* t = b->t;
+    return launch_batch(b->kernel, b->grid, &b->views, b->d_items, b->n_items, b->d_counter, b->cand,
+                        b->d_partial, t->tab64, t->tab32, b->d_desc, b->n_cand, b->K, b->dK,
+                        b->d2K, st);
+}
+
+extern "C" int k2_batch_run_direct(k2_batch* b, void* stream) {
+    if (!b) return berr(K2_E_ARG, "k2_batch_run_direct: null batch");
+    hipError_t e = hipSetDevice(b->t->device);
+    if (e == hipSuccess) e = batch_launches(b, (hipStream_t)stream);
+    if (e != hipSuccess) return berr(K2_E_HIP, std::string("k2_batch_run_direct: ") + hipGetErrorString(e));
+    return K2_OK;
+}
+
+extern "C" int k2_batch_run(k2_batch* b, void* stream) {
+    if (!b) return berr(K2_E_ARG, "k2_batch_run: null batch");
+    hipError_t e = hipSetDevice(b->t->device);
+    if (e == hipSuccess && !b->exec) {
+        // capture once on the batch's own stream; replays may go to any stream
+        e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t el = batch_launches(b, b->cap);
+            e = hipStreamEndCapture(b->cap, &b->graph);
+            if (el != hipSuccess) e = el;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    }
+    if (e == hipSuccess) e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return berr(K2_E_HIP, std::string("k2_batch_run: ") + hipGetErrorString(e));
+    return K2_OK;
+}
+
+extern "C" int k2_batch_get_info(const k2_batch* b, k2_batch_info* info) {
+    if (!b || !info) return berr(K2_E_ARG, "k2_batch_get_info: null pointer");
+    *info = b->info;
+    return K2_OK;
+}
+
+extern "C" int k2_eval_batched_device(k2_table* t, uint32_t n_req, const k2_request* req,
+                                      const uint64_t* V_dev, uint64_t n_cand_total, uint32_t flags,
+                                      double* K_dev, double* dK_dev, double* d2K_dev,
+                                      void* stream) {
+    k2_batch* b = nullptr;
+    int rc = k2_batch_create(t, n_req, req, V_dev, n_cand_total, flags, nullptr, K_dev, dK_dev,
+                             d2K_dev, &b);
+    if (rc) return rc;
+    rc = k2_batch_run_direct(b, stream);
+    if (rc == K2_OK) {
+        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+        if (e != hipSuccess) rc = berr(K2_E_HIP, std::string("k2_eval_batched_device: ") + hipGetErrorString(e));
+    }
+    k2_batch_destroy(b);
+    return rc;
+}

@@ -2,6 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "../../include/k2eval.h"
 
 namespace k2impl {
@@ -40,6 +44,52 @@ struct LaunchArgs {
 
 // host-side (cos, sin)(2 pi k / n) tables, rounded from long double
 void sincos_tables(int n, double2* t64, float2* t32);
+// device copies of those tables (K2_TAB entries)
+int upload_tables(double2** t64, float2** t32, hipStream_t st);
+// records the calling thread's last error (k2_last_error) and returns code
+int set_error(int code, const std::string& msg);
+int alloc_weights(WeightArrays& w, size_t n_pad);
+void free_weights(WeightArrays& w);
+
+}  // namespace k2impl
+
+namespace k2impl {
+struct ProfPair {
+    hipEvent_t a, b;
+};
+
+}  // namespace k2impl
+
+struct k2_table {
+    using WeightArrays = k2impl::WeightArrays;
+    using ProfPair = k2impl::ProfPair;
+    int device = 0;
+    uint32_t n = 0, n_pad = 0;
+    uint4* g = nullptr;
+    double* gamma = nullptr;
+    double* w = nullptr;
+    double2* tab64 = nullptr;   // (cos, sin)(2 pi k / K2_TAB_N)
+    float2* tab32 = nullptr;
+    std::vector<WeightArrays> wsets;
+    uint64_t use_clock = 0;
+    double* partial = nullptr;
+    size_t partial_bytes = 0;
+    void* host_io = nullptr;   // device staging for host-buffer entry points
+    size_t host_io_bytes = 0;
+    int* polish_active = nullptr;
+    size_t polish_cap = 0;
+    hipStream_t stream = nullptr;
+    k2_tiling tiling{4, 2048, 8192};
+    bool profile = false;
+    std::vector<ProfPair> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+    std::mutex mu;
+};
+
+
+namespace k2impl {
 
 hipError_t launch_eval(const LaunchArgs& a, bool mixed, hipStream_t st);
 hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,

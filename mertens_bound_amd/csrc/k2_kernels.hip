@@ -16,8 +16,10 @@
 //     into one XCD's L2 once and re-read from there.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "k2_device.h"
 #include "k2_internal.h"
+#include "k2_tiles.h"
 
 using namespace k2dev;
 
@@ -51,199 +53,42 @@ __global__ __launch_bounds__(256) void k2_weights_kernel(
     ef[j] = make_float2((float)ve.x, (float)ve.y);
 }
 
-// ---- plain FP64 evaluation ---------------------------------------------------------------------
-template <int NC, int FL>
-__global__ __launch_bounds__(256) void k2_plain_kernel(
-    const uint4* __restrict__ zg, const double2* __restrict__ za, const double2* __restrict__ zb,
-    const double2* __restrict__ ze, int z_begin, int z_end, int zeros_per_split,
-    const uint4* __restrict__ cand, int n_cand, int n_ctiles,
-    double* __restrict__ partial, int split_base, int n_cand_pad,
-    const double2* __restrict__ tab) {
-    constexpr bool D1 = (FL & K2_D1) != 0;
-    constexpr bool D2 = (FL & K2_D2) != 0;
-    __shared__ double2 s_tab[K2_TAB];
-    __shared__ uint4 s_g[2][K2_ZT];
-    __shared__ double2 s_a[2][K2_ZT];
-    __shared__ double2 s_b[2][D1 ? K2_ZT : 1];
-    __shared__ double2 s_e[2][D2 ? K2_ZT : 1];
-
+// ---- one-launch evaluation: C tiles over (split, candidate tile) workgroups -----------------------
+template <int NC, int FL, bool F32>
+__global__ __launch_bounds__(256) void k2_eval_kernel(
+    ZeroView zv, int z_begin, int z_end, int zeros_per_split, const uint4* __restrict__ cand,
+    int n_cand, int n_ctiles, double* __restrict__ partial, int split_base, int n_cand_pad,
+    const double2* __restrict__ tab64, const float2* __restrict__ tab32) {
+    __shared__ CTileSmem sm;
+    __shared__ typename std::conditional<F32, float2, double2>::type s_tab[K2_TAB];
     const int L = xcd_linear_block();
     const int split = L / n_ctiles, ct = L - split * n_ctiles;
     const int z0 = z_begin + split * zeros_per_split;
     const int z1 = min(z_end, z0 + zeros_per_split);
     const int tid = threadIdx.x;
-
-    for (int i = tid; i < K2_TAB; i += K2_ZT) s_tab[i] = tab[i];
+    for (int i = tid; i < K2_TAB; i += K2_ZT) {
+        if constexpr (F32) s_tab[i] = tab32[i];
+        else s_tab[i] = tab64[i];
+    }
     Limbs4 v[NC];
-    double acc0[NC], acc1[NC], acc2[NC];
+    Acc<F32> acc[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
         uint4 cv = make_uint4(0, 0, 0, 0);
         if (c < n_cand) cv = cand[c];
         v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
-        acc0[i] = 0.0; acc1[i] = 0.0; acc2[i] = 0.0;
+        acc[i].zero();
     }
-
-    const int ntiles = z1 > z0 ? (z1 - z0 + K2_ZT - 1) / K2_ZT : 0;
-    uint4 rg = make_uint4(0, 0, 0, 0);
-    double2 ra = make_double2(0.0, 0.0), rb = ra, re = ra;
-    auto fetch = [&](int t) {
-        const int j = z0 + t * K2_ZT + tid;
-        if (j < z1) {
-            rg = zg[j]; ra = za[j];
-            if (D1) rb = zb[j];
-            if (D2) re = ze[j];
-        } else {
-            rg = make_uint4(0, 0, 0, 0); ra = make_double2(0.0, 0.0); rb = ra; re = ra;
-        }
-    };
-    auto stage = [&](int buf) {
-        s_g[buf][tid] = rg; s_a[buf][tid] = ra;
-        if (D1) s_b[buf][tid] = rb;
-        if (D2) s_e[buf][tid] = re;
-    };
-    if (ntiles > 0) { fetch(0); stage(0); }
-    __syncthreads();
-    for (int t = 0; t < ntiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ntiles) fetch(t + 1);
-#pragma unroll 2
-        for (int z = 0; z < K2_ZT; ++z) {
-            const uint4 gg = s_g[buf][z];
-            const double2 aa = s_a[buf][z];
-            double2 bb, ee;
-            if (D1) bb = s_b[buf][z];
-            if (D2) ee = s_e[buf][z];
-            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const uint64_t W = phase_word(g, v[i]);
-                double s, c;
-                sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), s_tab, s, c);
-                acc0[i] = fma(aa.x, c, acc0[i]);
-                acc0[i] = fma(-aa.y, s, acc0[i]);
-                if (D1) {
-                    acc1[i] = fma(-bb.x, s, acc1[i]);
-                    acc1[i] = fma(-bb.y, c, acc1[i]);
-                }
-                if (D2) {
-                    acc2[i] = fma(-ee.x, c, acc2[i]);
-                    acc2[i] = fma(ee.y, s, acc2[i]);
-                }
-            }
-        }
-        if (t + 1 < ntiles) stage(buf ^ 1);
-        __syncthreads();
-    }
+    if constexpr (F32) ctile_f32<NC, FL>(zv, z0, z1, v, acc, sm, s_tab);
+    else ctile_f64<NC, FL>(zv, z0, z1, v, acc, sm, s_tab);
     double* prow = partial + (size_t)(split_base + split) * 3 * n_cand_pad;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
-        if (c < n_cand_pad) {
-            prow[c] = acc0[i];
-            if (D1) prow[n_cand_pad + c] = acc1[i];
-            if (D2) prow[2 * n_cand_pad + c] = acc2[i];
-        }
-    }
-}
-
-// ---- mixed-precision evaluation: exact top-32-bit phase, FP32 sin/cos and weights, -----------
-// ---- Kahan-compensated FP32 accumulation; partials leave in FP64 ------------------------------
-__device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
-    const float y = x - comp;
-    const float t = sum + y;
-    comp = (t - sum) - y;
-    sum = t;
-}
-
-template <int NC, int FL>
-__global__ __launch_bounds__(256) void k2_mixed_kernel(
-    const uint4* __restrict__ zg, const float2* __restrict__ za, const float2* __restrict__ zb,
-    const float2* __restrict__ ze, int z_begin, int z_end, int zeros_per_split,
-    const uint4* __restrict__ cand, int n_cand, int n_ctiles,
-    double* __restrict__ partial, int split_base, int n_cand_pad,
-    const float2* __restrict__ tab) {
-    constexpr bool D1 = (FL & K2_D1) != 0;
-    constexpr bool D2 = (FL & K2_D2) != 0;
-    __shared__ float2 s_tab[K2_TAB];
-    __shared__ uint4 s_g[2][K2_ZT];
-    __shared__ float4 s_ab[2][K2_ZT];
-    __shared__ float2 s_e[2][D2 ? K2_ZT : 1];
-
-    const int L = xcd_linear_block();
-    const int split = L / n_ctiles, ct = L - split * n_ctiles;
-    const int z0 = z_begin + split * zeros_per_split;
-    const int z1 = min(z_end, z0 + zeros_per_split);
-    const int tid = threadIdx.x;
-
-    for (int i = tid; i < K2_TAB; i += K2_ZT) s_tab[i] = tab[i];
-    Limbs4 v[NC];
-    float s0[NC], c0[NC], s1[NC], c1[NC], s2[NC], c2[NC];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
-        uint4 cv = make_uint4(0, 0, 0, 0);
-        if (c < n_cand) cv = cand[c];
-        v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
-        s0[i] = c0[i] = s1[i] = c1[i] = s2[i] = c2[i] = 0.0f;
-    }
-    const int ntiles = z1 > z0 ? (z1 - z0 + K2_ZT - 1) / K2_ZT : 0;
-    uint4 rg = make_uint4(0, 0, 0, 0);
-    float4 rab = make_float4(0.f, 0.f, 0.f, 0.f);
-    float2 re = make_float2(0.f, 0.f);
-    auto fetch = [&](int t) {
-        const int j = z0 + t * K2_ZT + tid;
-        if (j < z1) {
-            rg = zg[j];
-            const float2 a = za[j];
-            float2 b = make_float2(0.f, 0.f);
-            if (D1) b = zb[j];
-            rab = make_float4(a.x, a.y, b.x, b.y);
-            if (D2) re = ze[j];
-        } else {
-            rg = make_uint4(0, 0, 0, 0); rab = make_float4(0.f, 0.f, 0.f, 0.f);
-            re = make_float2(0.f, 0.f);
-        }
-    };
-    auto stage = [&](int buf) {
-        s_g[buf][tid] = rg; s_ab[buf][tid] = rab;
-        if (D2) s_e[buf][tid] = re;
-    };
-    if (ntiles > 0) { fetch(0); stage(0); }
-    __syncthreads();
-    for (int t = 0; t < ntiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ntiles) fetch(t + 1);
-#pragma unroll 2
-        for (int z = 0; z < K2_ZT; ++z) {
-            const uint4 gg = s_g[buf][z];
-            const float4 ab = s_ab[buf][z];
-            float2 ee;
-            if (D2) ee = s_e[buf][z];
-            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const uint32_t W = phase_word_hi32(g, v[i]);
-                float s, c;
-                sincos_tab32(W, s_tab, s, c);
-                kahan_add(s0[i], c0[i], fmaf(ab.x, c, -ab.y * s));
-                if (D1) kahan_add(s1[i], c1[i], -fmaf(ab.z, s, ab.w * c));
-                if (D2) kahan_add(s2[i], c2[i], fmaf(ee.y, s, -ee.x * c));
-            }
-        }
-        if (t + 1 < ntiles) stage(buf ^ 1);
-        __syncthreads();
-    }
-    double* prow = partial + (size_t)(split_base + split) * 3 * n_cand_pad;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
-        if (c < n_cand_pad) {
-            prow[c] = (double)s0[i] - (double)c0[i];
-            if (D1) prow[n_cand_pad + c] = (double)s1[i] - (double)c1[i];
-            if (D2) prow[2 * n_cand_pad + c] = (double)s2[i] - (double)c2[i];
-        }
+        prow[c] = acc[i].get(0);
+        if (FL & K2_D1) prow[n_cand_pad + c] = acc[i].get(1);
+        if (FL & K2_D2) prow[2 * n_cand_pad + c] = acc[i].get(2);
     }
 }
 
@@ -355,47 +200,36 @@ __global__ __launch_bounds__(256) void k2_ref_peak_ab_kernel(
 // ---- launchers -------------------------------------------------------------------------------------
 namespace k2impl {
 
-template <int NC, int FL>
-static hipError_t launch_plain_t(const LaunchArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k2_plain_kernel<NC, FL>), dim3(a.n_blocks), dim3(K2_ZT), 0, st,
-                       a.zg, a.za, a.zb, a.ze, a.z_begin, a.z_end, a.zeros_per_split, a.cand,
-                       a.n_cand, a.n_ctiles, a.partial, a.split_base, a.n_cand_pad, a.tab64);
+template <int NC, int FL, bool F32>
+static hipError_t launch_eval_t(const LaunchArgs& a, hipStream_t st) {
+    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef};
+    hipLaunchKernelGGL((k2_eval_kernel<NC, FL, F32>), dim3(a.n_blocks), dim3(K2_ZT), 0, st, zv,
+                       a.z_begin, a.z_end, a.zeros_per_split, a.cand, a.n_cand, a.n_ctiles,
+                       a.partial, a.split_base, a.n_cand_pad, a.tab64, a.tab32);
     return hipGetLastError();
 }
 
-template <int NC, int FL>
-static hipError_t launch_mixed_t(const LaunchArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k2_mixed_kernel<NC, FL>), dim3(a.n_blocks), dim3(K2_ZT), 0, st,
-                       a.zg, a.zaf, a.zbf, a.zef, a.z_begin, a.z_end, a.zeros_per_split, a.cand,
-                       a.n_cand, a.n_ctiles, a.partial, a.split_base, a.n_cand_pad, a.tab32);
-    return hipGetLastError();
+template <int NC, bool F32>
+static hipError_t dispatch_fl(const LaunchArgs& a, hipStream_t st) {
+    switch (a.flags & (K2_D1 | K2_D2)) {
+        case 0: return launch_eval_t<NC, 0, F32>(a, st);
+        case K2_D1: return launch_eval_t<NC, K2_D1, F32>(a, st);
+        case K2_D2: return launch_eval_t<NC, K2_D2, F32>(a, st);
+        default: return launch_eval_t<NC, K2_D1 | K2_D2, F32>(a, st);
+    }
 }
 
 template <int NC>
-static hipError_t dispatch_fl(const LaunchArgs& a, bool mixed, hipStream_t st) {
-    const unsigned fl = a.flags & (K2_D1 | K2_D2);
-    if (mixed) {
-        switch (fl) {
-            case 0: return launch_mixed_t<NC, 0>(a, st);
-            case K2_D1: return launch_mixed_t<NC, K2_D1>(a, st);
-            case K2_D2: return launch_mixed_t<NC, K2_D2>(a, st);
-            default: return launch_mixed_t<NC, K2_D1 | K2_D2>(a, st);
-        }
-    }
-    switch (fl) {
-        case 0: return launch_plain_t<NC, 0>(a, st);
-        case K2_D1: return launch_plain_t<NC, K2_D1>(a, st);
-        case K2_D2: return launch_plain_t<NC, K2_D2>(a, st);
-        default: return launch_plain_t<NC, K2_D1 | K2_D2>(a, st);
-    }
+static hipError_t dispatch_prec(const LaunchArgs& a, bool mixed, hipStream_t st) {
+    return mixed ? dispatch_fl<NC, true>(a, st) : dispatch_fl<NC, false>(a, st);
 }
 
 hipError_t launch_eval(const LaunchArgs& a, bool mixed, hipStream_t st) {
     switch (a.nc) {
-        case 1: return dispatch_fl<1>(a, mixed, st);
-        case 2: return dispatch_fl<2>(a, mixed, st);
-        case 4: return dispatch_fl<4>(a, mixed, st);
-        case 8: return dispatch_fl<8>(a, mixed, st);
+        case 1: return dispatch_prec<1>(a, mixed, st);
+        case 2: return dispatch_prec<2>(a, mixed, st);
+        case 4: return dispatch_prec<4>(a, mixed, st);
+        case 8: return dispatch_prec<8>(a, mixed, st);
         default: return hipErrorInvalidValue;
     }
 }

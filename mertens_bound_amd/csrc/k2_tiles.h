@@ -1,0 +1,275 @@
+// Tile bodies shared by the evaluation kernels (one launch per call) and the persistent batched
+// kernel: a "C tile" keeps NC candidates per lane in VGPRs and streams zeros through LDS (every
+// lane reads the same zero record, an LDS broadcast); a "Z tile" keeps CZ candidates in LDS and
+// gives each lane its own zeros, then reduces the per-lane sums with wavefront shuffles.  C tiles
+// serve long candidate lists, Z tiles the short requests of a batch, where a C tile would leave
+// most lanes idle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "k2_device.h"
+
+namespace k2dev {
+
+#define K2_WG 256   // lanes per workgroup = zeros per LDS tile
+
+// one k's weighted zero arrays (padded with zero weights past the table end)
+struct ZeroView {
+    const uint4* g;
+    const double2* a;   // 2 e^{-k g^2} w
+    const double2* b;   // g a
+    const double2* e;   // g^2 a
+    const float2* af;
+    const float2* bf;
+    const float2* ef;
+};
+
+// ---- accumulators -------------------------------------------------------------------------
+template <bool F32> struct Acc;
+
+template <> struct Acc<false> {
+    double s0, s1, s2;
+    __device__ __forceinline__ void zero() { s0 = s1 = s2 = 0.0; }
+    __device__ __forceinline__ double get(int q) const { return q == 0 ? s0 : (q == 1 ? s1 : s2); }
+};
+
+// FP32 sums with Kahan compensation (comp holds the negated lost low part)
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
+    const float y = x - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+template <> struct Acc<true> {
+    float s0, c0, s1, c1, s2, c2;
+    __device__ __forceinline__ void zero() { s0 = c0 = s1 = c1 = s2 = c2 = 0.0f; }
+    __device__ __forceinline__ double get(int q) const {
+        return q == 0 ? (double)s0 - (double)c0
+                      : (q == 1 ? (double)s1 - (double)c1 : (double)s2 - (double)c2);
+    }
+};
+
+// one term: K += Re(a e^{i th}), K' += Re(i b e^{i th}), K'' -= Re(e e^{i th})
+template <int FL>
+__device__ __forceinline__ void accumulate(Acc<false>& A, double S, double C, double2 a,
+                                           double2 b, double2 e) {
+    A.s0 = fma(a.x, C, A.s0);
+    A.s0 = fma(-a.y, S, A.s0);
+    if (FL & K2_D1) {
+        A.s1 = fma(-b.x, S, A.s1);
+        A.s1 = fma(-b.y, C, A.s1);
+    }
+    if (FL & K2_D2) {
+        A.s2 = fma(-e.x, C, A.s2);
+        A.s2 = fma(e.y, S, A.s2);
+    }
+}
+
+template <int FL>
+__device__ __forceinline__ void accumulate(Acc<true>& A, float S, float C, float2 a, float2 b,
+                                           float2 e) {
+    kahan_add(A.s0, A.c0, fmaf(a.x, C, -a.y * S));
+    if (FL & K2_D1) kahan_add(A.s1, A.c1, -fmaf(b.x, S, b.y * C));
+    if (FL & K2_D2) kahan_add(A.s2, A.c2, fmaf(e.y, S, -e.x * C));
+}
+
+// ---- C tile ---------------------------------------------------------------------------------
+// LDS: two buffers of K2_WG zero records (FP64: g, a, b, e = 64 B; FP32 reuses the space)
+struct CTileSmem {
+    uint4 g[2][K2_WG];
+    double2 a[2][K2_WG];
+    double2 b[2][K2_WG];
+    double2 e[2][K2_WG];
+};
+
+// Sums zeros [z0, z1) into acc for the NC candidates of this lane.  Enters and leaves with sm
+// free (the last action is a barrier).  tab: the (cos, sin) table in LDS of the matching width.
+template <int NC, int FL>
+__device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, const Limbs4 (&v)[NC],
+                                          Acc<false> (&acc)[NC], CTileSmem& sm,
+                                          const double2* tab) {
+    constexpr bool D1 = (FL & K2_D1) != 0;
+    constexpr bool D2 = (FL & K2_D2) != 0;
+    const int tid = threadIdx.x;
+    const int ntiles = z1 > z0 ? (z1 - z0 + K2_WG - 1) / K2_WG : 0;
+    uint4 rg = make_uint4(0, 0, 0, 0);
+    double2 ra = make_double2(0.0, 0.0), rb = ra, re = ra;
+    auto fetch = [&](int t) {
+        const int j = z0 + t * K2_WG + tid;
+        if (j < z1) {
+            rg = zv.g[j]; ra = zv.a[j];
+            if (D1) rb = zv.b[j];
+            if (D2) re = zv.e[j];
+        } else {
+            rg = make_uint4(0, 0, 0, 0); ra = make_double2(0.0, 0.0); rb = ra; re = ra;
+        }
+    };
+    auto stage = [&](int buf) {
+        sm.g[buf][tid] = rg; sm.a[buf][tid] = ra;
+        if (D1) sm.b[buf][tid] = rb;
+        if (D2) sm.e[buf][tid] = re;
+    };
+    if (ntiles > 0) { fetch(0); stage(0); }
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) fetch(t + 1);
+#pragma unroll 2
+        for (int z = 0; z < K2_WG; ++z) {
+            const uint4 gg = sm.g[buf][z];
+            const double2 aa = sm.a[buf][z];
+            double2 bb = make_double2(0.0, 0.0), ee = bb;
+            if (D1) bb = sm.b[buf][z];
+            if (D2) ee = sm.e[buf][z];
+            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const uint64_t W = phase_word(g, v[i]);
+                double S, C;
+                sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab, S, C);
+                accumulate<FL>(acc[i], S, C, aa, bb, ee);
+            }
+        }
+        if (t + 1 < ntiles) stage(buf ^ 1);
+        __syncthreads();
+    }
+}
+
+template <int NC, int FL>
+__device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, const Limbs4 (&v)[NC],
+                                          Acc<true> (&acc)[NC], CTileSmem& sm,
+                                          const float2* tab) {
+    constexpr bool D1 = (FL & K2_D1) != 0;
+    constexpr bool D2 = (FL & K2_D2) != 0;
+    float4 (*s_ab)[K2_WG] = reinterpret_cast<float4 (*)[K2_WG]>(sm.a);
+    float2 (*s_e)[K2_WG] = reinterpret_cast<float2 (*)[K2_WG]>(sm.e);
+    const int tid = threadIdx.x;
+    const int ntiles = z1 > z0 ? (z1 - z0 + K2_WG - 1) / K2_WG : 0;
+    uint4 rg = make_uint4(0, 0, 0, 0);
+    float4 rab = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 re = make_float2(0.f, 0.f);
+    auto fetch = [&](int t) {
+        const int j = z0 + t * K2_WG + tid;
+        if (j < z1) {
+            rg = zv.g[j];
+            const float2 a = zv.af[j];
+            float2 b = make_float2(0.f, 0.f);
+            if (D1) b = zv.bf[j];
+            rab = make_float4(a.x, a.y, b.x, b.y);
+            if (D2) re = zv.ef[j];
+        } else {
+            rg = make_uint4(0, 0, 0, 0); rab = make_float4(0.f, 0.f, 0.f, 0.f);
+            re = make_float2(0.f, 0.f);
+        }
+    };
+    auto stage = [&](int buf) {
+        sm.g[buf][tid] = rg; s_ab[buf][tid] = rab;
+        if (D2) s_e[buf][tid] = re;
+    };
+    if (ntiles > 0) { fetch(0); stage(0); }
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) fetch(t + 1);
+#pragma unroll 2
+        for (int z = 0; z < K2_WG; ++z) {
+            const uint4 gg = sm.g[buf][z];
+            const float4 ab = s_ab[buf][z];
+            float2 ee = make_float2(0.f, 0.f);
+            if (D2) ee = s_e[buf][z];
+            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const uint32_t W = phase_word_hi32(g, v[i]);
+                float S, C;
+                sincos_tab32(W, tab, S, C);
+                accumulate<FL>(acc[i], S, C, make_float2(ab.x, ab.y), make_float2(ab.z, ab.w),
+                               ee);
+            }
+        }
+        if (t + 1 < ntiles) stage(buf ^ 1);
+        __syncthreads();
+    }
+}
+
+// ---- Z tile ---------------------------------------------------------------------------------
+template <int CZ>
+struct ZTileSmem {
+    uint4 v[CZ];
+    double red[K2_WG / 64][3 * CZ];
+};
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Sums zeros [z0, z1) for candidates cand[c_first .. c_first + c_count) (c_count <= CZ); lanes
+// stride over the zeros.  Writes out[q * CZ + c] (q = value, K', K'') after a fixed-order
+// reduction: butterfly within each wave, then the four wave sums in order.
+template <int CZ, int FL, bool F32>
+__device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const uint4* cand,
+                                      int c_first, int c_count, double* out, ZTileSmem<CZ>& sm,
+                                      const double2* tab64, const float2* tab32) {
+    constexpr bool D1 = (FL & K2_D1) != 0;
+    constexpr bool D2 = (FL & K2_D2) != 0;
+    const int tid = threadIdx.x;
+    if (tid < CZ) sm.v[tid] = tid < c_count ? cand[c_first + tid] : make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    Acc<F32> acc[CZ];
+#pragma unroll
+    for (int c = 0; c < CZ; ++c) acc[c].zero();
+    for (int j = z0 + tid; j < z1; j += K2_WG) {
+        const uint4 gg = zv.g[j];
+        const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+        if constexpr (F32) {
+            const float2 a = zv.af[j];
+            const float2 b = D1 ? zv.bf[j] : make_float2(0.f, 0.f);
+            const float2 e = D2 ? zv.ef[j] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int c = 0; c < CZ; ++c) {
+                const uint4 cv = sm.v[c];
+                const uint32_t W = phase_word_hi32(g, Limbs4{cv.x, cv.y, cv.z, cv.w});
+                float S, C;
+                sincos_tab32(W, tab32, S, C);
+                accumulate<FL>(acc[c], S, C, a, b, e);
+            }
+        } else {
+            const double2 a = zv.a[j];
+            const double2 b = D1 ? zv.b[j] : make_double2(0.0, 0.0);
+            const double2 e = D2 ? zv.e[j] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < CZ; ++c) {
+                const uint4 cv = sm.v[c];
+                const uint64_t W = phase_word(g, Limbs4{cv.x, cv.y, cv.z, cv.w});
+                double S, C;
+                sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab64, S, C);
+                accumulate<FL>(acc[c], S, C, a, b, e);
+            }
+        }
+    }
+    const int lane = tid & 63, wid = tid >> 6;
+    constexpr int NQ = D2 ? 3 : (D1 ? 2 : 1);
+#pragma unroll
+    for (int c = 0; c < CZ; ++c) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const double x = wave_sum(acc[c].get(q));
+            if (lane == 0) sm.red[wid][q * CZ + c] = x;
+        }
+    }
+    __syncthreads();
+    if (tid < 3 * CZ) {
+        double s = 0.0;
+        if (tid < NQ * CZ) {
+#pragma unroll
+            for (int w = 0; w < K2_WG / 64; ++w) s += sm.red[w][tid];
+        }
+        out[tid] = s;
+    }
+    __syncthreads();
+}
+
+}  // namespace k2dev

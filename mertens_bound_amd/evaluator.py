@@ -45,6 +45,17 @@ class k2_tiling(ctypes.Structure):
                 ("mixed_head", ctypes.c_int32)]
 
 
+class k2_batch_tiling(ctypes.Structure):
+    _fields_ = [("nc", ctypes.c_int32), ("cz", ctypes.c_int32), ("chunk", ctypes.c_int32),
+                ("c_min_fill", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32)]
+
+
+class k2_batch_info(ctypes.Structure):
+    _fields_ = [("n_items", ctypes.c_int64), ("grid", ctypes.c_int32), ("n_k", ctypes.c_int32),
+                ("useful_evals", ctypes.c_double), ("padded_evals", ctypes.c_double),
+                ("partial_bytes", ctypes.c_int64)]
+
+
 _lib = None
 
 
@@ -63,6 +74,13 @@ def _declare(L):
                                  ctypes.c_uint32, _vp, _vp, _vp, _vp]
     L.k2_eval_batched_device.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(k2_request), _vp,
                                          ctypes.c_uint64, ctypes.c_uint32, _vp, _vp, _vp, _vp]
+    L.k2_batch_create.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(k2_request), _vp,
+                                  ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(k2_batch_tiling),
+                                  _vp, _vp, _vp, ctypes.POINTER(_vp)]
+    L.k2_batch_run.argtypes = [_vp, _vp]
+    L.k2_batch_run_direct.argtypes = [_vp, _vp]
+    L.k2_batch_get_info.argtypes = [_vp, ctypes.POINTER(k2_batch_info)]
+    L.k2_batch_destroy.argtypes = [_vp]
     L.k2_polish_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                    ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp, _vp,
                                    _vp, _vp]
@@ -189,19 +207,17 @@ class K2Table:
         return K, dK, d2K
 
     def eval_batched_device(self, requests, V, flags=VALUE | D1, out=None, stream=None):
-        """requests: iterable of (n_zeros, n_cand, cand_offset, k); V: (N, 2) tensor."""
+        """requests: iterable of (n_zeros, n_cand, cand_offset, k); V: (N, 2) tensor.  One
+        persistent launch; waits for it to finish."""
         import torch
-        reqs = list(requests)
-        arr = (k2_request * max(1, len(reqs)))()
-        for i, (nz, nc, off, k) in enumerate(reqs):
-            arr[i] = k2_request(int(nz), int(nc), int(off), float(k))
+        arr, n = _requests(requests)
         N = V.shape[0]
         if out is None:
             out = tuple(torch.zeros(N, dtype=torch.float64, device=V.device) for _ in range(3))
         K, dK, d2K = out
         st = stream if stream is not None else torch.cuda.current_stream(V.device).cuda_stream
         _check(load_library().k2_eval_batched_device(
-            self._h, len(reqs), arr, V.data_ptr(), N, int(flags), K.data_ptr(),
+            self._h, n, arr, V.data_ptr(), N, int(flags), K.data_ptr(),
             dK.data_ptr() if dK is not None else None, d2K.data_ptr() if d2K is not None else None,
             st), "k2_eval_batched_device")
         return K, dK, d2K
@@ -228,6 +244,72 @@ class K2Table:
         _check(load_library().k2_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n)),
                "k2_kernel_time")
         return ms.value, n.value
+
+
+def _requests(requests):
+    reqs = list(requests)
+    arr = (k2_request * max(1, len(reqs)))()
+    for i, (nz, nc, off, k) in enumerate(reqs):
+        arr[i] = k2_request(int(nz), int(nc), int(off), float(k))
+    return arr, len(reqs)
+
+
+class K2Batch:
+    """A batched plan: uneven requests (n_zeros, n_cand, cand_offset, k) over one table, bound to
+    the candidate tensor V and output tensors, run as one persistent launch; ``run`` replays a
+    HIP graph captured on first use, ``run_direct`` issues the launches without the graph."""
+
+    def __init__(self, table: "K2Table", requests, V, flags=VALUE | D1, tiling=None, out=None):
+        import torch
+        L = load_library()
+        self.table = table          # keeps the table alive
+        self.V = V
+        N = V.shape[0]
+        if out is None:
+            out = tuple(torch.zeros(N, dtype=torch.float64, device=V.device) for _ in range(3))
+        self.K, self.dK, self.d2K = out
+        arr, n = _requests(requests)
+        ti = None
+        if tiling is not None:
+            ti = k2_batch_tiling(int(tiling.get("nc", 0)), int(tiling.get("cz", 0)),
+                                 int(tiling.get("chunk", 0)), int(tiling.get("c_min_fill", 0)),
+                                 int(tiling.get("wg_per_cu", 0)))
+        h = _vp()
+        _check(L.k2_batch_create(table.handle, n, arr, V.data_ptr(), N, int(flags),
+                                 ctypes.byref(ti) if ti is not None else None, self.K.data_ptr(),
+                                 self.dK.data_ptr(), self.d2K.data_ptr(), ctypes.byref(h)),
+               "k2_batch_create")
+        self._h = h
+
+    def run(self, stream=None):
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(load_library().k2_batch_run(self._h, st), "k2_batch_run")
+        return self.K, self.dK, self.d2K
+
+    def run_direct(self, stream=None):
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(load_library().k2_batch_run_direct(self._h, st), "k2_batch_run_direct")
+        return self.K, self.dK, self.d2K
+
+    def info(self):
+        i = k2_batch_info()
+        _check(load_library().k2_batch_get_info(self._h, ctypes.byref(i)), "k2_batch_get_info")
+        return {"n_items": i.n_items, "grid": i.grid, "n_k": i.n_k,
+                "useful_evals": i.useful_evals, "padded_evals": i.padded_evals,
+                "partial_bytes": i.partial_bytes}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().k2_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def sincos_words(W, mode=0):
