@@ -1,12 +1,17 @@
 """Throughput of the K2 evaluator on one MI355X.
 
-Workload (one "step"): K2 and dK2/dv of 65,536 candidates v in [3.4e18, 3.6e18] (64.64 fixed
-point) against all 2,325,572 synthetic zeros (gamma < 1.3e6, Q18.110 words, complex128 weights),
-k = 1.5e-10, seed 0 -- 1.524e11 candidate-zero evaluations.  Inputs are resident in HBM before
-the timed region.  Metric: candidate-zero evaluations per second, valid only with |dK2| <= 1e-9
-against the multiprecision oracle (checked on the committed 256-candidate fixture).
+Default workload (one "step"): K2 and dK2/dv of 65,536 candidates v in [3.4e18, 3.6e18] (64.64
+fixed point) against all 2,325,572 synthetic zeros (gamma < 1.3e6, Q18.110 words, complex128
+weights), k = 1.5e-10, seed 0 -- 1.524e11 candidate-zero evaluations, plain FP64 form.  Inputs are
+resident in HBM before the timed region.  The metric (candidate-zero evaluations per second)
+counts only with |dK2| <= 1e-9 against the multiprecision oracle, checked on the committed
+256-candidate fixture.
 
-Prints one JSON line.  --form plain (FP64, default) or mixed (FP32 sin/cos after the exact phase).
+Other workloads (--workload): mixed (FP32 form of the same), batched (512 uneven requests in one
+persistent launch, graph replay), polish (4,096 candidates, 8 Newton steps), small (10,000 zeros x
+1,024 candidates), copy (a 2 GiB device copy, the byte-counter calibration for rocprofv3 --pmc).
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -18,11 +23,17 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic work per candidate-zero evaluation of K2 + K2' (see DESIGN.md):
-FP64_FLOPS_PER_EVAL = 42      # sin/cos of the reduced phase + weighting + two accumulations
-INT32_MAD_PER_EVAL = 15       # exact 128x128-bit phase product window
-FP64_PEAK_TFLOPS = 78.6       # MI355X vector FP64, published
-CDNA_CLOCK_HZ = 2.4e9
+# Algorithmic VALU work per candidate-zero evaluation (lane operations), K2 + K2':
+#   plain: 15 INT32 multiply-adds for the exact phase window + 21 FP64 FMA-equivalents
+#          (42 flops: sin/cos of the reduced phase, weighting, two accumulations)
+#   mixed: 15 INT32 multiply-adds + 2 transcendental-equivalents (counted as 2 slots each,
+#          half rate) + 4 FP32 FMA-equivalents (8 flops)
+WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
+        "mixed": {"int32_mad": 15, "trans": 2, "fp32_fma": 4, "slots": 23}}
+# Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
+# published 78.6 TFLOP/s FP64 vector peak counted per FMA (2 flops).
+VALU_PEAK_TOPS = 39.3
+FP64_PEAK_TFLOPS = 78.6
 
 
 def cpu_baseline(tb, V, k, budget_s=20.0, n_c=64, chunk=1 << 18):
@@ -77,8 +88,7 @@ def ref_words_fast(tb):
     s36 = np.uint64(36); s28 = np.uint64(28); s12 = np.uint64(12); s52 = np.uint64(52)
     ghi_lo = lo << s36
     ghi_hi = (hi << s36) | (lo >> s28)
-    # glo = round(G / 2^12)
-    add = np.uint64(1 << 11)
+    add = np.uint64(1 << 11)                 # glo = round(G / 2^12)
     lo2 = lo + add
     carry = (lo2 < lo).astype(np.uint64)
     hi2 = hi + carry
@@ -101,14 +111,217 @@ def cpu_info():
         return {}
 
 
+def traffic_summary(workload):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        d = json.load(open(p))
+        return d.get(workload, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def roofline(form, evals, kern_ms, alg_bytes, workload):
+    w = WORK[form]
+    rate = evals / (kern_ms * 1e-3)
+    achieved = rate * w["slots"] / 1e12
+    r = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+         "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic_summary(workload),
+         "kernel_ms": kern_ms, "kernel_evals_per_s": rate, "work_per_eval": w,
+         "int32_mad_tops": rate * w["int32_mad"] / 1e12,
+         "algorithmic_bytes": alg_bytes,
+         "hbm_GBps_algorithmic": alg_bytes / (kern_ms * 1e-3) / 1e9}
+    if form == "plain":
+        r["fp64_tflops"] = rate * 2 * w["fp64_fma"] / 1e12
+        r["fp64_frac"] = r["fp64_tflops"] / FP64_PEAK_TFLOPS
+    return r
+
+
+def timed(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    e0.record(st)
+    for _ in range(steps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, time.perf_counter() - w0
+
+
+def base_record(args, metric, value, unit, ms, dtype, config):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded RvM zeros, log-normal |zeta'|, uniform candidates)",
+            "config": config}
+
+
+def run_eval(args, form):
+    import numpy as np
+    import torch
+    from mertens_bound_amd import autotune
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import synth
+    c = synth.CONFIG1 if args.workload == "small" else synth.CONFIG2
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    n_cand = args.n_cand or c["n_cand"]
+    V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
+    torch.cuda.set_device(0)
+    t = ev.K2Table(tb)
+    if args.nc or args.target_blocks:
+        t.set_tiling(nc=args.nc or None, target_blocks=args.target_blocks or None)
+    else:
+        autotune.apply_tuned(t, n_cand, tb.n, "mixed" if form == "mixed" else "plain")
+    Vd = ev.v_tensor(V)
+    flags = ev.VALUE | ev.D1 | (ev.MIXED if form == "mixed" else 0)
+    out = tuple(torch.empty(n_cand, dtype=torch.float64, device="cuda") for _ in range(3))
+    st = torch.cuda.current_stream().cuda_stream
+
+    acc = None
+    gname = "config1_oracle.npz" if args.workload == "small" else "config2_sub256.npz"
+    gpath = os.path.join(ROOT, "tests", "golden", gname)
+    if os.path.exists(gpath) and not args.no_check:
+        g = np.load(gpath, allow_pickle=False)
+        m = min(len(g["K"]), n_cand)
+        if np.array_equal(g["V"][:m], V[:m]):
+            K, _, _ = t.eval(V[:m], c["k"], flags=flags)
+            acc = float(np.max(np.abs(K - g["K"][:m])))
+
+    def step():
+        t.eval_device(Vd, c["k"], flags=flags, out=out, stream=st)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t.profile(True)
+    ms, wall = timed(step, args.steps, 0, torch)
+    k_ms, k_n = t.kernel_time()
+    t.profile(False)
+    evals = float(n_cand) * tb.n
+    alg_bytes = tb.n * 48 + n_cand * (16 + 16)     # zero records read + candidates + outputs
+    res = base_record(
+        args, "candidate-zero evals/sec (K2 + dK2/dv, 2.33M zeros) at |dK2|<=1e-9 vs oracle",
+        evals / (ms * 1e-3), "candidate-zero evals/s", ms,
+        "f64" if form == "plain" else "f32+f64",
+        {"workload": f"K2+dK2/dv, {n_cand:,} candidates x {tb.n:,} zeros, k={c['k']}",
+         "form": form, "n_cand": n_cand, "n_zeros": tb.n, "tiling": t.get_tiling()})
+    res["accuracy"] = {"max_abs_dK2_vs_oracle": acc, "oracle_sample": gname, "bar": 1e-9,
+                       "ok": (acc is not None and acc <= 1e-9)}
+    res["roofline"] = roofline(form, evals, k_ms / max(1, k_n), alg_bytes,
+                               f"{args.workload}_{form}")
+    res["wall_s"] = wall
+    if not args.no_cpu and args.workload == "metric":
+        try:
+            cb = cpu_baseline(tb, V, c["k"])
+            cb["host"] = cpu_info()
+            res["cpu_baseline"] = cb
+        except Exception as exc:  # reported, never silently dropped
+            res["cpu_baseline"] = {"error": repr(exc)}
+    return res
+
+
+def run_batched(args):
+    import numpy as np
+    import torch
+    from mertens_bound_amd import autotune
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import synth
+    c = synth.CONFIG2
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    torch.cuda.set_device(0)
+    t = ev.K2Table(tb)
+    nz, ks, cnt, offs, V = synth.make_batched_requests(tb, n_req=512, seed=0)
+    reqs = [(int(a), int(b), int(o), float(k)) for a, b, o, k in zip(nz, cnt, offs, ks)]
+    Vt = ev.v_tensor(V)
+    form = "mixed" if args.form == "mixed" else "plain"
+    flags = ev.VALUE | ev.D1 | (ev.MIXED if form == "mixed" else 0)
+    b = ev.K2Batch(t, reqs, Vt, flags=flags, tiling=autotune.batch_tiling(form))
+    useful = float(np.sum(cnt.astype(float) * nz))
+    ms, wall = timed(b.run, args.steps, args.warmup, torch)
+    # the same requests one launch each, for comparison
+    out = tuple(torch.zeros(V.shape[0], dtype=torch.float64, device="cuda") for _ in range(3))
+
+    def loop():
+        for a, n, o, k in reqs:
+            t.eval_device(Vt[o:o + n], k, n_zeros=a, flags=flags,
+                          out=(out[0][o:o + n], out[1][o:o + n], out[2][o:o + n]))
+
+    ms_loop, _ = timed(loop, 1, 1, torch)
+    d = (out[0] - b.K).abs().max().item()
+    info = b.info()
+    res = base_record(args, "candidate-zero evals/sec, 512 uneven requests (one graph launch)",
+                      useful / (ms * 1e-3), "candidate-zero evals/s", ms,
+                      "f64" if form == "plain" else "f32+f64",
+                      {"workload": "512 requests: T log-uniform [1e5,1.3e6], k in "
+                                   "{3e-9,1e-9,1.5e-10}, counts log-uniform [1,4096], 2.33M-zero "
+                                   "table", "form": form, "batch": info})
+    res["per_request_loop"] = {"ms": ms_loop, "evals_per_s": useful / (ms_loop * 1e-3),
+                               "max_abs_diff_vs_batch": d}
+    res["roofline"] = roofline(form, useful, ms, tb.n * 48 * 3 + V.shape[0] * 32,
+                               f"batched_{form}")
+    res["wall_s"] = wall
+    return res
+
+
+def run_polish(args):
+    import numpy as np
+    import torch
+    from mertens_bound_amd import autograd as ag
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import synth, workloads
+    c = synth.CONFIG2
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    torch.cuda.set_device(0)
+    t = ev.K2Table(tb)
+    V0 = workloads.polish_candidates(t, 4096, seed=0)
+    V0t = ev.v_tensor(V0)
+    k = c["k"]
+
+    def step():
+        ag.polish(t, V0t, k, n_steps=8)
+
+    ms, wall = timed(step, args.steps, args.warmup, torch)
+    V, K, dK, steps = ag.polish(t, V0t, k, n_steps=8)
+    torch.cuda.synchronize()
+    passes = 1 + 8 + 1                     # start value, 8 Newton evaluations, final value
+    evals = float(V0.shape[0]) * tb.n * passes
+    res = base_record(args, "candidate-zero evals/sec, Newton polish (K2, K2', K2'')",
+                      evals / (ms * 1e-3), "candidate-zero evals/s", ms, "f64",
+                      {"workload": "4,096 candidates near synthetic peaks (v0 + N(0,1e-3)) x "
+                                   "2,325,572 zeros, 8 Newton steps", "passes": passes})
+    res["polish"] = {"mean_steps": float(steps.float().mean().item()),
+                     "max_abs_dK_final": float(dK.abs().max().item()),
+                     "mean_abs_K": float(K.abs().mean().item())}
+    res["wall_s"] = wall
+    return res
+
+
+def run_copy(args):
+    import torch
+    n = 1 << 28                                  # 2 GiB of float64
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    ms, wall = timed(lambda: y.copy_(x), args.steps, args.warmup, torch)
+    return {"metric": "device copy (byte-counter calibration)", "value": 2 * 8 * n / (ms * 1e-3),
+            "unit": "B/s", "bytes_read": 8 * n, "bytes_written": 8 * n, "ms": ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["metric", "mixed", "batched", "polish", "small",
+                                           "copy"], default="metric")
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
-    ap.add_argument("--n-cand", type=int, default=65_536)
+    ap.add_argument("--n-cand", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--nc", type=int, default=0)
     ap.add_argument("--target-blocks", type=int, default=0)
     args = ap.parse_args()
@@ -116,98 +329,16 @@ def main():
         print("bench.py runs on one GPU only: the evaluator shards by candidates with no "
               "communication, and multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
-    import torch
-    from mertens_bound_amd import evaluator as ev
-    from mertens_bound_amd import synth
-
-    c = synth.CONFIG2
-    t_gen = time.perf_counter()
-    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
-    V = synth.make_candidates(args.n_cand, c["v_lo"], c["v_hi"], c["seed"])
-    print(f"[bench] inputs generated in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr,
-          flush=True)
-    torch.cuda.set_device(0)
-    t = ev.K2Table(tb)
-    if args.nc or args.target_blocks:
-        t.set_tiling(nc=args.nc or None, target_blocks=args.target_blocks or None)
-    Vd = ev.v_tensor(V)
-    flags = ev.VALUE | ev.D1 | (ev.MIXED if args.form == "mixed" else 0)
-    n = Vd.shape[0]
-    out = tuple(torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3))
-    stream = torch.cuda.current_stream()
-
-    # accuracy gate: the first 256 candidates against the multiprecision oracle fixture
-    acc = None
-    gpath = os.path.join(ROOT, "tests", "golden", "config2_sub256.npz")
-    if os.path.exists(gpath):
-        g = np.load(gpath, allow_pickle=False)
-        if np.array_equal(g["V"], V[:256]):
-            K, dK, _ = t.eval(V[:256], c["k"], flags=flags)
-            acc = float(np.max(np.abs(K - g["K"])))
-
-    for _ in range(args.warmup):
-        t.eval_device(Vd, c["k"], flags=flags, out=out, stream=stream.cuda_stream)
-    torch.cuda.synchronize()
-    t.profile(True)
-    e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        t.eval_device(Vd, c["k"], flags=flags, out=out, stream=stream.cuda_stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    ms = e0.elapsed_time(e1) / args.steps
-    k_ms, k_n = t.kernel_time()
-    t.profile(False)
-    kern_ms = k_ms / max(1, k_n)
-    evals = float(n) * tb.n
-    value = evals / (ms * 1e-3)
-    kern_rate = evals / (kern_ms * 1e-3)
-    fp64_tflops = kern_rate * FP64_FLOPS_PER_EVAL / 1e12
-    int_tops = kern_rate * INT32_MAD_PER_EVAL / 1e12
-    # HBM bytes that must move per launch: the zero table records read by the kernel
-    # (16 B phase word + 16 B weight + 16 B derivative weight per zero) plus the candidates and
-    # the outputs
-    alg_bytes = tb.n * 48 + n * (16 + 16)
-    res = {
-        "metric": "candidate-zero evals/sec (K2 + dK2/dv, 2.33M zeros) at |dK2|<=1e-9 vs oracle",
-        "value": value,
-        "unit": "candidate-zero evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.form == "plain" else "f32+f64",
-        "data": "synthetic (seeded RvM zeros, log-normal |zeta'|, uniform candidates)",
-        "config": {"workload": "K2+dK2/dv, 65,536 candidates x 2,325,572 zeros, k=1.5e-10",
-                   "form": args.form, "n_cand": n, "n_zeros": tb.n, "tiling": t.get_tiling()},
-        "accuracy": {"max_abs_dK2_vs_oracle_256": acc, "bar": 1e-9,
-                     "ok": (acc is not None and acc <= 1e-9)},
-        "roofline": {"bound": "valu", "achieved": fp64_tflops, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": fp64_tflops / FP64_PEAK_TFLOPS,
-                     "traffic": None,
-                     "kernel_ms": kern_ms, "kernel_evals_per_s": kern_rate,
-                     "int32_mad_tops": int_tops,
-                     "algorithmic_bytes": alg_bytes,
-                     "hbm_GBps_algorithmic": alg_bytes / (kern_ms * 1e-3) / 1e9,
-                     "work_per_eval": {"fp64_flops": FP64_FLOPS_PER_EVAL,
-                                       "int32_mad": INT32_MAD_PER_EVAL}},
-        "wall_s": wall,
-    }
-    if not args.no_cpu:
-        try:
-            cb = cpu_baseline(tb, V, c["k"])
-            cb["host"] = cpu_info()
-            res["cpu_baseline"] = cb
-        except Exception as exc:  # reported, never silently dropped
-            res["cpu_baseline"] = {"error": repr(exc)}
+    if args.workload in ("metric", "small"):
+        res = run_eval(args, args.form)
+    elif args.workload == "mixed":
+        res = run_eval(args, "mixed")
+    elif args.workload == "batched":
+        res = run_batched(args)
+    elif args.workload == "polish":
+        res = run_polish(args)
+    else:
+        res = run_copy(args)
     print(json.dumps(res))
 
 

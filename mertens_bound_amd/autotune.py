@@ -1,0 +1,178 @@
+"""Tile autotuner for the evaluation launches.
+
+Tunable shapes:
+  one-launch evaluation  nc (candidates per lane; 256*nc candidates per workgroup) x
+                         target_blocks (workgroups the zero splits aim at, i.e. zeros per split)
+  batched plan           nc x cz (candidates per Z tile) x chunk (zeros per work item) x
+                         wg_per_cu (resident workgroups of the persistent grid)
+
+Each candidate shape is timed with HIP events on the GPU; the fastest per shape class is written
+to tuned_tiles.json beside this file, which the host layer consults (``apply_tuned``,
+``batch_tiling``).  Shape classes bucket the candidate and zero counts by powers of two.
+
+Run on the GPU:  python -m mertens_bound_amd.autotune [--out PATH] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import math
+import os
+import sys
+import time
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+TUNED = os.path.join(PKG, "tuned_tiles.json")
+
+EVAL_GRID = {"nc": [1, 2, 4, 8], "target_blocks": [768, 1536, 2048, 3072, 4096, 6144]}
+BATCH_GRID = {"nc": [2, 4], "cz": [8, 16], "chunk": [4096, 8192, 16384], "wg_per_cu": [0, 2]}
+
+_cache = None
+
+
+def _bucket(n: int) -> int:
+    return int(round(math.log2(max(1, n))))
+
+
+def eval_key(n_cand: int, n_zeros: int, form: str) -> str:
+    return f"eval/{form}/c{_bucket(n_cand)}/z{_bucket(n_zeros)}"
+
+
+def load(path: str = TUNED) -> dict:
+    global _cache
+    if _cache is None:
+        try:
+            with open(path) as f:
+                _cache = json.load(f)
+        except (OSError, ValueError):
+            _cache = {}
+    return _cache
+
+
+def lookup_eval(n_cand: int, n_zeros: int, form: str):
+    """Tuned one-launch tiling for the nearest shape class of this form, or None."""
+    tab = load().get("eval", {})
+    best, dist = None, None
+    cb, zb = _bucket(n_cand), _bucket(n_zeros)
+    for key, rec in tab.items():
+        _, f, c, z = key.split("/")
+        if f != form:
+            continue
+        d = abs(int(c[1:]) - cb) + abs(int(z[1:]) - zb)
+        if dist is None or d < dist:
+            best, dist = rec["tiling"], d
+    return best
+
+
+def apply_tuned(table, n_cand: int, n_zeros: int, form: str = "plain") -> bool:
+    ti = lookup_eval(n_cand, n_zeros, form)
+    if ti is None:
+        return False
+    table.set_tiling(nc=ti["nc"], target_blocks=ti["target_blocks"])
+    return True
+
+
+def batch_tiling(form: str = "plain"):
+    rec = load().get("batch", {}).get(form)
+    return None if rec is None else rec["tiling"]
+
+
+# ---- tuning (GPU) --------------------------------------------------------------------------
+
+def _time(fn, torch, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_eval(table, V, k, form, grid=EVAL_GRID, log=print):
+    import torch
+    from . import evaluator as ev
+    flags = ev.VALUE | ev.D1 | (ev.MIXED if form == "mixed" else 0)
+    n = V.shape[0]
+    out = tuple(torch.empty(n, dtype=torch.float64, device=V.device) for _ in range(3))
+    results = []
+    for nc, tb in itertools.product(grid["nc"], grid["target_blocks"]):
+        table.set_tiling(nc=nc, target_blocks=tb)
+        ms = _time(lambda: table.eval_device(V, k, flags=flags, out=out), torch)
+        results.append({"tiling": {"nc": nc, "target_blocks": tb}, "ms": ms})
+        log(f"  eval {form} n={n} nc={nc} target_blocks={tb}: {ms:.3f} ms")
+    results.sort(key=lambda r: r["ms"])
+    return results
+
+
+def tune_batch(table, reqs, V, form, grid=BATCH_GRID, log=print):
+    import torch
+    from . import evaluator as ev
+    flags = ev.VALUE | ev.D1 | (ev.MIXED if form == "mixed" else 0)
+    results = []
+    for nc, cz, chunk, wpc in itertools.product(grid["nc"], grid["cz"], grid["chunk"],
+                                                grid["wg_per_cu"]):
+        ti = {"nc": nc, "cz": cz, "chunk": chunk, "wg_per_cu": wpc}
+        b = ev.K2Batch(table, reqs, V, flags=flags, tiling=ti)
+        ms = _time(b.run, torch, reps=2)
+        info = b.info()
+        b.close()
+        results.append({"tiling": ti, "ms": ms, "grid": info["grid"], "items": info["n_items"]})
+        log(f"  batch {form} {ti}: {ms:.2f} ms (grid {info['grid']}, items {info['n_items']})")
+    results.sort(key=lambda r: r["ms"])
+    return results
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=TUNED)
+    ap.add_argument("--quick", action="store_true", help="smaller grids")
+    args = ap.parse_args()
+    import numpy as np
+    import torch
+    from . import evaluator as ev
+    from . import synth
+
+    def log(msg):
+        print(msg, flush=True)
+
+    eg = dict(EVAL_GRID)
+    bg = dict(BATCH_GRID)
+    if args.quick:
+        eg = {"nc": [2, 4], "target_blocks": [1536, 2048, 3072]}
+        bg = {"nc": [4], "cz": [8, 16], "chunk": [4096, 8192], "wg_per_cu": [0]}
+    torch.cuda.set_device(0)
+    c2 = synth.CONFIG2
+    big = synth.make_zero_table(c2["n_zeros"], c2["seed"])
+    t_big = ev.K2Table(big)
+    small = synth.make_zero_table(10_000, 0)
+    t_small = ev.K2Table(small)
+    out = {"eval": {}, "batch": {}, "device": torch.cuda.get_device_name(0),
+           "created": time.strftime("%Y-%m-%d %H:%M:%S")}
+    shapes = [(t_big, big.n, 65_536), (t_big, big.n, 4_096), (t_small, small.n, 1_024)]
+    for form in ("plain", "mixed"):
+        for t, nz, nc in shapes:
+            V = ev.v_tensor(synth.make_candidates(nc, 3.4e18, 3.6e18, 1))
+            res = tune_eval(t, V, 1.5e-10, form, eg, log)
+            out["eval"][eval_key(nc, nz, form)] = {"tiling": res[0]["tiling"], "ms": res[0]["ms"],
+                                                   "all": res}
+            log(f"best {eval_key(nc, nz, form)}: {res[0]}")
+    nzr, ks, cnt, offs, Vb = synth.make_batched_requests(big, n_req=512, seed=0)
+    reqs = [(int(a), int(b), int(o), float(k)) for a, b, o, k in zip(nzr, cnt, offs, ks)]
+    Vbt = ev.v_tensor(Vb)
+    for form in ("plain", "mixed"):
+        res = tune_batch(t_big, reqs, Vbt, form, bg, log)
+        out["batch"][form] = {"tiling": res[0]["tiling"], "ms": res[0]["ms"], "all": res}
+        log(f"best batch {form}: {res[0]}")
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    log(f"wrote {args.out}")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

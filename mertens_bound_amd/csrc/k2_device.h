@@ -41,6 +41,18 @@ __device__ __forceinline__ uint64_t mad_carry(uint32_t a, uint32_t b, uint64_t c
     return r;
 }
 
+// a*b + c over 64 bits, returning the carry out of bit 64 in k (first sum of a column)
+__device__ __forceinline__ uint64_t mad_carry0(uint32_t a, uint32_t b, uint64_t c, uint32_t& k) {
+    uint64_t r;
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %1, 0, 1, vcc"
+        : "=&v"(r), "=v"(k)
+        : "v"(a), "v"(b), "v"(c)
+        : "vcc");
+    return r;
+}
+
 // exact W via a straightforward 256-bit schoolbook product (rare fix-up path and debug use)
 __device__ __noinline__ uint64_t phase_word_exact(Limbs4 g, Limbs4 v) {
     const uint32_t G[4] = {g.x0, g.x1, g.x2, g.x3};
@@ -65,14 +77,14 @@ __device__ __noinline__ uint64_t phase_word_exact(Limbs4 g, Limbs4 v) {
 __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
     // column 2 (weight 2^64): g0 v2 + g1 v1 + g2 v0; a carry out of a 64-bit add shows as the
     // new sum being below the old one
-    uint32_t k2 = 0;
+    uint32_t k2;
     uint64_t t = mul32(g.x0, v.x2);
-    t = mad_carry(g.x1, v.x1, t, k2);
+    t = mad_carry0(g.x1, v.x1, t, k2);
     t = mad_carry(g.x2, v.x0, t, k2);
     // column 3 (2^96): carry-in hi32(t) + k2 2^32
     uint64_t s3 = (t >> 32) | ((uint64_t)k2 << 32);
-    uint32_t k3 = 0;
-    s3 = mad_carry(g.x0, v.x3, s3, k3);
+    uint32_t k3;
+    s3 = mad_carry0(g.x0, v.x3, s3, k3);
     s3 = mad_carry(g.x1, v.x2, s3, k3);
     s3 = mad_carry(g.x2, v.x1, s3, k3);
     s3 = mad_carry(g.x3, v.x0, s3, k3);
@@ -98,8 +110,8 @@ __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
 // [130,142) of the rest to be all ones.
 __device__ __forceinline__ uint32_t phase_word_hi32(Limbs4 g, Limbs4 v) {
     uint64_t s3 = mul32(g.x0, v.x3);
-    uint32_t k3 = 0;
-    s3 = mad_carry(g.x1, v.x2, s3, k3);
+    uint32_t k3;
+    s3 = mad_carry0(g.x1, v.x2, s3, k3);
     s3 = mad_carry(g.x2, v.x1, s3, k3);
     s3 = mad_carry(g.x3, v.x0, s3, k3);
     uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
@@ -155,8 +167,9 @@ __device__ __forceinline__ void sincos_poly(double x, double& s, double& c) {
 
 __device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const double2* tab,
                                              double& S, double& C) {
-    const uint32_t k = ((whi + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT) & (K2_TAB - 1);
-    const int32_t rhi = (int32_t)(whi - (k << K2_TAB_SHIFT));      // r in [-2^54, 2^54)
+    const uint32_t y = whi + (1u << (K2_TAB_SHIFT - 1));
+    const uint32_t k = y >> K2_TAB_SHIFT;                            // nearest table point
+    const int32_t rhi = (int32_t)(y & ((1u << K2_TAB_SHIFT) - 1)) - (1 << (K2_TAB_SHIFT - 1));
     const double r = fma((double)rhi, 4294967296.0, (double)wlo);
     const double x = r * K2_TWO_PI_2M64;
     const double z = x * x;
@@ -170,8 +183,9 @@ __device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const d
 // FP32 form: W32 = top 32 bits of the phase word; |x| <= pi/512, x - x^3/6 and 1 - x^2/2 are
 // exact to FP32 rounding (next terms < 2e-11 relative)
 __device__ __forceinline__ void sincos_tab32(uint32_t W, const float2* tab, float& S, float& C) {
-    const uint32_t k = ((W + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT) & (K2_TAB - 1);
-    const int32_t r = (int32_t)(W - (k << K2_TAB_SHIFT));           // |r| <= 2^22: exact in FP32
+    const uint32_t y = W + (1u << (K2_TAB_SHIFT - 1));
+    const uint32_t k = y >> K2_TAB_SHIFT;
+    const int32_t r = (int32_t)(y & ((1u << K2_TAB_SHIFT) - 1)) - (1 << (K2_TAB_SHIFT - 1));
     const float x = (float)r * 1.4629181e-09f;                       // 2 pi / 2^32
     const float z = x * x;
     const float sp = fmaf(x * z, -1.6666667e-01f, x);
