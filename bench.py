@@ -36,7 +36,7 @@ VALU_PEAK_TOPS = 39.3
 FP64_PEAK_TFLOPS = 78.6
 
 
-def cpu_baseline(tb, V, k, budget_s=20.0, n_c=64, chunk=1 << 18):
+def cpu_baseline(tb, V, k, budget_s=20.0, n_c=256, chunk=1 << 17):
     """The reference evaluator (oracle/_ref, compiled from search/path1_eval_fast.cpp; 16 OpenMP
     threads, hard-coded there) on a bounded sample: compute_peak_AB over zero chunks plus the
     weighted sums K = sum e^{-k g^2} A and K' = -sum g e^{-k g^2} B.  Falls back to the oracle's
