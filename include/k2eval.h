@@ -136,6 +136,18 @@ int k2_polish_device(k2_table *t, uint32_t n_cand, uint64_t *V_dev, double k, ui
                      int n_steps, double clamp, uint32_t flags, double *K_dev, double *dK_dev,
                      double *d2K_dev, int32_t *steps_dev, void *stream);
 
+/* Staged pre-screen on the first zeros (device buffers): for shifts m = -n_shift..n_shift the
+ * sums h_m = K2 over the first n zeros at v + m*shift_step, stage value max_m |h_m|; stage s
+ * (stage_zeros[s] zeros, increasing, <= 1024) continues only if its value reaches
+ * stage_thresholds[s].  val_dev: [n_cand][4] stage values (-1 where not reached), best_shift_dev:
+ * argmax m of the last completed stage, pass_dev: 1 if all stages passed.  FP32 sums after the
+ * exact top-32-bit phase.  Replaces the leaf screen of search/gpu_mertens_stream.cu:186-250
+ * (64/128/256 zeros, 7 shifts of 0.0014, thresholds 0.78/0.855/0.892). */
+int k2_screen_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double k,
+                     double shift_step, int n_shift, int n_stages, const int32_t *stage_zeros,
+                     const float *stage_thresholds, float *val_dev, int32_t *best_shift_dev,
+                     int32_t *pass_dev, void *stream);
+
 /* Debug: phase words floor(G_j V_c / 2^110) mod 2^64 for zeros [z0, z0+nz) (host buffers,
  * out[c*nz + j]).  mode 0 = evaluator path, 1 = schoolbook product, 2 = the FP32 form's top-32
  * word in the high half. */

@@ -124,6 +124,7 @@ extern "C" int k2_table_destroy(k2_table* t) {
     (void)hipFree(t->g); (void)hipFree(t->gamma); (void)hipFree(t->w);
     (void)hipFree(t->tab64); (void)hipFree(t->tab32);
     (void)hipFree(t->partial); (void)hipFree(t->host_io); (void)hipFree(t->polish_active);
+    (void)hipFree(t->screen_params);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return K2_OK;

@@ -77,6 +77,7 @@ struct k2_table {
     void* host_io = nullptr;   // device staging for host-buffer entry points
     size_t host_io_bytes = 0;
     int* polish_active = nullptr;
+    int32_t* screen_params = nullptr;   // stage zero counts and thresholds of k2_screen_device
     size_t polish_cap = 0;
     hipStream_t stream = nullptr;
     k2_tiling tiling{4, 2048, 8192};

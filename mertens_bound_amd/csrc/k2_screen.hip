@@ -1,0 +1,182 @@
+// Staged pre-screen of candidates on the first zeros of the table, with shifted copies.
+//
+// Semantics of the reference's GPU leaf screen (search/gpu_mertens_stream.cu:186-250): for each
+// candidate v, the sums h_m = sum_{j<n} 2 e^{-k g_j^2} Re(w_j e^{i g_j (v + m d)}) over the first
+// n zeros at 2M+1 shifts m = -M..M (reference: M = 3, d = 0.0014), the stage value max_m |h_m|,
+// and staged cutoffs: stage s continues only if the value after n_s zeros reaches thr_s
+// (reference: 64/128/256 zeros, 0.78/0.855/0.892).  Here the phase is the exact top-32-bit word
+// (not a wrapped 64-bit approximation) and sin/cos come from the FP32 LDS table (not a 12-bit
+// cos lookup); the shifted terms use one rotation per shift step, e^{i g_j d}, per zero.
+//
+// One lane per candidate; the zeros of all stages (<= K2_SCREEN_MAX) are staged in LDS once per
+// workgroup.  A lane that fails a stage stops accumulating; its later stage values stay -1.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "k2_device.h"
+#include "k2_internal.h"
+#include "k2_tiles.h"
+
+using namespace k2dev;
+
+#define K2_SCREEN_MAX 1024   // zeros the screen can stage
+#define K2_SCREEN_STAGES 4
+#define K2_SCREEN_MAXM 8     // shifts -M..M, M <= 8
+
+struct ScreenZero {
+    uint4 g;
+    float2 a;     // 2 e^{-k g^2} w (FP32)
+    float2 r;     // e^{i g d}
+};
+
+template <int M>
+__global__ __launch_bounds__(256) void k2_screen_kernel(
+    const uint4* __restrict__ zg, const double* __restrict__ gamma, const double2* __restrict__ w,
+    double k, double d, int n_zeros, const uint4* __restrict__ cand, int n_cand, int n_stages,
+    const int* __restrict__ stage_nz, const float* __restrict__ stage_thr,
+    const float2* __restrict__ tab, float* __restrict__ out_val, int* __restrict__ out_shift,
+    int* __restrict__ out_pass) {
+    __shared__ ScreenZero sz[K2_SCREEN_MAX];
+    __shared__ float2 s_tab[K2_TAB];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < K2_TAB; i += blockDim.x) s_tab[i] = tab[i];
+    for (int j = tid; j < n_zeros; j += blockDim.x) {
+        const double g = gamma[j];
+        const double f = 2.0 * exp(-k * g * g);
+        const double2 ww = w[j];
+        double sr, cr;
+        sincos_general(g * d, sr, cr);
+        sz[j] = ScreenZero{zg[j], make_float2((float)(f * ww.x), (float)(f * ww.y)),
+                           make_float2((float)cr, (float)sr)};
+    }
+    __syncthreads();
+    const int c = blockIdx.x * blockDim.x + tid;
+    if (c >= n_cand) return;
+    const uint4 cv = cand[c];
+    const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
+    float acc[2 * M + 1];
+#pragma unroll
+    for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
+    int j = 0, best = M, passed = 1;
+    for (int s = 0; s < n_stages; ++s) {
+        const int nz = stage_nz[s];
+        for (; j < nz; ++j) {
+            const ScreenZero z = sz[j];
+            const uint32_t W = phase_word_hi32(Limbs4{z.g.x, z.g.y, z.g.z, z.g.w}, v);
+            float S, C;
+            sincos_tab32(W, s_tab, S, C);
+            // q = a e^{i th}
+            const float qr = fmaf(z.a.x, C, -z.a.y * S);
+            const float qi = fmaf(z.a.x, S, z.a.y * C);
+            acc[M] += qr;
+            float pr = qr, pi = qi, nr = qr, ni = qi;
+#pragma unroll
+            for (int m = 1; m <= M; ++m) {
+                const float t1 = fmaf(pr, z.r.x, -pi * z.r.y);   // p *= r
+                pi = fmaf(pr, z.r.y, pi * z.r.x);
+                pr = t1;
+                const float t2 = fmaf(nr, z.r.x, ni * z.r.y);    // n *= conj(r)
+                ni = fmaf(ni, z.r.x, -nr * z.r.y);
+                nr = t2;
+                acc[M + m] += pr;
+                acc[M - m] += nr;
+            }
+        }
+        float mv = -1.0f;
+#pragma unroll
+        for (int m = 0; m < 2 * M + 1; ++m) {
+            const float a = fabsf(acc[m]);
+            if (a > mv) { mv = a; best = m; }
+        }
+        out_val[(size_t)c * K2_SCREEN_STAGES + s] = mv;
+        if (mv < stage_thr[s]) {
+            passed = 0;
+            for (int t = s + 1; t < K2_SCREEN_STAGES; ++t) out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
+            break;
+        }
+    }
+    for (int t = n_stages; t < K2_SCREEN_STAGES; ++t) out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
+    out_shift[c] = best - M;
+    out_pass[c] = passed;
+}
+
+namespace k2impl {
+
+template <int M>
+static hipError_t launch_screen_m(const uint4* zg, const double* gamma, const double2* w, double k,
+                                  double d, int n_zeros, const uint4* cand, int n_cand,
+                                  int n_stages, const int* nz, const float* thr, const float2* tab,
+                                  float* val, int* shift, int* pass, hipStream_t st) {
+    hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st, zg,
+                       gamma, w, k, d, n_zeros, cand, n_cand, n_stages, nz, thr, tab, val, shift,
+                       pass);
+    return hipGetLastError();
+}
+
+hipError_t launch_screen(int M, const uint4* zg, const double* gamma, const double2* w, double k,
+                         double d, int n_zeros, const uint4* cand, int n_cand, int n_stages,
+                         const int* nz, const float* thr, const float2* tab, float* val,
+                         int* shift, int* pass, hipStream_t st) {
+    switch (M) {
+#define K2_M(X) case X: return launch_screen_m<X>(zg, gamma, w, k, d, n_zeros, cand, n_cand, \
+                                                  n_stages, nz, thr, tab, val, shift, pass, st);
+        K2_M(0) K2_M(1) K2_M(2) K2_M(3) K2_M(4) K2_M(5) K2_M(6) K2_M(7) K2_M(8)
+#undef K2_M
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace k2impl
+
+// ---- C ABI -------------------------------------------------------------------------------------
+
+using namespace k2impl;
+
+extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_dev, double k,
+                                double shift_step, int n_shift, int n_stages,
+                                const int32_t* stage_zeros, const float* stage_thresholds,
+                                float* val_dev, int32_t* best_shift_dev, int32_t* pass_dev,
+                                void* stream) {
+    if (!t || (n_cand && (!V_dev || !val_dev || !best_shift_dev || !pass_dev)) || !stage_zeros ||
+        !stage_thresholds)
+        return set_error(K2_E_ARG, "k2_screen_device: null pointer");
+    if (n_stages < 1 || n_stages > K2_SCREEN_STAGES || n_shift < 0 || n_shift > K2_SCREEN_MAXM)
+        return set_error(K2_E_ARG, "k2_screen_device: 1..4 stages and 0..8 shifts per side");
+    int prev = 0;
+    for (int s = 0; s < n_stages; ++s) {
+        if (stage_zeros[s] <= prev || stage_zeros[s] > K2_SCREEN_MAX || (uint32_t)stage_zeros[s] > t->n)
+            return set_error(K2_E_ARG, "k2_screen_device: stage zero counts must increase, <= 1024");
+        prev = stage_zeros[s];
+    }
+    if (!(k >= 0.0) || !(shift_step == shift_step))
+        return set_error(K2_E_ARG, "k2_screen_device: bad k or shift step");
+    if (n_cand == 0) return K2_OK;
+    std::lock_guard<std::mutex> lk(t->mu);
+    hipError_t e = hipSetDevice(t->device);
+    hipStream_t st = (hipStream_t)stream;
+    // stage parameters travel in a small device buffer owned by the table
+    const size_t need = 2 * K2_SCREEN_STAGES * 4;
+    if (e == hipSuccess && t->screen_params == nullptr)
+        e = hipMalloc((void**)&t->screen_params, 256);
+    int32_t hostp[2 * K2_SCREEN_STAGES] = {0};
+    for (int s = 0; s < n_stages; ++s) {
+        hostp[s] = stage_zeros[s];
+        float f = stage_thresholds[s];
+        memcpy(&hostp[K2_SCREEN_STAGES + s], &f, 4);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->screen_params, hostp, need, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess)
+        e = launch_screen(n_shift, t->g, t->gamma, (const double2*)t->w, k, shift_step,
+                          stage_zeros[n_stages - 1], (const uint4*)V_dev, (int)n_cand, n_stages,
+                          (const int*)t->screen_params,
+                          (const float*)(t->screen_params + K2_SCREEN_STAGES), t->tab32, val_dev,
+                          best_shift_dev, pass_dev, st);
+    if (e != hipSuccess) return set_error(K2_E_HIP, std::string("k2_screen_device: ") + hipGetErrorString(e));
+    return K2_OK;
+}

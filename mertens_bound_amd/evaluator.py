@@ -81,6 +81,9 @@ def _declare(L):
     L.k2_batch_run_direct.argtypes = [_vp, _vp]
     L.k2_batch_get_info.argtypes = [_vp, ctypes.POINTER(k2_batch_info)]
     L.k2_batch_destroy.argtypes = [_vp]
+    L.k2_screen_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_float), _vp, _vp, _vp, _vp]
     L.k2_polish_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                    ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp, _vp,
                                    _vp, _vp]
@@ -234,6 +237,24 @@ class K2Table:
             int(flags), K.data_ptr(), dK.data_ptr(), d2K.data_ptr(), steps.data_ptr(), st),
             "k2_polish_device")
         return K, dK, d2K, steps
+
+    def screen_device(self, V, k, shift_step=0.0014, n_shift=3,
+                      stages=((64, 0.78), (128, 0.855), (256, 0.892)), stream=None):
+        """Staged pre-screen (k2_screen_device).  Returns (values [n, 4] float32 with -1 where a
+        stage was not reached, best shift index [n] int32, passed-all-stages [n] int32)."""
+        import torch
+        n = V.shape[0]
+        nz = (ctypes.c_int32 * len(stages))(*[int(a) for a, _ in stages])
+        th = (ctypes.c_float * len(stages))(*[float(b) for _, b in stages])
+        val = torch.empty((n, 4), dtype=torch.float32, device=V.device)
+        best = torch.empty(n, dtype=torch.int32, device=V.device)
+        ok = torch.empty(n, dtype=torch.int32, device=V.device)
+        st = stream if stream is not None else torch.cuda.current_stream(V.device).cuda_stream
+        _check(load_library().k2_screen_device(self._h, n, V.data_ptr(), float(k),
+                                               float(shift_step), int(n_shift), len(stages), nz,
+                                               th, val.data_ptr(), best.data_ptr(),
+                                               ok.data_ptr(), st), "k2_screen_device")
+        return val, best, ok
 
     # profiling ----------------------------------------------------------------------------
     def profile(self, enable=True):

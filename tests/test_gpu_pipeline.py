@@ -1,0 +1,97 @@
+"""Pre-screen, grid + Newton polish and the hierarchical grid cascade against the oracle and the
+reference's polish semantics."""
+import numpy as np
+import pytest
+
+from mertens_bound_amd import evaluator as ev
+from mertens_bound_amd import formats, pipeline, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import torch
+    tb = synth.make_zero_table(5_000, 21)
+    return tb, ev.K2Table(tb), torch
+
+
+def _shift_int(V, m, d):
+    return synth.v_from_int([v + int(round(m * d * 2.0 ** 64)) for v in synth.v_to_int(V)])
+
+
+def test_screen_values_vs_oracle(setup, oracle_lib):
+    tb, t, torch = setup
+    k, d = 3e-9, 0.0014
+    V = synth.make_candidates(512, 3.4e18, 3.6e18, 21)
+    stages = ((64, 0.0), (128, 0.0), (256, 0.0))           # thresholds 0: every stage runs
+    val, best, ok = t.screen_device(ev.v_tensor(V), k, d, 3, stages)
+    val = val.cpu().numpy()
+    assert np.all(ok.cpu().numpy() == 1)
+    for s, (nz, _) in enumerate(stages):
+        hs = np.stack([oracle_lib.eval_f64(tb, _shift_int(V, m, d), k, flags=1, n_zeros=nz)[0]
+                       for m in range(-3, 4)], axis=1)
+        want = np.max(np.abs(hs), axis=1)
+        scale = np.sum(2 * np.exp(-k * tb.gamma[:nz] ** 2) * np.abs(tb.w[:nz]))
+        err = np.max(np.abs(val[:, s] - want)) / scale
+        print(f"screen stage {nz}: max error {err:.2e} of sum|a|")
+        assert err <= 2e-6
+    np.testing.assert_array_equal(val[:, 3], -1.0)
+
+
+def test_screen_staging_and_survivors(setup):
+    tb, t, torch = setup
+    V = synth.make_candidates(4096, 3.4e18, 3.6e18, 22)
+    Vt = ev.v_tensor(V)
+    val, best, ok = t.screen_device(Vt, 3e-9, 0.0014, 3, ((64, 0.3), (128, 0.4), (256, 0.5)))
+    val = val.cpu().numpy(); ok = ok.cpu().numpy()
+    # a candidate passes iff every stage value reaches its threshold; failed stages stop the rest
+    thr = np.array([0.3, 0.4, 0.5])
+    for i in range(len(ok)):
+        reached = val[i, :3] >= 0
+        n = int(reached.sum())
+        assert np.all(val[i, :n - 1] >= thr[:n - 1])
+        assert ok[i] == int(n == 3 and val[i, 2] >= thr[2])
+        assert np.all(val[i, n:] == -1)
+    idx, Vs, _ = pipeline.screen(t, Vt, 3e-9, 0.0014, 3, ((64, 0.3), (128, 0.4), (256, 0.5)))
+    assert idx.numel() == int(ok.sum())
+    # survivors sit at their best shift: K over 256 zeros there equals the stage-3 value
+    K, _, _ = t.eval_device(Vs, 3e-9, n_zeros=256, flags=ev.VALUE)
+    np.testing.assert_allclose(np.abs(K.cpu().numpy()), val[idx.cpu().numpy(), 2], atol=2e-5)
+
+
+def test_grid_then_newton_matches_reference_polish(setup, oracle_lib):
+    """Canonical grid + Newton vs the reference's polish_peaks_with_weights on A/B rows of the
+    same table (reference-format words, dy0 = 0)."""
+    tb, t, torch = setup
+    k = 3e-9
+    V = synth.make_candidates(24, 3.4e18, 3.6e18, 23)
+    h0, hp, Vp, ddy = pipeline.grid_then_newton(t, ev.v_tensor(V), k)
+    rw = formats.reference_words(tb)
+    xs, dy = formats.candidates_to_x_dy(V)
+    xlo, xhi = formats.x_words(xs)
+    A, B = oracle_lib.ref_compute_peak_AB(xlo, xhi, dy, rw["alpha"], rw["gamma"], rw["ghi_lo"],
+                                          rw["ghi_hi"], rw["glo_lo"], rw["glo_hi"], rw["psi_lo"],
+                                          rw["psi_hi"])
+    w = np.exp(-k * tb.gamma ** 2)
+    hu_r, hp_r, dd_r = oracle_lib.ref_polish_peaks_with_weights(A, B, tb.gamma, w, 2.5e-4, 40)
+    np.testing.assert_allclose(h0.cpu().numpy(), hu_r, atol=1e-11)
+    np.testing.assert_allclose(hp.cpu().numpy(), hp_r, atol=1e-9)
+    np.testing.assert_allclose(ddy.cpu().numpy(), dd_r, atol=1e-9)
+
+
+def test_cascade_stages(setup, oracle_lib):
+    tb, t, torch = setup
+    k = 3e-9
+    V = synth.make_candidates(64, 3.4e18, 3.6e18, 24)
+    stages = ((256, 10, 1.5e-4, 0.2, False), (512, 20, 2.5e-5, 0.3, False),
+              (2048, 8, 1e-5, 0.4, True))
+    babs, bh, bd, passed = pipeline.cascade(t, ev.v_tensor(V), k, stages)
+    bd = bd.cpu().numpy(); passed = passed.cpu().numpy(); bh = bh.cpu().numpy()
+    for i in range(0, 64, 9):
+        s = passed[i]
+        last = min(s, len(stages) - 1)
+        nz = stages[last][0]
+        want = oracle_lib.eval_f64(tb, _shift_int(V[i:i + 1], 1, bd[i]), k, flags=1,
+                                   n_zeros=nz)[0][0]
+        assert abs(want - bh[i]) <= 1e-12
