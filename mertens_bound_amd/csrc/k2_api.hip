@@ -125,6 +125,10 @@ extern "C" int k2_table_destroy(k2_table* t) {
     (void)hipFree(t->tab64); (void)hipFree(t->tab32);
     (void)hipFree(t->partial); (void)hipFree(t->host_io); (void)hipFree(t->polish_active);
     (void)hipFree(t->screen_params);
+    if (t->aux) (void)hipStreamSynchronize(t->aux);
+    if (t->aux) (void)hipStreamDestroy(t->aux);
+    if (t->fork) (void)hipEventDestroy(t->fork);
+    if (t->join) (void)hipEventDestroy(t->join);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return K2_OK;
@@ -236,14 +240,17 @@ static Plan make_plan(const k2_table* t, int n_cand, int nz, bool mixed) {
     p.n_cand_pad = p.n_ctiles * 256 * nc;
     p.head_end = mixed ? std::min(nz, t->tiling.mixed_head) : nz;
     const int tb = t->tiling.target_blocks;
-    if (mixed) {
-        // share the workgroup budget in proportion to the zero counts
+    if (mixed && p.head_end < nz) {
+        // the FP64 head runs beside the FP32 tail on a second stream; give it workgroups in
+        // proportion to its work (an FP64 zero costs about 1.45 FP32 zeros), at least one split
         const int nt = nz - p.head_end;
-        const int tb_head = std::max(1, (int)((double)tb * p.head_end / std::max(1, nz)));
+        const double wh = 1.45 * p.head_end, wt = (double)nt;
+        const int tb_head = std::max(p.n_ctiles, (int)std::lround(tb * wh / (wh + wt)));
         split_range(p.head_end, p.n_ctiles, tb_head, &p.sp_head, &p.zps_head);
         split_range(nt, p.n_ctiles, std::max(1, tb - tb_head), &p.sp_tail, &p.zps_tail);
     } else {
         split_range(nz, p.n_ctiles, tb, &p.sp_head, &p.zps_head);
+        p.head_end = nz;
     }
     return p;
 }
@@ -276,17 +283,31 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
         pp.b = pool_event(t);
         HIPCHK(hipEventRecord(pp.a, st));
     }
-    if (p.sp_head > 0) {
-        a.z_begin = 0; a.z_end = p.head_end; a.zeros_per_split = p.zps_head;
-        a.split_base = 0;
+    if (p.sp_tail > 0 && p.sp_head > 0) {
+        // mixed form: FP64 head on the auxiliary stream, FP32 tail on st, joined before reducing
+        if (!t->aux) {
+            HIPCHK(hipStreamCreateWithFlags(&t->aux, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&t->fork, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&t->join, hipEventDisableTiming));
+        }
+        HIPCHK(hipEventRecord(t->fork, st));
+        HIPCHK(hipStreamWaitEvent(t->aux, t->fork, 0));
+        a.z_begin = 0; a.z_end = p.head_end; a.zps = p.zps_head; a.row_base = 0;
         a.n_blocks = p.n_ctiles * p.sp_head;
-        HIPCHK(launch_eval(a, false, st));
-    }
-    if (p.sp_tail > 0) {
-        a.z_begin = p.head_end; a.z_end = nz; a.zeros_per_split = p.zps_tail;
-        a.split_base = p.sp_head;
+        HIPCHK(launch_eval(a, false, t->aux));
+        a.z_begin = p.head_end; a.z_end = nz; a.zps = p.zps_tail; a.row_base = p.sp_head;
         a.n_blocks = p.n_ctiles * p.sp_tail;
         HIPCHK(launch_eval(a, true, st));
+        HIPCHK(hipEventRecord(t->join, t->aux));
+        HIPCHK(hipStreamWaitEvent(st, t->join, 0));
+    } else if (p.sp_tail > 0) {
+        a.z_begin = p.head_end; a.z_end = nz; a.zps = p.zps_tail; a.row_base = 0;
+        a.n_blocks = p.n_ctiles * p.sp_tail;
+        HIPCHK(launch_eval(a, true, st));
+    } else if (p.sp_head > 0) {
+        a.z_begin = 0; a.z_end = p.head_end; a.zps = p.zps_head; a.row_base = 0;
+        a.n_blocks = p.n_ctiles * p.sp_head;
+        HIPCHK(launch_eval(a, false, st));
     }
     if (t->profile) {
         HIPCHK(hipEventRecord(pp.b, st));

@@ -33,11 +33,11 @@ struct LaunchArgs {
     const float2* zaf = nullptr;
     const float2* zbf = nullptr;
     const float2* zef = nullptr;
-    int z_begin = 0, z_end = 0, zeros_per_split = 0;
+    int z_begin = 0, z_end = 0, zps = 0;   // zero range and zeros per split of this launch
     const uint4* cand = nullptr;
     int n_cand = 0, n_ctiles = 0;
     double* partial = nullptr;
-    int split_base = 0, n_cand_pad = 0;
+    int row_base = 0, n_cand_pad = 0;      // first partial row written by this launch
     const double2* tab64 = nullptr;   // (cos, sin)(2 pi k / K2_TAB)
     const float2* tab32 = nullptr;
 };
@@ -80,7 +80,9 @@ struct k2_table {
     int32_t* screen_params = nullptr;   // stage zero counts and thresholds of k2_screen_device
     size_t polish_cap = 0;
     hipStream_t stream = nullptr;
-    k2_tiling tiling{4, 2048, 8192};
+    hipStream_t aux = nullptr;             // second stream for the FP64 head of the mixed form
+    hipEvent_t fork = nullptr, join = nullptr;
+    k2_tiling tiling{2, 3072, 65536};
     bool profile = false;
     std::vector<ProfPair> prof_pending;
     std::vector<hipEvent_t> ev_pool;

@@ -54,17 +54,20 @@ __global__ __launch_bounds__(256) void k2_weights_kernel(
 }
 
 // ---- one-launch evaluation: C tiles over (split, candidate tile) workgroups -----------------------
+// Split s covers zeros [z_begin + s*zps, ...) up to z_end and writes partial row row_base + s.
+// F32 selects the precision form (exact top-32-bit phase, FP32 sin/cos and weights, compensated
+// FP32 sums); the mixed form launches an FP64 head and an FP32 tail side by side.
 template <int NC, int FL, bool F32>
 __global__ __launch_bounds__(256) void k2_eval_kernel(
-    ZeroView zv, int z_begin, int z_end, int zeros_per_split, const uint4* __restrict__ cand,
-    int n_cand, int n_ctiles, double* __restrict__ partial, int split_base, int n_cand_pad,
+    ZeroView zv, int z_begin, int z_end, int zps, const uint4* __restrict__ cand, int n_cand,
+    int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
     const double2* __restrict__ tab64, const float2* __restrict__ tab32) {
     __shared__ CTileSmem sm;
     __shared__ typename std::conditional<F32, float2, double2>::type s_tab[K2_TAB];
     const int L = xcd_linear_block();
     const int split = L / n_ctiles, ct = L - split * n_ctiles;
-    const int z0 = z_begin + split * zeros_per_split;
-    const int z1 = min(z_end, z0 + zeros_per_split);
+    const int z0 = z_begin + split * zps;
+    const int z1 = min(z_end, z0 + zps);
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB; i += K2_ZT) {
         if constexpr (F32) s_tab[i] = tab32[i];
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(256) void k2_eval_kernel(
     }
     if constexpr (F32) ctile_f32<NC, FL>(zv, z0, z1, v, acc, sm, s_tab);
     else ctile_f64<NC, FL>(zv, z0, z1, v, acc, sm, s_tab);
-    double* prow = partial + (size_t)(split_base + split) * 3 * n_cand_pad;
+    double* prow = partial + (size_t)(row_base + split) * 3 * n_cand_pad;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
@@ -204,8 +207,8 @@ template <int NC, int FL, bool F32>
 static hipError_t launch_eval_t(const LaunchArgs& a, hipStream_t st) {
     const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef};
     hipLaunchKernelGGL((k2_eval_kernel<NC, FL, F32>), dim3(a.n_blocks), dim3(K2_ZT), 0, st, zv,
-                       a.z_begin, a.z_end, a.zeros_per_split, a.cand, a.n_cand, a.n_ctiles,
-                       a.partial, a.split_base, a.n_cand_pad, a.tab64, a.tab32);
+                       a.z_begin, a.z_end, a.zps, a.cand, a.n_cand, a.n_ctiles, a.partial,
+                       a.row_base, a.n_cand_pad, a.tab64, a.tab32);
     return hipGetLastError();
 }
 

@@ -72,7 +72,7 @@ def test_tilings_agree(cfg1, nc, target):
     try:
         K, dK, _ = t.eval(V, k, flags=ev.VALUE | ev.D1)
     finally:
-        t.set_tiling(nc=4, target_blocks=2048)
+        t.set_tiling(nc=2, target_blocks=3072)
     assert np.max(np.abs(K - g["K"])) <= 1e-12
 
 
