@@ -23,26 +23,28 @@ using namespace k2impl;
 
 static int fail(int code, const std::string& msg);
 
-void k2impl::sincos_tables(int n, double2* t64, float2* t32) {
+void k2impl::sincos_tables(int n, k2dev::Tab64* t64, k2dev::Tab32* t32) {
     const long double two_pi = 6.283185307179586476925286766559005768L;
+    const long double c64 = two_pi / 18446744073709551616.0L;   // 2 pi / 2^64
+    const long double c32 = two_pi / 4294967296.0L;             // 2 pi / 2^32
     for (int k = 0; k < n; ++k) {
         const long double a = two_pi * (long double)k / (long double)n;
         const long double c = cosl(a), s = sinl(a);
-        t64[k] = make_double2((double)c, (double)s);
-        t32[k] = make_float2((float)c, (float)s);
+        t64[k] = k2dev::Tab64{(double)c, (double)s, (double)(c * c64), (double)(s * c64)};
+        t32[k] = k2dev::Tab32{(float)c, (float)s, (float)(c * c32), (float)(s * c32)};
     }
 }
 
-int k2impl::upload_tables(double2** t64, float2** t32, hipStream_t st) {
-    std::vector<double2> h64(K2_TAB_N);
-    std::vector<float2> h32(K2_TAB_N);
+int k2impl::upload_tables(k2dev::Tab64** t64, k2dev::Tab32** t32, hipStream_t st) {
+    std::vector<k2dev::Tab64> h64(K2_TAB_N);
+    std::vector<k2dev::Tab32> h32(K2_TAB_N);
     sincos_tables(K2_TAB_N, h64.data(), h32.data());
-    hipError_t e = hipMalloc((void**)t64, K2_TAB_N * sizeof(double2));
-    if (e == hipSuccess) e = hipMalloc((void**)t32, K2_TAB_N * sizeof(float2));
+    hipError_t e = hipMalloc((void**)t64, K2_TAB_N * sizeof(k2dev::Tab64));
+    if (e == hipSuccess) e = hipMalloc((void**)t32, K2_TAB_N * sizeof(k2dev::Tab32));
     if (e == hipSuccess)
-        e = hipMemcpyAsync(*t64, h64.data(), K2_TAB_N * sizeof(double2), hipMemcpyHostToDevice, st);
+        e = hipMemcpyAsync(*t64, h64.data(), K2_TAB_N * sizeof(k2dev::Tab64), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(*t32, h32.data(), K2_TAB_N * sizeof(float2), hipMemcpyHostToDevice, st);
+        e = hipMemcpyAsync(*t32, h32.data(), K2_TAB_N * sizeof(k2dev::Tab32), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e == hipSuccess ? K2_OK : fail(K2_E_HIP, std::string("sin/cos table: ") + hipGetErrorString(e));
 }
@@ -443,8 +445,8 @@ extern "C" int k2_sincos_words(uint32_t n, const uint64_t* W, int mode, double* 
     if (mode < 0 || mode > 1) return fail(K2_E_ARG, "k2_sincos_words: mode must be 0 or 1");
     uint64_t* dW = nullptr;
     double* dO = nullptr;
-    double2* t64 = nullptr;
-    float2* t32 = nullptr;
+    k2dev::Tab64* t64 = nullptr;
+    k2dev::Tab32* t32 = nullptr;
     int rc = upload_tables(&t64, &t32, nullptr);
     if (rc) return rc;
     hipError_t e = hipMalloc((void**)&dW, (size_t)n * 8);

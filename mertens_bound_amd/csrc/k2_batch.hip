@@ -55,11 +55,13 @@ template <int NC, int CZ, int FL, bool ALLOW32>
 __global__ __launch_bounds__(256) void k2_batch_kernel(
     BatchViews views, const BatchItem* __restrict__ items, int n_items, int* counter,
     const uint4* __restrict__ cand, double* __restrict__ partial,
-    const double2* __restrict__ tab64, const float2* __restrict__ tab32) {
-    __shared__ CTileSmem sm;
+    const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
+    __shared__ __align__(16) unsigned char sm_raw[ctile_smem_bytes<FL>()];
+    auto& sm64 = *reinterpret_cast<CTileSmem64<FL>*>(sm_raw);
+    auto& sm32 = *reinterpret_cast<CTileSmem32<FL>*>(sm_raw);
     __shared__ ZTileSmem<CZ> zsm;
-    __shared__ double2 s_tab64[K2_TAB];
-    __shared__ float2 s_tab32[ALLOW32 ? K2_TAB : 1];
+    __shared__ Tab64 s_tab64[K2_TAB];
+    __shared__ Tab32 s_tab32[ALLOW32 ? K2_TAB : 1];
     __shared__ int s_item;
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB; i += K2_WG) {
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
                 Acc<true> acc[NC];
 #pragma unroll
                 for (int i = 0; i < NC; ++i) acc[i].zero();
-                ctile_f32<NC, FL>(zv, I.z0, I.z1, v, acc, sm, s_tab32);
+                ctile_f32<NC, FL>(zv, I.z0, I.z1, v, acc, sm32, s_tab32);
 #pragma unroll
                 for (int i = 0; i < NC; ++i)
 #pragma unroll
@@ -113,7 +115,7 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
             Acc<false> acc[NC];
 #pragma unroll
             for (int i = 0; i < NC; ++i) acc[i].zero();
-            ctile_f64<NC, FL>(zv, I.z0, I.z1, v, acc, sm, s_tab64);
+            ctile_f64<NC, FL>(zv, I.z0, I.z1, v, acc, sm64, s_tab64);
 #pragma unroll
             for (int i = 0; i < NC; ++i)
 #pragma unroll
@@ -254,7 +256,7 @@ const void* batch_kernel(int nc, int cz, unsigned flags, bool allow32) {
 
 hipError_t launch_batch(const void* kernel, int grid, const BatchViews* views,
                         const BatchItem* items, int n_items, int* counter, const uint4* cand,
-                        double* partial, const double2* tab64, const float2* tab32,
+                        double* partial, const Tab64* tab64, const Tab32* tab32,
                         const CandDesc* desc, uint64_t n_cand, double* K, double* dK, double* d2K,
                         hipStream_t st) {
     hipLaunchKernelGGL(k2_counter_reset_kernel, dim3(1), dim3(64), 0, st, counter);

@@ -158,41 +158,50 @@ __device__ __forceinline__ void sincos_poly(double x, double& s, double& c) {
 
 // ---- table-driven sin/cos used by the evaluation kernels ------------------------------------
 // The angle 2 pi W / 2^64 is split into a table point 2 pi k / 512 (k = W rounded to 9 bits) and
-// a remainder |x| <= pi/512 = 6.2e-3.  cos/sin of the table points sit in LDS (K2_TAB entries);
-// the remainder needs only x - x^3/6 + x^5/120 (truncation < 1.1e-17 relative) and
-// 1 - x^2/2 + x^4/24 (truncation < 7.5e-17), and one complex rotation combines the two.
-// This replaces the quadrant reduction (no selects) and halves the polynomial work.
+// a remainder |x| <= pi/512 = 6.2e-3.  Table entries hold (cos, sin) of the table points and the
+// same scaled by c = 2 pi / 2^64 (K2_TAB entries, staged in LDS); the remainder enters unscaled as
+// u = x / c, so sin x = c u (1 - c^2 u^2/6 + c^4 u^4/120) (truncation < 1.1e-17 relative) and
+// cos x = 1 - c^2 u^2/2 + c^4 u^4/24 (< 7.5e-17) take their scale factors from the coefficients
+// and the table, and one complex rotation combines the two.  No quadrant selects, no range
+// reduction: the split comes straight from the integer word.
 #define K2_TAB 512
 #define K2_TAB_SHIFT 23   // 64-bit word: index bits 63..55 -> high word bits 31..23
 
-__device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const double2* tab,
+struct Tab64 {
+    double c, s;      // cos, sin of 2 pi k / K2_TAB
+    double cc, sc;    // the same times 2 pi / 2^64
+};
+
+struct Tab32 {
+    float c, s;       // cos, sin of 2 pi k / K2_TAB
+    float cc, sc;     // the same times 2 pi / 2^32
+};
+
+__device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const Tab64* tab,
                                              double& S, double& C) {
-    const uint32_t y = whi + (1u << (K2_TAB_SHIFT - 1));
-    const uint32_t k = y >> K2_TAB_SHIFT;                            // nearest table point
-    const int32_t rhi = (int32_t)(y & ((1u << K2_TAB_SHIFT) - 1)) - (1 << (K2_TAB_SHIFT - 1));
-    const double r = fma((double)rhi, 4294967296.0, (double)wlo);
-    const double x = r * K2_TWO_PI_2M64;
-    const double z = x * x;
-    const double sp = fma(x * z, fma(z, 8.3333333333333333e-03, -1.6666666666666666e-01), x);
-    const double cp = fma(z, fma(z, 4.1666666666666664e-02, -0.5), 1.0);
-    const double2 t = tab[k];                                        // (cos, sin)(2 pi k / 512)
-    C = fma(t.x, cp, -t.y * sp);
-    S = fma(t.y, cp, t.x * sp);
+    const uint32_t k = (whi + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT;   // nearest point
+    const int32_t rhi = ((int32_t)(whi << (32 - K2_TAB_SHIFT))) >> (32 - K2_TAB_SHIFT);
+    const double u = fma((double)rhi, 4294967296.0, (double)wlo);    // |u| <= 2^54
+    const double z = u * u;
+    const double sp = fma(u * z, fma(z, 1.121655105290912e-76, -1.9336107030552212e-38), u);
+    const double cp = fma(z, fma(z, 5.608275526454561e-76, -5.800832109165663e-38), 1.0);
+    const Tab64 t = tab[k];
+    C = fma(t.c, cp, -t.sc * sp);
+    S = fma(t.s, cp, t.cc * sp);
 }
 
-// FP32 form: W32 = top 32 bits of the phase word; |x| <= pi/512, x - x^3/6 and 1 - x^2/2 are
-// exact to FP32 rounding (next terms < 2e-11 relative)
-__device__ __forceinline__ void sincos_tab32(uint32_t W, const float2* tab, float& S, float& C) {
-    const uint32_t y = W + (1u << (K2_TAB_SHIFT - 1));
-    const uint32_t k = y >> K2_TAB_SHIFT;
-    const int32_t r = (int32_t)(y & ((1u << K2_TAB_SHIFT) - 1)) - (1 << (K2_TAB_SHIFT - 1));
-    const float x = (float)r * 1.4629181e-09f;                       // 2 pi / 2^32
-    const float z = x * x;
-    const float sp = fmaf(x * z, -1.6666667e-01f, x);
-    const float cp = fmaf(z, -0.5f, 1.0f);
-    const float2 t = tab[k];
-    C = fmaf(t.x, cp, -t.y * sp);
-    S = fmaf(t.y, cp, t.x * sp);
+// FP32 form: W = top 32 bits of the phase word, |r| <= 2^22 (exact in FP32); x - x^3/6 and
+// 1 - x^2/2 are exact to FP32 rounding at |x| <= pi/512 (next terms < 2e-11 relative)
+__device__ __forceinline__ void sincos_tab32(uint32_t W, const Tab32* tab, float& S, float& C) {
+    const uint32_t k = (W + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT;
+    const int32_t r = ((int32_t)(W << (32 - K2_TAB_SHIFT))) >> (32 - K2_TAB_SHIFT);
+    const float u = (float)r;
+    const float z = u * u;
+    const float sp = fmaf(u * z, -3.566882158968672e-19f, u);
+    const float cp = fmaf(z, -1.0700646218412075e-18f, 1.0f);
+    const Tab32 t = tab[k];
+    C = fmaf(t.c, cp, -t.sc * sp);
+    S = fmaf(t.s, cp, t.cc * sp);
 }
 
 // ---- general FP64 sincos (|x| < 2^20 pi/2) for the reference-format entry points -----------

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/k2eval.h"
+#include "k2_device.h"
 
 namespace k2impl {
 
@@ -38,14 +39,14 @@ struct LaunchArgs {
     int n_cand = 0, n_ctiles = 0;
     double* partial = nullptr;
     int row_base = 0, n_cand_pad = 0;      // first partial row written by this launch
-    const double2* tab64 = nullptr;   // (cos, sin)(2 pi k / K2_TAB)
-    const float2* tab32 = nullptr;
+    const k2dev::Tab64* tab64 = nullptr;   // sin/cos tables (k2_device.h)
+    const k2dev::Tab32* tab32 = nullptr;
 };
 
-// host-side (cos, sin)(2 pi k / n) tables, rounded from long double
-void sincos_tables(int n, double2* t64, float2* t32);
+// host-side sin/cos tables of 2 pi k / n (plain and scaled entries), rounded from long double
+void sincos_tables(int n, k2dev::Tab64* t64, k2dev::Tab32* t32);
 // device copies of those tables (K2_TAB entries)
-int upload_tables(double2** t64, float2** t32, hipStream_t st);
+int upload_tables(k2dev::Tab64** t64, k2dev::Tab32** t32, hipStream_t st);
 // records the calling thread's last error (k2_last_error) and returns code
 int set_error(int code, const std::string& msg);
 int alloc_weights(WeightArrays& w, size_t n_pad);
@@ -68,8 +69,8 @@ struct k2_table {
     uint4* g = nullptr;
     double* gamma = nullptr;
     double* w = nullptr;
-    double2* tab64 = nullptr;   // (cos, sin)(2 pi k / K2_TAB_N)
-    float2* tab32 = nullptr;
+    k2dev::Tab64* tab64 = nullptr;   // sin/cos tables of 2 pi k / K2_TAB
+    k2dev::Tab32* tab32 = nullptr;
     std::vector<WeightArrays> wsets;
     uint64_t use_clock = 0;
     double* partial = nullptr;
@@ -101,8 +102,8 @@ hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_
                          unsigned flags, double* K, double* dK, double* d2K, hipStream_t st);
 hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand, int n_cand,
                               int mode, uint64_t* out, hipStream_t st);
-hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const double2* tab64,
-                               const float2* tab32, double* out, hipStream_t st);
+hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const k2dev::Tab64* tab64,
+                               const k2dev::Tab32* tab32, double* out, hipStream_t st);
 hipError_t launch_newton(uint4* cand, const double* dK, const double* d2K, int n, double clamp,
                          int* active, int* steps, hipStream_t st);
 hipError_t launch_ref_peak_ab(int n_peaks, const uint64_t* const* p, const double* dy0,

@@ -57,13 +57,14 @@ __global__ __launch_bounds__(256) void k2_weights_kernel(
 // Split s covers zeros [z_begin + s*zps, ...) up to z_end and writes partial row row_base + s.
 // F32 selects the precision form (exact top-32-bit phase, FP32 sin/cos and weights, compensated
 // FP32 sums); the mixed form launches an FP64 head and an FP32 tail side by side.
+// NC <= 2 keeps to 128 VGPRs so that four workgroups (one wave per SIMD each) share a CU
 template <int NC, int FL, bool F32>
-__global__ __launch_bounds__(256) void k2_eval_kernel(
+__global__ __launch_bounds__(256, (NC <= 2 ? 4 : 2)) void k2_eval_kernel(
     ZeroView zv, int z_begin, int z_end, int zps, const uint4* __restrict__ cand, int n_cand,
     int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
-    const double2* __restrict__ tab64, const float2* __restrict__ tab32) {
-    __shared__ CTileSmem sm;
-    __shared__ typename std::conditional<F32, float2, double2>::type s_tab[K2_TAB];
+    const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
+    __shared__ typename std::conditional<F32, CTileSmem32<FL>, CTileSmem64<FL>>::type sm;
+    __shared__ typename std::conditional<F32, Tab32, Tab64>::type s_tab[K2_TAB];
     const int L = xcd_linear_block();
     const int split = L / n_ctiles, ct = L - split * n_ctiles;
     const int z0 = z_begin + split * zps;
@@ -134,8 +135,8 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
 // mode 0: FP64 table path of the plain kernel, 1: FP32 table path of the mixed kernel (top 32 bits)
 __global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __restrict__ W,
                                                               int n, int mode,
-                                                              const double2* __restrict__ tab64,
-                                                              const float2* __restrict__ tab32,
+                                                              const Tab64* __restrict__ tab64,
+                                                              const Tab32* __restrict__ tab32,
                                                               double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -263,8 +264,8 @@ hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand
     return hipGetLastError();
 }
 
-hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const double2* tab64,
-                               const float2* tab32, double* out, hipStream_t st) {
+hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const Tab64* tab64,
+                               const Tab32* tab32, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k2_sincos_words_kernel, dim3((n + 255) / 256), dim3(256), 0, st, W, n,
                        mode, tab64, tab32, out);
     return hipGetLastError();

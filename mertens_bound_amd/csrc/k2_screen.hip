@@ -38,10 +38,10 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
     const uint4* __restrict__ zg, const double* __restrict__ gamma, const double2* __restrict__ w,
     double k, double d, int n_zeros, const uint4* __restrict__ cand, int n_cand, int n_stages,
     const int* __restrict__ stage_nz, const float* __restrict__ stage_thr,
-    const float2* __restrict__ tab, float* __restrict__ out_val, int* __restrict__ out_shift,
+    const Tab32* __restrict__ tab, float* __restrict__ out_val, int* __restrict__ out_shift,
     int* __restrict__ out_pass) {
     __shared__ ScreenZero sz[K2_SCREEN_MAX];
-    __shared__ float2 s_tab[K2_TAB];
+    __shared__ Tab32 s_tab[K2_TAB];
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB; i += blockDim.x) s_tab[i] = tab[i];
     for (int j = tid; j < n_zeros; j += blockDim.x) {
@@ -109,7 +109,7 @@ namespace k2impl {
 template <int M>
 static hipError_t launch_screen_m(const uint4* zg, const double* gamma, const double2* w, double k,
                                   double d, int n_zeros, const uint4* cand, int n_cand,
-                                  int n_stages, const int* nz, const float* thr, const float2* tab,
+                                  int n_stages, const int* nz, const float* thr, const Tab32* tab,
                                   float* val, int* shift, int* pass, hipStream_t st) {
     hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st, zg,
                        gamma, w, k, d, n_zeros, cand, n_cand, n_stages, nz, thr, tab, val, shift,
@@ -119,7 +119,7 @@ static hipError_t launch_screen_m(const uint4* zg, const double* gamma, const do
 
 hipError_t launch_screen(int M, const uint4* zg, const double* gamma, const double2* w, double k,
                          double d, int n_zeros, const uint4* cand, int n_cand, int n_stages,
-                         const int* nz, const float* thr, const float2* tab, float* val,
+                         const int* nz, const float* thr, const Tab32* tab, float* val,
                          int* shift, int* pass, hipStream_t st) {
     switch (M) {
 #define K2_M(X) case X: return launch_screen_m<X>(zg, gamma, w, k, d, n_zeros, cand, n_cand, \

@@ -75,20 +75,40 @@ __device__ __forceinline__ void accumulate(Acc<true>& A, float S, float C, float
 }
 
 // ---- C tile ---------------------------------------------------------------------------------
-// LDS: two buffers of K2_WG zero records (FP64: g, a, b, e = 64 B; FP32 reuses the space)
-struct CTileSmem {
+// LDS: two buffers of K2_WG zero records holding only what the requested sums read
+// (FP64: g 16 B, a 16 B, b 16 B with K2', e 16 B with K2''; FP32: g, a|b 16 B, e 8 B).
+template <int FL>
+struct CTileSmem64 {
+    static constexpr int NB = (FL & K2_D1) ? 1 : 0, NE = (FL & K2_D2) ? 1 : 0;
     uint4 g[2][K2_WG];
-    double2 a[2][K2_WG];
-    double2 b[2][K2_WG];
-    double2 e[2][K2_WG];
+    double2 w[2][(1 + NB + NE) * K2_WG];   // a, then b, then e
+    __device__ __forceinline__ double2& a(int buf, int z) { return w[buf][z]; }
+    __device__ __forceinline__ double2& b(int buf, int z) { return w[buf][K2_WG + z]; }
+    __device__ __forceinline__ double2& e(int buf, int z) { return w[buf][(1 + NB) * K2_WG + z]; }
 };
+
+template <int FL>
+struct CTileSmem32 {
+    static constexpr int NE = (FL & K2_D2) ? 1 : 0;
+    uint4 g[2][K2_WG];
+    float4 ab[2][K2_WG];
+    float2 ew[2][NE ? K2_WG : 2];   // 2 entries (16 B) keep the struct 16-byte sized
+    __device__ __forceinline__ float2& e(int buf, int z) { return ew[buf][z]; }
+};
+
+// raw LDS large enough for either record layout (the batched kernel runs both)
+template <int FL>
+constexpr size_t ctile_smem_bytes() {
+    return sizeof(CTileSmem64<FL>) > sizeof(CTileSmem32<FL>) ? sizeof(CTileSmem64<FL>)
+                                                               : sizeof(CTileSmem32<FL>);
+}
 
 // Sums zeros [z0, z1) into acc for the NC candidates of this lane.  Enters and leaves with sm
 // free (the last action is a barrier).  tab: the (cos, sin) table in LDS of the matching width.
 template <int NC, int FL>
 __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, const Limbs4 (&v)[NC],
-                                          Acc<false> (&acc)[NC], CTileSmem& sm,
-                                          const double2* tab) {
+                                          Acc<false> (&acc)[NC], CTileSmem64<FL>& sm,
+                                          const Tab64* tab) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
     const int tid = threadIdx.x;
@@ -106,9 +126,9 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
         }
     };
     auto stage = [&](int buf) {
-        sm.g[buf][tid] = rg; sm.a[buf][tid] = ra;
-        if (D1) sm.b[buf][tid] = rb;
-        if (D2) sm.e[buf][tid] = re;
+        sm.g[buf][tid] = rg; sm.a(buf, tid) = ra;
+        if (D1) sm.b(buf, tid) = rb;
+        if (D2) sm.e(buf, tid) = re;
     };
     if (ntiles > 0) { fetch(0); stage(0); }
     __syncthreads();
@@ -118,10 +138,10 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
 #pragma unroll 2
         for (int z = 0; z < K2_WG; ++z) {
             const uint4 gg = sm.g[buf][z];
-            const double2 aa = sm.a[buf][z];
+            const double2 aa = sm.a(buf, z);
             double2 bb = make_double2(0.0, 0.0), ee = bb;
-            if (D1) bb = sm.b[buf][z];
-            if (D2) ee = sm.e[buf][z];
+            if (D1) bb = sm.b(buf, z);
+            if (D2) ee = sm.e(buf, z);
             const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
@@ -138,12 +158,10 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
 
 template <int NC, int FL>
 __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, const Limbs4 (&v)[NC],
-                                          Acc<true> (&acc)[NC], CTileSmem& sm,
-                                          const float2* tab) {
+                                          Acc<true> (&acc)[NC], CTileSmem32<FL>& sm,
+                                          const Tab32* tab) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
-    float4 (*s_ab)[K2_WG] = reinterpret_cast<float4 (*)[K2_WG]>(sm.a);
-    float2 (*s_e)[K2_WG] = reinterpret_cast<float2 (*)[K2_WG]>(sm.e);
     const int tid = threadIdx.x;
     const int ntiles = z1 > z0 ? (z1 - z0 + K2_WG - 1) / K2_WG : 0;
     uint4 rg = make_uint4(0, 0, 0, 0);
@@ -164,8 +182,8 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
         }
     };
     auto stage = [&](int buf) {
-        sm.g[buf][tid] = rg; s_ab[buf][tid] = rab;
-        if (D2) s_e[buf][tid] = re;
+        sm.g[buf][tid] = rg; sm.ab[buf][tid] = rab;
+        if (D2) sm.e(buf, tid) = re;
     };
     if (ntiles > 0) { fetch(0); stage(0); }
     __syncthreads();
@@ -175,9 +193,9 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 #pragma unroll 2
         for (int z = 0; z < K2_WG; ++z) {
             const uint4 gg = sm.g[buf][z];
-            const float4 ab = s_ab[buf][z];
+            const float4 ab = sm.ab[buf][z];
             float2 ee = make_float2(0.f, 0.f);
-            if (D2) ee = s_e[buf][z];
+            if (D2) ee = sm.e(buf, z);
             const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
@@ -212,7 +230,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 template <int CZ, int FL, bool F32>
 __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const uint4* cand,
                                       int c_first, int c_count, double* out, ZTileSmem<CZ>& sm,
-                                      const double2* tab64, const float2* tab32) {
+                                      const Tab64* tab64, const Tab32* tab32) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
     const int tid = threadIdx.x;
