@@ -131,6 +131,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=TUNED)
     ap.add_argument("--quick", action="store_true", help="smaller grids")
+    ap.add_argument("--only", choices=("eval", "batch"), default=None,
+                    help="tune one kind; the other kind's entries are kept from the current file")
     args = ap.parse_args()
     import numpy as np
     import torch
@@ -151,9 +153,14 @@ def main():
     t_big = ev.K2Table(big)
     small = synth.make_zero_table(10_000, 0)
     t_small = ev.K2Table(small)
-    out = {"eval": {}, "batch": {}, "device": torch.cuda.get_device_name(0),
+    prev = load()
+    out = {"eval": dict(prev.get("eval", {})) if args.only == "batch" else {},
+           "batch": dict(prev.get("batch", {})) if args.only == "eval" else {},
+           "device": torch.cuda.get_device_name(0),
            "created": time.strftime("%Y-%m-%d %H:%M:%S")}
     shapes = [(t_big, big.n, 65_536), (t_big, big.n, 4_096), (t_small, small.n, 1_024)]
+    if args.only == "batch":
+        shapes = []
     for form in ("plain", "mixed"):
         for t, nz, nc in shapes:
             V = ev.v_tensor(synth.make_candidates(nc, 3.4e18, 3.6e18, 1))
@@ -164,7 +171,7 @@ def main():
     nzr, ks, cnt, offs, Vb = synth.make_batched_requests(big, n_req=512, seed=0)
     reqs = [(int(a), int(b), int(o), float(k)) for a, b, o, k in zip(nzr, cnt, offs, ks)]
     Vbt = ev.v_tensor(Vb)
-    for form in ("plain", "mixed"):
+    for form in (() if args.only == "eval" else ("plain", "mixed")):
         res = tune_batch(t_big, reqs, Vbt, form, bg, log)
         out["batch"][form] = {"tiling": res[0]["tiling"], "ms": res[0]["ms"], "all": res}
         log(f"best batch {form}: {res[0]}")

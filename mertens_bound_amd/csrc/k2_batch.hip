@@ -56,13 +56,17 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
     BatchViews views, const BatchItem* __restrict__ items, int n_items, int* counter,
     const uint4* __restrict__ cand, double* __restrict__ partial,
     const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
-    __shared__ __align__(16) unsigned char sm_raw[ctile_smem_bytes<FL>()];
+    // one raw LDS block serves the C-tile records, the Z-tile scratch and the item index: an item
+    // is either kind, and the index is read into a register before the item's work begins
+    constexpr size_t RAW = ctile_smem_bytes<FL>() > sizeof(ZTileSmem<CZ>)
+                               ? ctile_smem_bytes<FL>() : sizeof(ZTileSmem<CZ>);
+    __shared__ __align__(16) unsigned char sm_raw[RAW];
     auto& sm64 = *reinterpret_cast<CTileSmem64<FL>*>(sm_raw);
     auto& sm32 = *reinterpret_cast<CTileSmem32<FL>*>(sm_raw);
-    __shared__ ZTileSmem<CZ> zsm;
+    auto& zsm = *reinterpret_cast<ZTileSmem<CZ>*>(sm_raw);
+    int& s_item = *reinterpret_cast<int*>(sm_raw);
     __shared__ Tab64 s_tab64[K2_TAB];
     __shared__ Tab32 s_tab32[ALLOW32 ? K2_TAB : 1];
-    __shared__ int s_item;
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB; i += K2_WG) {
         s_tab64[i] = tab64[i];
@@ -71,6 +75,7 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
     __syncthreads();
     constexpr int CW = K2_WG * NC;
     for (;;) {
+        __syncthreads();   // every wave is done with the previous item's LDS before s_item lands
         if (tid == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
         const int it = s_item;

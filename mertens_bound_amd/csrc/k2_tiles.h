@@ -212,6 +212,8 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 }
 
 // ---- Z tile ---------------------------------------------------------------------------------
+#define K2_ZG 4   // candidates a Z-tile lane evaluates per pass over its zeros
+
 template <int CZ>
 struct ZTileSmem {
     uint4 v[CZ];
@@ -225,57 +227,64 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 // Sums zeros [z0, z1) for candidates cand[c_first .. c_first + c_count) (c_count <= CZ); lanes
-// stride over the zeros.  Writes out[q * CZ + c] (q = value, K', K'') after a fixed-order
-// reduction: butterfly within each wave, then the four wave sums in order.
+// stride over the zeros, K2_ZG candidates at a time (the lane's zero records are re-read from
+// cache for each group, which bounds the register footprint).  Writes
+// out[q * CZ + c] (q = value, K', K'') after a fixed-order reduction: butterfly within each
+// wave, then the four wave sums in order.
 template <int CZ, int FL, bool F32>
 __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const uint4* cand,
                                       int c_first, int c_count, double* out, ZTileSmem<CZ>& sm,
                                       const Tab64* tab64, const Tab32* tab32) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
+    constexpr int NQ = D2 ? 3 : (D1 ? 2 : 1);
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
     if (tid < CZ) sm.v[tid] = tid < c_count ? cand[c_first + tid] : make_uint4(0, 0, 0, 0);
     __syncthreads();
-    Acc<F32> acc[CZ];
+    for (int cp = 0; cp < CZ; cp += K2_ZG) {
+        Acc<F32> acc[K2_ZG];
+        Limbs4 v[K2_ZG];
 #pragma unroll
-    for (int c = 0; c < CZ; ++c) acc[c].zero();
-    for (int j = z0 + tid; j < z1; j += K2_WG) {
-        const uint4 gg = zv.g[j];
-        const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
-        if constexpr (F32) {
-            const float2 a = zv.af[j];
-            const float2 b = D1 ? zv.bf[j] : make_float2(0.f, 0.f);
-            const float2 e = D2 ? zv.ef[j] : make_float2(0.f, 0.f);
+        for (int c = 0; c < K2_ZG; ++c) {
+            acc[c].zero();
+            const uint4 cv = sm.v[cp + c];
+            v[c] = Limbs4{cv.x, cv.y, cv.z, cv.w};
+        }
+        for (int j = z0 + tid; j < z1; j += K2_WG) {
+            const uint4 gg = zv.g[j];
+            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+            if constexpr (F32) {
+                const float2 a = zv.af[j];
+                const float2 b = D1 ? zv.bf[j] : make_float2(0.f, 0.f);
+                const float2 e = D2 ? zv.ef[j] : make_float2(0.f, 0.f);
 #pragma unroll
-            for (int c = 0; c < CZ; ++c) {
-                const uint4 cv = sm.v[c];
-                const uint32_t W = phase_word_hi32(g, Limbs4{cv.x, cv.y, cv.z, cv.w});
-                float S, C;
-                sincos_tab32(W, tab32, S, C);
-                accumulate<FL>(acc[c], S, C, a, b, e);
-            }
-        } else {
-            const double2 a = zv.a[j];
-            const double2 b = D1 ? zv.b[j] : make_double2(0.0, 0.0);
-            const double2 e = D2 ? zv.e[j] : make_double2(0.0, 0.0);
+                for (int c = 0; c < K2_ZG; ++c) {
+                    const uint32_t W = phase_word_hi32(g, v[c]);
+                    float S, C;
+                    sincos_tab32(W, tab32, S, C);
+                    accumulate<FL>(acc[c], S, C, a, b, e);
+                }
+            } else {
+                const double2 a = zv.a[j];
+                const double2 b = D1 ? zv.b[j] : make_double2(0.0, 0.0);
+                const double2 e = D2 ? zv.e[j] : make_double2(0.0, 0.0);
 #pragma unroll
-            for (int c = 0; c < CZ; ++c) {
-                const uint4 cv = sm.v[c];
-                const uint64_t W = phase_word(g, Limbs4{cv.x, cv.y, cv.z, cv.w});
-                double S, C;
-                sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab64, S, C);
-                accumulate<FL>(acc[c], S, C, a, b, e);
+                for (int c = 0; c < K2_ZG; ++c) {
+                    const uint64_t W = phase_word(g, v[c]);
+                    double S, C;
+                    sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab64, S, C);
+                    accumulate<FL>(acc[c], S, C, a, b, e);
+                }
             }
         }
-    }
-    const int lane = tid & 63, wid = tid >> 6;
-    constexpr int NQ = D2 ? 3 : (D1 ? 2 : 1);
 #pragma unroll
-    for (int c = 0; c < CZ; ++c) {
+        for (int c = 0; c < K2_ZG; ++c) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const double x = wave_sum(acc[c].get(q));
-            if (lane == 0) sm.red[wid][q * CZ + c] = x;
+            for (int q = 0; q < NQ; ++q) {
+                const double x = wave_sum(acc[c].get(q));
+                if (lane == 0) sm.red[wid][q * CZ + cp + c] = x;
+            }
         }
     }
     __syncthreads();
