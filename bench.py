@@ -9,7 +9,7 @@ counts only with |dK2| <= 1e-9 against the multiprecision oracle, checked on the
 
 Other workloads (--workload): mixed (FP32 form of the same), batched (512 uneven requests in one
 persistent launch, graph replay), polish (4,096 candidates, 8 Newton steps), small (10,000 zeros x
-1,024 candidates), copy (a 2 GiB device copy, the byte-counter calibration for rocprofv3 --pmc).
+1,024 candidates), enclose (the default workload with rigorous radii), copy (a 2 GiB device copy, the byte-counter calibration for rocprofv3 --pmc).
 
 Prints one JSON line.
 """
@@ -28,8 +28,10 @@ sys.path.insert(0, ROOT)
 #          (42 flops: sin/cos of the reduced phase, weighting, two accumulations)
 #   mixed: 15 INT32 multiply-adds + 2 transcendental-equivalents (counted as 2 slots each,
 #          half rate) + 4 FP32 FMA-equivalents (8 flops)
+#   enclose: the plain work + 2 x 6 FP64 additions (TwoSum error-free accumulation of K2, K2')
 WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
-        "mixed": {"int32_mad": 15, "trans": 2, "fp32_fma": 4, "slots": 23}}
+        "mixed": {"int32_mad": 15, "trans": 2, "fp32_fma": 4, "slots": 23},
+        "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48}}
 # Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
 # published 78.6 TFLOP/s FP64 vector peak counted per FMA (2 flops).
 VALU_PEAK_TOPS = 39.3
@@ -132,8 +134,8 @@ def roofline(form, evals, kern_ms, alg_bytes, workload):
          "int32_mad_tops": rate * w["int32_mad"] / 1e12,
          "algorithmic_bytes": alg_bytes,
          "hbm_GBps_algorithmic": alg_bytes / (kern_ms * 1e-3) / 1e9}
-    if form == "plain":
-        r["fp64_tflops"] = rate * 2 * w["fp64_fma"] / 1e12
+    if form in ("plain", "enclose"):
+        r["fp64_tflops"] = rate * (2 * w["fp64_fma"] + w.get("fp64_add", 0)) / 1e12
         r["fp64_frac"] = r["fp64_tflops"] / FP64_PEAK_TFLOPS
     return r
 
@@ -301,6 +303,52 @@ def run_polish(args):
     return res
 
 
+def run_enclose(args):
+    """Enclosure mode (midpoint + rigorous radius of K2 and K2') on the metric workload."""
+    import numpy as np
+    import torch
+    from mertens_bound_amd import autotune
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import synth
+    c = synth.CONFIG2
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    n_cand = args.n_cand or c["n_cand"]
+    V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
+    torch.cuda.set_device(0)
+    t = ev.K2Table(tb)
+    autotune.apply_tuned(t, n_cand, tb.n, "plain")
+    Vd = ev.v_tensor(V)
+    acc = None
+    gpath = os.path.join(ROOT, "tests", "golden", "config2_sub256.npz")
+    if os.path.exists(gpath) and not args.no_check:
+        g = np.load(gpath, allow_pickle=False)
+        K, Kr, dK, dKr, _ = t.enclose(g["V"], c["k"])
+        inside = np.abs(K - g["K"]) <= Kr + 2.0 ** -53 * np.abs(g["K"])
+        acc = {"max_abs_dK2_vs_oracle": float(np.max(np.abs(K - g["K"]))),
+               "max_radius": float(Kr.max()), "max_dK_radius": float(dKr.max()),
+               "oracle_inside_all": bool(inside.all()), "oracle_sample": "config2_sub256.npz"}
+    state = {}
+
+    def step():
+        state["r"] = t.enclose_device(Vd, c["k"])
+
+    ms, wall = timed(step, args.steps, args.warmup, torch)
+    K, Kr = state["r"][0], state["r"][1]
+    evals = float(n_cand) * tb.n
+    alg_bytes = tb.n * (16 + 32) + n_cand * (16 + 32)
+    res = base_record(
+        args, "candidate-zero evals/sec (K2 + dK2/dv enclosure, 2.33M zeros)",
+        evals / (ms * 1e-3), "candidate-zero evals/s", ms, "f64",
+        {"workload": f"K2+dK2/dv with radii, {n_cand:,} candidates x {tb.n:,} zeros, "
+                     f"k={c['k']}", "form": "enclose", "n_cand": n_cand, "n_zeros": tb.n,
+         "tiling": t.get_tiling()})
+    res["accuracy"] = acc
+    res["radius_max_all"] = float(Kr.max().item())
+    res["roofline"] = roofline("enclose", evals, ms, alg_bytes, "enclose")
+    res["wall_s"] = wall
+    return res
+
+
 def run_copy(args):
     import torch
     n = 1 << 28                                  # 2 GiB of float64
@@ -317,7 +365,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["metric", "mixed", "batched", "polish", "small",
-                                           "copy"], default="metric")
+                                           "enclose", "copy"], default="metric")
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
     ap.add_argument("--n-cand", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -337,6 +385,8 @@ def main():
         res = run_batched(args)
     elif args.workload == "polish":
         res = run_polish(args)
+    elif args.workload == "enclose":
+        res = run_enclose(args)
     else:
         res = run_copy(args)
     print(json.dumps(res))

@@ -136,6 +136,20 @@ int k2_polish_device(k2_table *t, uint32_t n_cand, uint64_t *V_dev, double k, ui
                      int n_steps, double clamp, uint32_t flags, double *K_dev, double *dK_dev,
                      double *d2K_dev, int32_t *steps_dev, void *stream);
 
+/* Enclosure of K2 (and K2' with K2_D1) over the first n_zeros zeros (device buffers): the exact
+ * sum over the table's definitions (gamma_j = 2 pi G_j / 2^110, w_j as stored, k and v as given)
+ * lies in [K - K_rad, K + K_rad].  FP64 with error-free (TwoSum) accumulation and per-zero error
+ * bounds; tail_dev (2 doubles, optional) receives bounds on sum_{j >= n_zeros} 2 e^{-k g^2} |w_j|
+ * and the same with the factor g_j, i.e. on what the rest of the table can add to K2 and K2'.
+ * The ball-sum semantics of certification/certify_K2.py:112-121 (S, absS; combined at :186-192),
+ * evaluated on the GPU.  flags: K2_VALUE or K2_VALUE | K2_D1. */
+int k2_enclose_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double k,
+                      uint32_t n_zeros, uint32_t flags, double *K_dev, double *K_rad_dev,
+                      double *dK_dev, double *dK_rad_dev, double *tail_dev, void *stream);
+/* debug: the enclosure mode's e^{-x} (same operations as on the device), for checking its
+ * error bound on the host */
+int k2_exp_neg_host(uint32_t n, const double *x, double *out);
+
 /* Staged pre-screen on the first zeros (device buffers): for shifts m = -n_shift..n_shift the
  * sums h_m = K2 over the first n zeros at v + m*shift_step, stage value max_m |h_m|; stage s
  * (stage_zeros[s] zeros, increasing, <= 1024) continues only if its value reaches

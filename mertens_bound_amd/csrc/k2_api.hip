@@ -127,6 +127,7 @@ extern "C" int k2_table_destroy(k2_table* t) {
     (void)hipFree(t->tab64); (void)hipFree(t->tab32);
     (void)hipFree(t->partial); (void)hipFree(t->host_io); (void)hipFree(t->polish_active);
     (void)hipFree(t->screen_params);
+    free_enclose(t);
     if (t->aux) (void)hipStreamSynchronize(t->aux);
     if (t->aux) (void)hipStreamDestroy(t->aux);
     if (t->fork) (void)hipEventDestroy(t->fork);
@@ -224,7 +225,7 @@ struct Plan {
     int head_end = 0;
 };
 
-static void split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps) {
+void k2impl::split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps) {
     if (nz <= 0) { *n_splits = 0; *zps = 0; return; }
     const int max_sp = (nz + 255) / 256;
     int sp = (target + n_ctiles - 1) / n_ctiles;

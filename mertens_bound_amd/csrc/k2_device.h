@@ -226,4 +226,37 @@ __device__ __forceinline__ void sincos_general(double x, double& s, double& c) {
     s = neg ? -sn : sn;
 }
 
+// ---- e^{-x} with a stated error bound (enclosure mode) ----------------------------------------
+// For 0 <= x <= 708 the result is within 2.5 * 2^-53 relative of e^{-x}; larger x return 0
+// (e^{-x} < 3.1e-308 there, which the caller bounds separately).  Reduction x = n ln2 + r with
+// ln2 = L1 + L2, L1 carrying 32 significant bits, so n L1 is exact for n < 2^21 and x - n L1 is
+// exact (Cody and Waite 1980); |r| <= ln2/2 + 2^-40.  e^{-r} = sum_k (-r)^k / k! to degree 13
+// (truncation < 4.3e-18 relative), by Horner in FP64 (< 2 * 2^-53); r's rounding adds < 0.2 *
+// 2^-53; scaling by 2^-n is exact above the subnormal range.  Host and device run the same
+// operations (explicit fma), so host builds reproduce device results bit for bit.
+__host__ __device__ inline double exp_neg_bounded(double x) {
+    if (!(x <= 708.0)) return 0.0;
+    const double L1 = 6.9314718036912382e-01;   // ln 2 to 32 significant bits
+    const double L2 = 1.9082149292705877e-10;   // ln 2 - L1
+    const double n = rint(x * 1.4426950408889634);
+    const double r1 = fma(-n, L1, x);
+    const double r = fma(-n, L2, r1);
+    const double mr = -r;
+    double p = 1.6059043836821613e-10;           // 1/13!
+    p = fma(p, mr, 2.08767569878681e-09);         // 1/12!
+    p = fma(p, mr, 2.505210838544172e-08);        // 1/11!
+    p = fma(p, mr, 2.755731922398589e-07);        // 1/10!
+    p = fma(p, mr, 2.7557319223985893e-06);       // 1/9!
+    p = fma(p, mr, 2.48015873015873e-05);         // 1/8!
+    p = fma(p, mr, 1.984126984126984e-04);        // 1/7!
+    p = fma(p, mr, 1.388888888888889e-03);        // 1/6!
+    p = fma(p, mr, 8.333333333333333e-03);        // 1/5!
+    p = fma(p, mr, 4.1666666666666664e-02);       // 1/4!
+    p = fma(p, mr, 1.6666666666666666e-01);       // 1/3!
+    p = fma(p, mr, 0.5);
+    p = fma(p, mr, 1.0);
+    p = fma(p, mr, 1.0);
+    return ldexp(p, -(int)n);
+}
+
 }  // namespace k2dev

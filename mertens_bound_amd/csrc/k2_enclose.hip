@@ -1,0 +1,292 @@
+// Enclosure mode: K2 (and K2') over the first N zeros with a rigorous error radius, in the style
+// of the ball sums of certification/certify_K2.py:112-121 (term 2 e^{-k g^2} Re(w e^{i g v}),
+// running sum S and magnitude sum absS = sum 2 e^{-k g^2} |w| bounding the zeros left out,
+// :163-165,186-192).  The exact quantity is the sum over the table's definitions: g_j =
+// 2 pi G_j / 2^110 exactly, w_j the stored complex128 values, k and v as given (v exact in
+// 64.64).  The result is a midpoint K and radius R with |K - K2_exact| <= R.
+//
+// Error budget (u = 2^-53; x_j = k g_j^2; per zero, independent of the candidate):
+//   g_j     from G_j in FP64 (two conversions, one sum, one product)         rel <= 2.5 u
+//   x_j     k g g                                                             rel <= 6 u
+//   e^{-x}  exp_neg_bounded                                                   rel <= 2.5 u
+//   2 e^{-x} w  -> a_j (one product per component)                            |a - A| <= |A| (3 + 6.1 x) u
+//   phase   W = floor(G V / 2^110) mod 2^64 exact; dropped fraction < 2 pi 2^-64 rad
+//   sin/cos sincos_tab64: table rounding 0.5 u, remainder polynomial 1.2 u, combine 0.55 u
+//           -> |S - sin|, |C - cos| <= 2.3 u (bounded here by 3 u)
+//   term    fma(a.x, C, -a.y S): sqrt(2) |a| 3u + 0.5 |a| u + 0.5 |a| u = 5.25 |a| u
+//   =>      |term_j - T_j| <= |a_j| (9 + 6.5 x_j) u                  (K2)
+//           |term'_j - T'_j| <= g_j |a_j| (12 + 6.5 x_j) u           (K2', b = g a adds 3.5 u)
+//   sums    TwoSum (Knuth) per lane and across splits: every addition's rounding error is kept
+//           exactly in a second word; only the second word's own sum rounds, which adds at most
+//           (m u)^2 sum|a_j| (m = zeros + 2 splits + 2), and the final hi + lo rounds by u |K|.
+//   Zeros with x_j > 708 get weight 0 and contribute 2 |w_j| 1e-307 to the bound; 1e-300 covers
+//   every absolute (subnormal) rounding.  The bound sums are formed in FP64 and the whole radius
+//   is scaled by 1.01, which covers their own rounding (< 1e-9 relative) and |A| vs |a|.
+// The radius is R = 1.01 (u B + (m u)^2 sum|a| + u |K|) + 1e-300 with B = sum_{j<N} |a_j|
+// (9 + 6.5 x_j); the zeros of the table past N contribute at most tail = 1.01 sum_{j>=N} |a_j|.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <string>
+
+#include "k2_device.h"
+#include "k2_internal.h"
+#include "k2_tiles.h"
+
+using namespace k2dev;
+
+#define K2_ENC_Q 6   // bound sums: B, B', sum|a|, sum g|a| over j < N; sum|a|, sum g|a| over j >= N
+#define K2_ENC_U 1.1102230246251565e-16   // 2^-53
+#define K2_ENC_SAFE 1.01
+#define K2_ENC_ABS 1e-300
+
+// fixed-order sum over the 256 threads of a workgroup; result valid in thread 0
+__device__ __forceinline__ double enc_block_sum(double x, double* red) {
+    x = wave_sum(x);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0) red[tid >> 6] = x;
+    __syncthreads();
+    const double s = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return s;
+}
+
+// a_j = 2 e^{-k g^2} w_j and b_j = g_j a_j from the exact words, and per-block bound sums
+__global__ __launch_bounds__(256) void k2_enclose_weights_kernel(
+    const uint4* __restrict__ zg, const double2* __restrict__ w, int n, int n_pad, int N, double k,
+    double2* __restrict__ a, double2* __restrict__ b, double* __restrict__ part) {
+    __shared__ double red[4];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double q[K2_ENC_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double2 va = make_double2(0.0, 0.0), vb = va;
+    if (j < n) {
+        const uint4 gg = zg[j];
+        const uint64_t ghi = ((uint64_t)gg.w << 32) | gg.z;
+        const uint64_t glo = ((uint64_t)gg.y << 32) | gg.x;
+        const double s = (double)ghi + (double)glo * 5.421010862427522e-20;   // G / 2^64
+        const double g = s * 8.928943354902097e-14;                           // 2 pi / 2^46
+        const double x = k * g * g;
+        const double E = exp_neg_bounded(x);
+        const double2 ww = w[j];
+        double mag, magb, m, mb;
+        if (E > 0.0) {
+            const double f = 2.0 * E;
+            va = make_double2(f * ww.x, f * ww.y);
+            vb = make_double2(g * va.x, g * va.y);
+            mag = sqrt(va.x * va.x + va.y * va.y);
+            magb = g * mag;
+            m = mag * (9.0 + 6.5 * x);
+            mb = magb * (12.0 + 6.5 * x);
+        } else {
+            mag = 2.0 * sqrt(ww.x * ww.x + ww.y * ww.y) * 1e-307;
+            magb = g * mag;
+            m = mag;
+            mb = magb;
+        }
+        if (j < N) {
+            q[0] = m; q[1] = mb; q[2] = mag; q[3] = magb;
+        } else {
+            q[4] = mag; q[5] = magb;
+        }
+    }
+    if (j < n_pad) {
+        a[j] = va;
+        b[j] = vb;
+    }
+#pragma unroll
+    for (int i = 0; i < K2_ENC_Q; ++i) {
+        const double s = enc_block_sum(q[i], red);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * K2_ENC_Q + i] = s;
+    }
+}
+
+// one workgroup: bound sums over the block partials in a fixed order; tail outputs (optional)
+__global__ __launch_bounds__(256) void k2_enclose_bounds_kernel(const double* __restrict__ part,
+                                                                int n_blocks,
+                                                                double* __restrict__ bounds,
+                                                                double* __restrict__ tail) {
+    __shared__ double red[4];
+#pragma unroll
+    for (int i = 0; i < K2_ENC_Q; ++i) {
+        double s = 0.0;
+        for (int blk = threadIdx.x; blk < n_blocks; blk += blockDim.x)
+            s += part[(size_t)blk * K2_ENC_Q + i];
+        s = enc_block_sum(s, red);
+        if (threadIdx.x == 0) bounds[i] = s;
+    }
+    if (threadIdx.x == 0 && tail) {
+        tail[0] = K2_ENC_SAFE * bounds[4] + K2_ENC_ABS;
+        tail[1] = K2_ENC_SAFE * bounds[5] + K2_ENC_ABS;
+    }
+}
+
+// C tiles over (split, candidate tile) workgroups with error-free accumulation; split s writes
+// rows [4 s, 4 s + 4) of partial: hi, lo of K2, then hi, lo of K2'
+template <int NC, int FL>
+__global__ __launch_bounds__(256, 2) void k2_enclose_kernel(
+    ZeroView zv, int nz, int zps, const uint4* __restrict__ cand, int n_cand, int n_ctiles,
+    double* __restrict__ partial, int n_cand_pad, const Tab64* __restrict__ tab64) {
+    __shared__ CTileSmem64<FL> sm;
+    __shared__ Tab64 s_tab[K2_TAB];
+    const int L = xcd_linear_block();
+    const int split = L / n_ctiles, ct = L - split * n_ctiles;
+    const int z0 = split * zps;
+    const int z1 = min(nz, z0 + zps);
+    const int tid = threadIdx.x;
+    for (int i = tid; i < K2_TAB; i += K2_WG) s_tab[i] = tab64[i];
+    Limbs4 v[NC];
+    AccDD acc[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * (K2_WG * NC) + i * K2_WG + tid;
+        uint4 cv = make_uint4(0, 0, 0, 0);
+        if (c < n_cand) cv = cand[c];
+        v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
+        acc[i].zero();
+    }
+    ctile_f64<NC, FL>(zv, z0, z1, v, acc, sm, s_tab);
+    double* prow = partial + (size_t)split * 4 * n_cand_pad;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * (K2_WG * NC) + i * K2_WG + tid;
+        prow[c] = acc[i].h0;
+        prow[n_cand_pad + c] = acc[i].l0;
+        if (FL & K2_D1) {
+            prow[2 * n_cand_pad + c] = acc[i].h1;
+            prow[3 * n_cand_pad + c] = acc[i].l1;
+        }
+    }
+}
+
+// midpoint and radius per candidate; splits combined by TwoSum in a fixed order
+__global__ __launch_bounds__(256) void k2_enclose_reduce_kernel(
+    const double* __restrict__ partial, int n_splits, int n_cand, int n_cand_pad, int n_q,
+    const double* __restrict__ bounds, double m_terms, double* __restrict__ K,
+    double* __restrict__ K_rad, double* __restrict__ dK, double* __restrict__ dK_rad) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cand) return;
+    const double mu = m_terms * K2_ENC_U;
+    for (int q = 0; q < n_q; ++q) {
+        double h = 0.0, l = 0.0;
+        for (int s = 0; s < n_splits; ++s) {
+            const double* row = partial + (size_t)s * 4 * n_cand_pad + (size_t)(2 * q) * n_cand_pad;
+            two_sum_acc(h, l, row[c]);
+            l += row[n_cand_pad + c];
+        }
+        const double val = h + l;
+        const double rad =
+            K2_ENC_SAFE * (K2_ENC_U * bounds[q] + mu * mu * bounds[2 + q] + K2_ENC_U * fabs(val)) +
+            K2_ENC_ABS;
+        if (q == 0) { K[c] = val; K_rad[c] = rad; }
+        else { dK[c] = val; dK_rad[c] = rad; }
+    }
+}
+
+namespace k2impl {
+
+void free_enclose(k2_table* t) {
+    (void)hipFree(t->enc_a); (void)hipFree(t->enc_b); (void)hipFree(t->enc_part);
+    (void)hipFree(t->enc_bounds); (void)hipFree(t->enc_partial);
+    t->enc_a = t->enc_b = nullptr;
+    t->enc_part = t->enc_bounds = t->enc_partial = nullptr;
+    t->enc_partial_bytes = 0;
+}
+
+template <int NC, int FL>
+static hipError_t launch_enclose_t(dim3 grid, const ZeroView& zv, int nz, int zps,
+                                   const uint4* cand, int n_cand, int n_ctiles, double* partial,
+                                   int n_cand_pad, const Tab64* tab64, hipStream_t st) {
+    hipLaunchKernelGGL((k2_enclose_kernel<NC, FL>), grid, dim3(K2_WG), 0, st, zv, nz, zps, cand,
+                       n_cand, n_ctiles, partial, n_cand_pad, tab64);
+    return hipGetLastError();
+}
+
+}  // namespace k2impl
+
+// ---- C ABI -------------------------------------------------------------------------------------
+
+using namespace k2impl;
+
+extern "C" int k2_exp_neg_host(uint32_t n, const double* x, double* out) {
+    if (n && (!x || !out)) return set_error(K2_E_ARG, "k2_exp_neg_host: null pointer");
+    for (uint32_t i = 0; i < n; ++i) out[i] = exp_neg_bounded(x[i]);
+    return K2_OK;
+}
+
+extern "C" int k2_enclose_device(k2_table* t, uint32_t n_cand, const uint64_t* V_dev, double k,
+                                 uint32_t n_zeros, uint32_t flags, double* K_dev,
+                                 double* K_rad_dev, double* dK_dev, double* dK_rad_dev,
+                                 double* tail_dev, void* stream) {
+    if (!t) return set_error(K2_E_ARG, "k2_enclose_device: null table");
+    if (flags & ~(K2_VALUE | K2_D1))
+        return set_error(K2_E_ARG, "k2_enclose_device: the enclosure mode computes K2 and K2' only");
+    const bool d1 = (flags & K2_D1) != 0;
+    if (n_cand && (!V_dev || !K_dev || !K_rad_dev || (d1 && (!dK_dev || !dK_rad_dev))))
+        return set_error(K2_E_ARG, "k2_enclose_device: null pointer");
+    if (n_cand > (1u << 30)) return set_error(K2_E_ARG, "k2_enclose_device: too many candidates");
+    if (n_zeros == 0) n_zeros = t->n;
+    if (n_zeros > t->n) return set_error(K2_E_ARG, "k2_enclose_device: n_zeros exceeds the table");
+    if (!(k >= 0.0) || !(k < 1e300)) return set_error(K2_E_ARG, "k2_enclose_device: k must be >= 0");
+    std::lock_guard<std::mutex> lk(t->mu);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipSetDevice(t->device);
+    const int n_blk = (int)(t->n_pad / K2_WG);
+    if (e == hipSuccess && !t->enc_a) e = hipMalloc((void**)&t->enc_a, (size_t)t->n_pad * sizeof(double2));
+    if (e == hipSuccess && !t->enc_b) e = hipMalloc((void**)&t->enc_b, (size_t)t->n_pad * sizeof(double2));
+    if (e == hipSuccess && !t->enc_part)
+        e = hipMalloc((void**)&t->enc_part, (size_t)n_blk * K2_ENC_Q * sizeof(double));
+    if (e == hipSuccess && !t->enc_bounds) e = hipMalloc((void**)&t->enc_bounds, 256);
+    if (e != hipSuccess) {
+        free_enclose(t);
+        return set_error(K2_E_NOMEM, std::string("k2_enclose_device: ") + hipGetErrorString(e));
+    }
+    const int nc = t->tiling.nc <= 1 ? 1 : 2;
+    const int n_ctiles = std::max(1, (int)((n_cand + 256u * nc - 1) / (256u * nc)));
+    const int n_cand_pad = n_ctiles * 256 * nc;
+    int n_splits = 0, zps = 0;
+    if (n_cand) split_range((int)n_zeros, n_ctiles, t->tiling.target_blocks, &n_splits, &zps);
+    const size_t need = (size_t)std::max(1, n_splits) * 4 * n_cand_pad * sizeof(double);
+    if (n_cand && need > t->enc_partial_bytes) {
+        (void)hipFree(t->enc_partial);
+        t->enc_partial = nullptr;
+        t->enc_partial_bytes = 0;
+        e = hipMalloc((void**)&t->enc_partial, need);
+        if (e != hipSuccess) return set_error(K2_E_NOMEM, "k2_enclose_device: partial sums");
+        t->enc_partial_bytes = need;
+    }
+    hipLaunchKernelGGL(k2_enclose_weights_kernel, dim3(n_blk), dim3(K2_WG), 0, st, t->g,
+                       (const double2*)t->w, (int)t->n, (int)t->n_pad, (int)n_zeros, k, t->enc_a,
+                       t->enc_b, t->enc_part);
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k2_enclose_bounds_kernel, dim3(1), dim3(K2_WG), 0, st, t->enc_part,
+                           n_blk, t->enc_bounds, tail_dev);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && n_cand && n_splits > 0) {
+        ZeroView zv{t->g, t->enc_a, t->enc_b, nullptr, nullptr, nullptr, nullptr};
+        const dim3 grid(n_ctiles * n_splits);
+        const uint4* cand = (const uint4*)V_dev;
+        if (nc == 1)
+            e = d1 ? launch_enclose_t<1, K2_D1>(grid, zv, n_zeros, zps, cand, n_cand, n_ctiles,
+                                                 t->enc_partial, n_cand_pad, t->tab64, st)
+                   : launch_enclose_t<1, 0>(grid, zv, n_zeros, zps, cand, n_cand, n_ctiles,
+                                            t->enc_partial, n_cand_pad, t->tab64, st);
+        else
+            e = d1 ? launch_enclose_t<2, K2_D1>(grid, zv, n_zeros, zps, cand, n_cand, n_ctiles,
+                                                 t->enc_partial, n_cand_pad, t->tab64, st)
+                   : launch_enclose_t<2, 0>(grid, zv, n_zeros, zps, cand, n_cand, n_ctiles,
+                                            t->enc_partial, n_cand_pad, t->tab64, st);
+    }
+    if (e == hipSuccess && n_cand) {
+        const double m_terms = (double)n_zeros + 2.0 * n_splits + 2.0;
+        hipLaunchKernelGGL(k2_enclose_reduce_kernel, dim3((n_cand + 255) / 256), dim3(256), 0, st,
+                           t->enc_partial, n_splits, (int)n_cand, n_cand_pad, d1 ? 2 : 1,
+                           t->enc_bounds, m_terms, K_dev, K_rad_dev, dK_dev, dK_rad_dev);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess)
+        return set_error(K2_E_HIP, std::string("k2_enclose_device: ") + hipGetErrorString(e));
+    return K2_OK;
+}

@@ -80,6 +80,14 @@ struct k2_table {
     int* polish_active = nullptr;
     int32_t* screen_params = nullptr;   // stage zero counts and thresholds of k2_screen_device
     size_t polish_cap = 0;
+    // enclosure mode (k2_enclose_device), allocated on first use: weights with a stated error
+    // bound, per-block bound sums, the final bound sums, split partials (hi, lo pairs)
+    double2* enc_a = nullptr;
+    double2* enc_b = nullptr;
+    double* enc_part = nullptr;
+    double* enc_bounds = nullptr;
+    double* enc_partial = nullptr;
+    size_t enc_partial_bytes = 0;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;             // second stream for the FP64 head of the mixed form
     hipEvent_t fork = nullptr, join = nullptr;
@@ -96,6 +104,11 @@ struct k2_table {
 namespace k2impl {
 
 hipError_t launch_eval(const LaunchArgs& a, bool mixed, hipStream_t st);
+// zeros per split (a multiple of 256) and split count for nz zeros, n_ctiles candidate tiles
+// and a target workgroup count
+void split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps);
+// releases the enclosure-mode buffers of a table
+void free_enclose(k2_table* t);
 hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,
                           WeightArrays& out, hipStream_t st);
 hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_cand_pad,

@@ -25,11 +25,6 @@ using namespace k2dev;
 
 #define K2_ZT 256   // zeros per LDS tile = threads per workgroup
 
-__device__ __forceinline__ int xcd_linear_block() {
-    const int nb = gridDim.x, b = blockIdx.x;
-    return (nb % 8 == 0) ? ((b & 7) * (nb >> 3) + (b >> 3)) : b;
-}
-
 // ---- per-k weighted zero arrays ---------------------------------------------------------------
 // a = 2 e^{-k g^2} w, b = g a, e = g^2 a (FP64) and the same in FP32; padded entries are zero.
 __global__ __launch_bounds__(256) void k2_weights_kernel(

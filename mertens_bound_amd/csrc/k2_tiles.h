@@ -13,6 +13,13 @@ namespace k2dev {
 
 #define K2_WG 256   // lanes per workgroup = zeros per LDS tile
 
+// workgroups are dealt round-robin to the 8 XCDs; this renumbers them so that consecutive
+// linear indices run on one XCD (and share its L2)
+__device__ __forceinline__ int xcd_linear_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    return (nb % 8 == 0) ? ((b & 7) * (nb >> 3) + (b >> 3)) : b;
+}
+
 // one k's weighted zero arrays (padded with zero weights past the table end)
 struct ZeroView {
     const uint4* g;
@@ -50,6 +57,21 @@ template <> struct Acc<true> {
     }
 };
 
+// FP64 sums with error-free accumulation (enclosure mode): h + l carries the sum, TwoSum
+// (Knuth) moves each addition's rounding error into l exactly
+__device__ __forceinline__ void two_sum_acc(double& h, double& l, double p) {
+    const double s = h + p;
+    const double bp = s - h;
+    const double e = (h - (s - bp)) + (p - bp);
+    h = s;
+    l += e;
+}
+
+struct AccDD {
+    double h0, l0, h1, l1;
+    __device__ __forceinline__ void zero() { h0 = l0 = h1 = l1 = 0.0; }
+};
+
 // one term: K += Re(a e^{i th}), K' += Re(i b e^{i th}), K'' -= Re(e e^{i th})
 template <int FL>
 __device__ __forceinline__ void accumulate(Acc<false>& A, double S, double C, double2 a,
@@ -64,6 +86,13 @@ __device__ __forceinline__ void accumulate(Acc<false>& A, double S, double C, do
         A.s2 = fma(-e.x, C, A.s2);
         A.s2 = fma(e.y, S, A.s2);
     }
+}
+
+template <int FL>
+__device__ __forceinline__ void accumulate(AccDD& A, double S, double C, double2 a, double2 b,
+                                           double2 e) {
+    two_sum_acc(A.h0, A.l0, fma(a.x, C, -a.y * S));
+    if (FL & K2_D1) two_sum_acc(A.h1, A.l1, -fma(b.x, S, b.y * C));
 }
 
 template <int FL>
@@ -103,11 +132,12 @@ constexpr size_t ctile_smem_bytes() {
                                                                : sizeof(CTileSmem32<FL>);
 }
 
-// Sums zeros [z0, z1) into acc for the NC candidates of this lane.  Enters and leaves with sm
-// free (the last action is a barrier).  tab: the (cos, sin) table in LDS of the matching width.
-template <int NC, int FL>
+// Sums zeros [z0, z1) into acc for the NC candidates of this lane (ACC: Acc<false>, or AccDD
+// in the enclosure mode).  Enters and leaves with sm free (the last action is a barrier).
+// tab: the (cos, sin) table in LDS of the matching width.
+template <int NC, int FL, class ACC>
 __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, const Limbs4 (&v)[NC],
-                                          Acc<false> (&acc)[NC], CTileSmem64<FL>& sm,
+                                          ACC (&acc)[NC], CTileSmem64<FL>& sm,
                                           const Tab64* tab) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;

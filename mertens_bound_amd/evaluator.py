@@ -84,6 +84,9 @@ def _declare(L):
     L.k2_screen_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                                    ctypes.POINTER(ctypes.c_float), _vp, _vp, _vp, _vp]
+    L.k2_enclose_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
+                                    ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.k2_exp_neg_host.argtypes = [ctypes.c_uint32, _f64p, _f64p]
     L.k2_polish_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                    ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp, _vp,
                                    _vp, _vp]
@@ -256,6 +259,32 @@ class K2Table:
                                                ok.data_ptr(), st), "k2_screen_device")
         return val, best, ok
 
+    def enclose_device(self, V, k, n_zeros=None, d1=True, stream=None):
+        """Enclosure mode (k2_enclose_device): the exact K2 (and K2') over the first n_zeros
+        zeros lies within K +- K_rad (dK +- dK_rad).  Returns (K, K_rad, dK, dK_rad, tail) with
+        tail = [bound on sum_{j >= n_zeros} 2 e^{-k g^2} |w_j|, the same with the factor g_j]
+        (float64 CUDA tensors; dK, dK_rad None when d1 is False)."""
+        import torch
+        n = V.shape[0]
+        mk = lambda: torch.empty(n, dtype=torch.float64, device=V.device)
+        K, K_rad = mk(), mk()
+        dK, dK_rad = (mk(), mk()) if d1 else (None, None)
+        tail = torch.empty(2, dtype=torch.float64, device=V.device)
+        st = stream if stream is not None else torch.cuda.current_stream(V.device).cuda_stream
+        _check(load_library().k2_enclose_device(
+            self._h, n, V.data_ptr(), float(k), int(n_zeros or 0), VALUE | (D1 if d1 else 0),
+            K.data_ptr(), K_rad.data_ptr(), dK.data_ptr() if d1 else None,
+            dK_rad.data_ptr() if d1 else None, tail.data_ptr(), st), "k2_enclose_device")
+        return K, K_rad, dK, dK_rad, tail
+
+    def enclose(self, V, k, n_zeros=None, d1=True):
+        """Host-array form of enclose_device: V (n, 2) uint64 words; returns numpy arrays."""
+        import torch
+        Vt = v_tensor(np.ascontiguousarray(V, dtype=np.uint64))
+        res = self.enclose_device(Vt, k, n_zeros, d1)
+        torch.cuda.synchronize()
+        return tuple(None if r is None else r.cpu().numpy() for r in res)
+
     # profiling ----------------------------------------------------------------------------
     def profile(self, enable=True):
         _check(load_library().k2_profile(self._h, 1 if enable else 0), "k2_profile")
@@ -265,6 +294,15 @@ class K2Table:
         _check(load_library().k2_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n)),
                "k2_kernel_time")
         return ms.value, n.value
+
+
+def exp_neg_host(x):
+    """The enclosure mode's e^{-x} evaluated on the host (same operations as on the device)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    _check(load_library().k2_exp_neg_host(x.size, x.ctypes.data_as(_f64p),
+                                          out.ctypes.data_as(_f64p)), "k2_exp_neg_host")
+    return out
 
 
 def _requests(requests):
