@@ -211,7 +211,8 @@ def run_eval(args, form):
         evals / (ms * 1e-3), "candidate-zero evals/s", ms,
         "f64" if form == "plain" else "f32+f64",
         {"workload": f"K2+dK2/dv, {n_cand:,} candidates x {tb.n:,} zeros, k={c['k']}",
-         "form": form, "n_cand": n_cand, "n_zeros": tb.n, "tiling": t.get_tiling()})
+         "form": form, "n_cand": n_cand, "n_zeros": tb.n, "tiling": t.get_tiling(),
+         "fp64_head_zeros": t.mixed_head(c["k"]) if form == "mixed" else tb.n})
     res["accuracy"] = {"max_abs_dK2_vs_oracle": acc, "oracle_sample": gname, "bar": 1e-9,
                        "ok": (acc is not None and acc <= 1e-9)}
     res["roofline"] = roofline(form, evals, k_ms / max(1, k_n), alg_bytes,

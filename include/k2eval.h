@@ -61,7 +61,10 @@ typedef struct k2_request {
 typedef struct k2_tiling {
     int32_t nc;            /* candidates per lane: 1, 2, 4 or 8 */
     int32_t target_blocks; /* workgroups per launch the zero splits aim at */
-    int32_t mixed_head;    /* zeros evaluated in FP64 before the FP32 form takes over */
+    int32_t mixed_head;    /* mixed form: zeros evaluated in FP64 before the FP32 form takes over;
+                              0 = the shortest multiple of 4096 zeros after which the weight
+                              magnitudes sum_j 2 e^{-k g_j^2} |w_j| add up to <= mixed_tail */
+    double mixed_tail;     /* 0 = 0.16 (FP32 tail error about 7e-10 per unit of magnitude) */
 } k2_tiling;
 
 const char *k2_last_error(void);
@@ -149,6 +152,10 @@ int k2_enclose_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, doubl
 /* debug: the enclosure mode's e^{-x} (same operations as on the device), for checking its
  * error bound on the host */
 int k2_exp_neg_host(uint32_t n, const double *x, double *out);
+
+/* The FP64 head (zeros) the mixed form uses for Gaussian constant k over the first n_zeros
+ * zeros under the table's tiling (computes and caches k's weights on first use). */
+int k2_mixed_head(k2_table *t, double k, uint32_t n_zeros, uint32_t *head);
 
 /* Staged pre-screen on the first zeros (device buffers): for shifts m = -n_shift..n_shift the
  * sums h_m = K2 over the first n zeros at v + m*shift_step, stage value max_m |h_m|; stage s

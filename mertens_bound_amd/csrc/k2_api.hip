@@ -142,12 +142,14 @@ extern "C" uint32_t k2_table_size(const k2_table* t) { return t ? t->n : 0; }
 extern "C" int k2_set_tiling(k2_table* t, const k2_tiling* ti) {
     if (!t || !ti) return fail(K2_E_ARG, "k2_set_tiling: null pointer");
     k2_tiling v = *ti;
-    if (v.nc == 0) v.nc = 4;
-    if (v.target_blocks == 0) v.target_blocks = 2048;
+    if (v.nc == 0) v.nc = 2;
+    if (v.target_blocks == 0) v.target_blocks = 3072;
+    if (v.mixed_tail == 0.0) v.mixed_tail = K2_MIXED_TAIL;
     if (v.nc != 1 && v.nc != 2 && v.nc != 4 && v.nc != 8)
         return fail(K2_E_ARG, "k2_set_tiling: nc must be 1, 2, 4 or 8");
-    if (v.target_blocks < 1 || v.mixed_head < 0)
-        return fail(K2_E_ARG, "k2_set_tiling: target_blocks >= 1 and mixed_head >= 0 required");
+    if (v.target_blocks < 1 || v.mixed_head < 0 || !(v.mixed_tail > 0.0))
+        return fail(K2_E_ARG,
+                    "k2_set_tiling: target_blocks >= 1, mixed_head >= 0 and mixed_tail > 0 required");
     std::lock_guard<std::mutex> lk(t->mu);
     t->tiling = v;
     return K2_OK;
@@ -161,6 +163,7 @@ extern "C" int k2_get_tiling(const k2_table* t, k2_tiling* ti) {
 
 int k2impl::alloc_weights(WeightArrays& w, size_t np) {
     hipError_t e = hipMalloc((void**)&w.a, np * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc((void**)&w.blk, (np / K2_ABS_BLOCK) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&w.b, np * sizeof(double2));
     if (e == hipSuccess) e = hipMalloc((void**)&w.e, np * sizeof(double2));
     if (e == hipSuccess) e = hipMalloc((void**)&w.af, np * sizeof(float2));
@@ -175,9 +178,39 @@ int k2impl::alloc_weights(WeightArrays& w, size_t np) {
 
 void k2impl::free_weights(WeightArrays& w) {
     (void)hipFree(w.a); (void)hipFree(w.b); (void)hipFree(w.e);
-    (void)hipFree(w.af); (void)hipFree(w.bf); (void)hipFree(w.ef);
+    (void)hipFree(w.af); (void)hipFree(w.bf); (void)hipFree(w.ef); (void)hipFree(w.blk);
     w.a = w.b = w.e = nullptr;
     w.af = w.bf = w.ef = nullptr;
+    w.blk = nullptr;
+    w.abs_blocks.clear();
+}
+
+int k2impl::compute_weights(const double* gamma, const double* wz, int n, int n_pad, double k,
+                            WeightArrays& w, hipStream_t st) {
+    hipError_t e = launch_weights(gamma, wz, n, n_pad, k, w, st);
+    if (e == hipSuccess) e = launch_abs_blocks(w.a, n_pad, w.blk, st);
+    w.abs_blocks.assign(n_pad / K2_ABS_BLOCK, 0.0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(w.abs_blocks.data(), w.blk, w.abs_blocks.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(K2_E_HIP, std::string("weight arrays: ") + hipGetErrorString(e));
+    w.k = k;
+    return K2_OK;
+}
+
+uint32_t k2impl::mixed_head_zeros(const std::vector<double>& blocks, uint32_t nz, int fixed_head,
+                                  double tail) {
+    if (fixed_head > 0) return std::min<uint32_t>(nz, (uint32_t)fixed_head);
+    const uint32_t nb = std::min<uint32_t>((uint32_t)blocks.size(),
+                                           (nz + K2_ABS_BLOCK - 1) / K2_ABS_BLOCK);
+    double sum = 0.0;
+    uint32_t hb = nb;   // head = hb whole blocks
+    while (hb > 0 && sum + blocks[hb - 1] <= tail) {
+        sum += blocks[hb - 1];
+        --hb;
+    }
+    return std::min<uint32_t>(nz, hb * K2_ABS_BLOCK);
 }
 
 // weight arrays for k, computed on stream st on first use and waited for, so that work on any
@@ -201,8 +234,11 @@ static int get_weights(k2_table* t, double k, hipStream_t st, WeightArrays** out
     }
     slot->k = k;
     slot->last_use = t->use_clock;
-    HIPCHK(launch_weights(t->gamma, t->w, (int)t->n, (int)t->n_pad, k, *slot, st));
-    HIPCHK(hipStreamSynchronize(st));
+    int rc = compute_weights(t->gamma, t->w, (int)t->n, (int)t->n_pad, k, *slot, st);
+    if (rc) {
+        slot->k = -1.0;   // never matches a valid k: recomputed on next use
+        return rc;
+    }
     *out = slot;
     return K2_OK;
 }
@@ -236,12 +272,14 @@ void k2impl::split_range(int nz, int n_ctiles, int target, int* n_splits, int* z
     *n_splits = (nz + z - 1) / z;
 }
 
-static Plan make_plan(const k2_table* t, int n_cand, int nz, bool mixed) {
+static Plan make_plan(const k2_table* t, const WeightArrays* ws, int n_cand, int nz, bool mixed) {
     Plan p;
     const int nc = t->tiling.nc;
     p.n_ctiles = (n_cand + 256 * nc - 1) / (256 * nc);
     p.n_cand_pad = p.n_ctiles * 256 * nc;
-    p.head_end = mixed ? std::min(nz, t->tiling.mixed_head) : nz;
+    p.head_end = mixed ? (int)mixed_head_zeros(ws->abs_blocks, (uint32_t)nz, t->tiling.mixed_head,
+                                               t->tiling.mixed_tail)
+                       : nz;
     const int tb = t->tiling.target_blocks;
     if (mixed && p.head_end < nz) {
         // the FP64 head runs beside the FP32 tail on a second stream; give it workgroups in
@@ -262,7 +300,7 @@ static Plan make_plan(const k2_table* t, int n_cand, int nz, bool mixed) {
 static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* ws, int nz,
                      unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
     const bool mixed = (flags & K2_MIXED) != 0;
-    Plan p = make_plan(t, n_cand, nz, mixed);
+    Plan p = make_plan(t, ws, n_cand, nz, mixed);
     const int n_rows = std::max(1, p.sp_head + p.sp_tail);
     int rc = ensure_bytes((void**)&t->partial, &t->partial_bytes,
                           (size_t)n_rows * 3 * p.n_cand_pad * sizeof(double));
@@ -415,6 +453,20 @@ extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, d
                              steps_dev, st));
     }
     return k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
+}
+
+extern "C" int k2_mixed_head(k2_table* t, double k, uint32_t n_zeros, uint32_t* head) {
+    if (!t || !head) return fail(K2_E_ARG, "k2_mixed_head: null pointer");
+    if (!(k >= 0.0) || !std::isfinite(k)) return fail(K2_E_ARG, "k2_mixed_head: k must be >= 0");
+    if (n_zeros == 0) n_zeros = t->n;
+    if (n_zeros > t->n) return fail(K2_E_ARG, "k2_mixed_head: n_zeros exceeds the table");
+    std::lock_guard<std::mutex> lk(t->mu);
+    HIPCHK(hipSetDevice(t->device));
+    WeightArrays* ws = nullptr;
+    int rc = get_weights(t, k, t->stream, &ws);
+    if (rc) return rc;
+    *head = mixed_head_zeros(ws->abs_blocks, n_zeros, t->tiling.mixed_head, t->tiling.mixed_tail);
+    return K2_OK;
 }
 
 extern "C" int k2_phase_words(k2_table* t, uint32_t n_cand, const uint64_t* V, uint32_t z0,

@@ -154,9 +154,6 @@ __global__ __launch_bounds__(256) void k2_batch_reduce_kernel(
 
 namespace k2impl {
 
-hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,
-                          WeightArrays& out, hipStream_t st);
-
 struct BatchPlan {
     std::vector<BatchItem> items;
     std::vector<CandDesc> desc;
@@ -166,8 +163,9 @@ struct BatchPlan {
 
 // requests -> items and candidate descriptors (see the file comment)
 std::string plan_batch(uint32_t n_req, const k2_request* req, uint64_t n_cand_total,
-                       uint32_t table_n, const std::vector<double>& kslots, bool mixed,
-                       int mixed_head, const k2_batch_tiling& ti, BatchPlan& P) {
+                       uint32_t table_n, const std::vector<double>& kslots,
+                       const std::vector<WeightArrays>& kw, bool mixed, const k2_tiling& et,
+                       const k2_batch_tiling& ti, BatchPlan& P) {
     const int NC = ti.nc, CZ = ti.cz;
     const uint32_t CW = 256u * NC;
     P.desc.assign(n_cand_total, CandDesc{0, 0, 0, 0});
@@ -185,7 +183,8 @@ std::string plan_batch(uint32_t n_req, const k2_request* req, uint64_t n_cand_to
             seen[c] = 1;
         }
         const int ks = (int)(std::find(kslots.begin(), kslots.end(), q.k) - kslots.begin());
-        const uint32_t head = mixed ? std::min<uint32_t>(nz, (uint32_t)mixed_head) : nz;
+        const uint32_t head =
+            mixed ? mixed_head_zeros(kw[ks].abs_blocks, nz, et.mixed_head, et.mixed_tail) : nz;
         std::vector<Tile> tiles;
         const uint32_t n_full = q.n_cand / CW, rem = q.n_cand % CW;
         for (uint32_t i = 0; i < n_full; ++i)
@@ -364,14 +363,18 @@ extern "C" int k2_batch_create(k2_table* t, uint32_t n_req, const k2_request* re
     }
     if (ks.size() > 4) return berr(K2_E_ARG, "k2_batch_create: more than 4 distinct k values");
     const bool mixed = (flags & K2_MIXED) != 0;
-    BatchPlan P;
-    std::string msg = plan_batch(n_req, req, n_cand_total, t->n, ks, mixed, t->tiling.mixed_head,
-                                 ti, P);
-    if (!msg.empty()) return berr(K2_E_ARG, "k2_batch_create: " + msg);
+    // requests are checked before anything is allocated (weights are not needed for that)
+    {
+        std::vector<WeightArrays> none(ks.size());
+        for (auto& w : none) w.abs_blocks.assign(t->n_pad / K2_ABS_BLOCK, 0.0);
+        BatchPlan Q;
+        std::string msg = plan_batch(n_req, req, n_cand_total, t->n, ks, none, false, t->tiling,
+                                     ti, Q);
+        if (!msg.empty()) return berr(K2_E_ARG, "k2_batch_create: " + msg);
+    }
 
     k2_batch* b = new k2_batch();
     b->t = t;
-    b->n_items = (int)P.items.size();
     b->n_cand = n_cand_total;
     b->cand = (const uint4*)V_dev;
     b->K = K_dev;
@@ -385,10 +388,19 @@ extern "C" int k2_batch_create(k2_table* t, uint32_t n_req, const k2_request* re
             k2_batch_destroy(b);
             return K2_E_NOMEM;
         }
-        w.k = k;
         b->w.push_back(w);
-        BCHK(launch_weights(t->gamma, t->w, (int)t->n, (int)t->n_pad, k, b->w.back(), b->cap));
+        if (compute_weights(t->gamma, t->w, (int)t->n, (int)t->n_pad, k, b->w.back(), b->cap)) {
+            k2_batch_destroy(b);
+            return K2_E_HIP;
+        }
     }
+    BatchPlan P;
+    std::string msg = plan_batch(n_req, req, n_cand_total, t->n, ks, b->w, mixed, t->tiling, ti, P);
+    if (!msg.empty()) {
+        k2_batch_destroy(b);
+        return berr(K2_E_ARG, "k2_batch_create: " + msg);
+    }
+    b->n_items = (int)P.items.size();
     for (size_t i = 0; i < 4; ++i) {
         const WeightArrays& w = b->w.empty() ? WeightArrays{} : b->w[std::min(i, b->w.size() - 1)];
         b->views.zv[i] = ZeroView{t->g, w.a, w.b, w.e, w.af, w.bf, w.ef};

@@ -20,8 +20,13 @@ struct WeightArrays {
     float2* af = nullptr;
     float2* bf = nullptr;
     float2* ef = nullptr;
+    double* blk = nullptr;            // sum |a| per K2_ABS_BLOCK zeros (device)
+    std::vector<double> abs_blocks;   // the same on the host
     uint64_t last_use = 0;
 };
+
+#define K2_ABS_BLOCK 4096   // zeros per weight-magnitude block (a multiple of the table padding)
+#define K2_MIXED_TAIL 0.16  // default FP32-tail magnitude of the mixed form
 
 struct LaunchArgs {
     int nc = 4;
@@ -51,6 +56,13 @@ int upload_tables(k2dev::Tab64** t64, k2dev::Tab32** t32, hipStream_t st);
 int set_error(int code, const std::string& msg);
 int alloc_weights(WeightArrays& w, size_t n_pad);
 void free_weights(WeightArrays& w);
+// a, b, e (FP64 and FP32) for k, then the magnitude blocks, copied to w.abs_blocks; synchronises st
+int compute_weights(const double* gamma, const double* wz, int n, int n_pad, double k,
+                    WeightArrays& w, hipStream_t st);
+// FP64 head of the mixed form over zeros [0, nz): fixed_head if > 0, else the shortest multiple of
+// K2_ABS_BLOCK zeros after which the magnitude blocks up to nz (whole blocks) sum to <= tail
+uint32_t mixed_head_zeros(const std::vector<double>& blocks, uint32_t nz, int fixed_head,
+                          double tail);
 
 }  // namespace k2impl
 
@@ -91,7 +103,7 @@ struct k2_table {
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;             // second stream for the FP64 head of the mixed form
     hipEvent_t fork = nullptr, join = nullptr;
-    k2_tiling tiling{2, 3072, 65536};
+    k2_tiling tiling{2, 3072, 0, K2_MIXED_TAIL};
     bool profile = false;
     std::vector<ProfPair> prof_pending;
     std::vector<hipEvent_t> ev_pool;
@@ -111,6 +123,7 @@ void split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps);
 void free_enclose(k2_table* t);
 hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,
                           WeightArrays& out, hipStream_t st);
+hipError_t launch_abs_blocks(const double2* a, int n_pad, double* blk, hipStream_t st);
 hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_cand_pad,
                          unsigned flags, double* K, double* dK, double* d2K, hipStream_t st);
 hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand, int n_cand,

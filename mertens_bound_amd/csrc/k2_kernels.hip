@@ -48,6 +48,22 @@ __global__ __launch_bounds__(256) void k2_weights_kernel(
     ef[j] = make_float2((float)ve.x, (float)ve.y);
 }
 
+// sum |a_j| per K2_ABS_BLOCK zeros (fixed order): the magnitudes that size the mixed form's head
+__global__ __launch_bounds__(256) void k2_abs_blocks_kernel(const double2* __restrict__ a,
+                                                            double* __restrict__ blk) {
+    __shared__ double red[4];
+    const size_t base = (size_t)blockIdx.x * K2_ABS_BLOCK;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < K2_ABS_BLOCK; i += 256) {
+        const double2 v = a[base + i];
+        s += sqrt(v.x * v.x + v.y * v.y);
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // ---- one-launch evaluation: C tiles over (split, candidate tile) workgroups -----------------------
 // Split s covers zeros [z_begin + s*zps, ...) up to z_end and writes partial row row_base + s.
 // F32 selects the precision form (exact top-32-bit phase, FP32 sin/cos and weights, compensated
@@ -238,6 +254,11 @@ hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad
     const int nb = (n_pad + 255) / 256;
     hipLaunchKernelGGL(k2_weights_kernel, dim3(nb), dim3(256), 0, st, gamma, (const double2*)w, n,
                        n_pad, k, out.a, out.b, out.e, out.af, out.bf, out.ef);
+    return hipGetLastError();
+}
+
+hipError_t launch_abs_blocks(const double2* a, int n_pad, double* blk, hipStream_t st) {
+    hipLaunchKernelGGL(k2_abs_blocks_kernel, dim3(n_pad / K2_ABS_BLOCK), dim3(256), 0, st, a, blk);
     return hipGetLastError();
 }
 

@@ -42,7 +42,7 @@ class k2_request(ctypes.Structure):
 
 class k2_tiling(ctypes.Structure):
     _fields_ = [("nc", ctypes.c_int32), ("target_blocks", ctypes.c_int32),
-                ("mixed_head", ctypes.c_int32)]
+                ("mixed_head", ctypes.c_int32), ("mixed_tail", ctypes.c_double)]
 
 
 class k2_batch_tiling(ctypes.Structure):
@@ -87,6 +87,8 @@ def _declare(L):
     L.k2_enclose_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                     ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
     L.k2_exp_neg_host.argtypes = [ctypes.c_uint32, _f64p, _f64p]
+    L.k2_mixed_head.argtypes = [_vp, ctypes.c_double, ctypes.c_uint32,
+                                ctypes.POINTER(ctypes.c_uint32)]
     L.k2_polish_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                    ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp, _vp,
                                    _vp, _vp]
@@ -166,17 +168,27 @@ class K2Table:
         return self._h
 
     # tiling -----------------------------------------------------------------------------
-    def set_tiling(self, nc=None, target_blocks=None, mixed_head=None):
+    def set_tiling(self, nc=None, target_blocks=None, mixed_head=None, mixed_tail=None):
+        """mixed_head 0 chooses the FP64 head from the weight magnitudes (tail <= mixed_tail)."""
         cur = self.get_tiling()
         t = k2_tiling(nc if nc is not None else cur["nc"],
                       target_blocks if target_blocks is not None else cur["target_blocks"],
-                      mixed_head if mixed_head is not None else cur["mixed_head"])
+                      mixed_head if mixed_head is not None else cur["mixed_head"],
+                      mixed_tail if mixed_tail is not None else cur["mixed_tail"])
         _check(load_library().k2_set_tiling(self._h, ctypes.byref(t)), "k2_set_tiling")
 
     def get_tiling(self):
         t = k2_tiling()
         _check(load_library().k2_get_tiling(self._h, ctypes.byref(t)), "k2_get_tiling")
-        return {"nc": t.nc, "target_blocks": t.target_blocks, "mixed_head": t.mixed_head}
+        return {"nc": t.nc, "target_blocks": t.target_blocks, "mixed_head": t.mixed_head,
+                "mixed_tail": t.mixed_tail}
+
+    def mixed_head(self, k, n_zeros=None):
+        """FP64 head (zeros) the mixed form uses for k over the first n_zeros zeros."""
+        h = ctypes.c_uint32()
+        _check(load_library().k2_mixed_head(self._h, float(k), int(n_zeros or 0), ctypes.byref(h)),
+               "k2_mixed_head")
+        return h.value
 
     # host-buffer entry points -------------------------------------------------------------
     def eval(self, V, k, n_zeros=None, flags=VALUE | D1 | D2):

@@ -131,3 +131,47 @@ def test_config2_subsample_vs_golden():
     em = np.max(np.abs(Km - g["K"]))
     print(f"config2 sub256 mixed max|dK2|={em:.3e}")
     assert em <= 1e-9
+
+
+def _head_rule(tb, k, nz, tail=0.16):
+    """The mixed form's FP64 head: shortest multiple of 4096 zeros after which the weight
+    magnitudes 2 e^{-k g^2} |w| of the whole 4096-zero blocks up to nz sum to <= tail."""
+    a = 2 * np.exp(-k * tb.gamma ** 2) * np.abs(tb.w)
+    npad = (tb.n + 4095) // 4096 * 4096
+    ap = np.zeros(npad)
+    ap[:tb.n] = a
+    blocks = ap.reshape(-1, 4096).sum(axis=1)
+    nb = min(len(blocks), (nz + 4095) // 4096)
+    s, hb = 0.0, nb
+    while hb > 0 and s + blocks[hb - 1] <= tail:
+        s += blocks[hb - 1]
+        hb -= 1
+    return min(nz, hb * 4096)
+
+
+@pytest.fixture(scope="module")
+def mid_table():
+    tb = synth.make_zero_table(200_000, 5)
+    return tb, ev.K2Table(tb)
+
+
+@pytest.mark.parametrize("k", list(synth.BATCH_KS))
+def test_adaptive_mixed_head(mid_table, k):
+    tb, t = mid_table
+    for nz in (tb.n, 50_000, 3000):
+        h = t.mixed_head(k, nz)
+        want = _head_rule(tb, k, nz)
+        print(f"k={k} n_zeros={nz}: FP64 head {h} (rule {want})")
+        assert abs(h - want) <= 4096 and h % 4096 in (0, nz % 4096)
+    V = synth.make_candidates(512, 3.4e18, 3.6e18, 5)
+    Kp, dKp, _ = t.eval(V, k, flags=ev.VALUE | ev.D1)
+    Km, dKm, _ = t.eval(V, k, flags=ev.VALUE | ev.D1 | ev.MIXED)
+    err = np.max(np.abs(Km - Kp))
+    print(f"k={k}: mixed vs FP64 max|dK2|={err:.3e}")
+    assert err <= 1e-9
+    t.set_tiling(mixed_head=8192)
+    try:
+        assert t.mixed_head(k) == 8192
+    finally:
+        t.set_tiling(mixed_head=0)
+    assert t.get_tiling()["mixed_tail"] == pytest.approx(0.16)
