@@ -65,6 +65,8 @@ typedef struct k2_tiling {
                               0 = the shortest multiple of 4096 zeros after which the weight
                               magnitudes sum_j 2 e^{-k g_j^2} |w_j| add up to <= mixed_tail */
     double mixed_tail;     /* 0 = 0.16 (FP32 tail error about 7e-10 per unit of magnitude) */
+    int32_t z_max_cand;    /* candidate counts up to this run as Z tiles (lanes over zeros, 8
+                              candidates per workgroup); 0 = library default, -1 = never */
 } k2_tiling;
 
 const char *k2_last_error(void);

@@ -145,6 +145,8 @@ extern "C" int k2_set_tiling(k2_table* t, const k2_tiling* ti) {
     if (v.nc == 0) v.nc = 2;
     if (v.target_blocks == 0) v.target_blocks = 3072;
     if (v.mixed_tail == 0.0) v.mixed_tail = K2_MIXED_TAIL;
+    if (v.z_max_cand == 0) v.z_max_cand = K2_Z_MAX_CAND;
+    if (v.z_max_cand < -1) return fail(K2_E_ARG, "k2_set_tiling: z_max_cand must be >= -1");
     if (v.nc != 1 && v.nc != 2 && v.nc != 4 && v.nc != 8)
         return fail(K2_E_ARG, "k2_set_tiling: nc must be 1, 2, 4 or 8");
     if (v.target_blocks < 1 || v.mixed_head < 0 || !(v.mixed_tail > 0.0))
@@ -263,7 +265,7 @@ struct Plan {
 
 void k2impl::split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps) {
     if (nz <= 0) { *n_splits = 0; *zps = 0; return; }
-    const int max_sp = (nz + 255) / 256;
+    const int max_sp = (nz + K2_MIN_SPLIT - 1) / K2_MIN_SPLIT;
     int sp = (target + n_ctiles - 1) / n_ctiles;
     sp = std::max(1, std::min(sp, max_sp));
     int z = (nz + sp - 1) / sp;
@@ -296,9 +298,77 @@ static Plan make_plan(const k2_table* t, const WeightArrays* ws, int n_cand, int
     return p;
 }
 
+// zero chunks of the Z-tile path: about target / n_groups chunks, at least K2_Z_CHUNK zeros each
+static void z_chunks(int len, int n_groups, int target, int* n_ch, int* zpc) {
+    if (len <= 0) { *n_ch = 0; *zpc = 0; return; }
+    const int want = std::max(1, target / n_groups);
+    int z = std::max(K2_Z_CHUNK, (len + want - 1) / want);
+    z = (z + 255) / 256 * 256;
+    *zpc = z;
+    *n_ch = (len + z - 1) / z;
+}
+
+static void fill_args(k2_table* t, WeightArrays* ws, const uint4* cand, int n_cand,
+                      unsigned flags, LaunchArgs& a) {
+    a.nc = t->tiling.nc;
+    a.flags = flags;
+    a.zg = t->g;
+    a.za = ws->a; a.zb = ws->b; a.ze = ws->e;
+    a.zaf = ws->af; a.zbf = ws->bf; a.zef = ws->ef;
+    a.cand = cand;
+    a.n_cand = n_cand;
+    a.partial = t->partial;
+    a.tab64 = t->tab64;
+    a.tab32 = t->tab32;
+}
+
+// few candidates: Z tiles (lanes over zeros, K2_Z_CZ candidates per workgroup), FP64 head chunks
+// then FP32 tail chunks in the mixed form, one wavefront per candidate in the reduction
+static int eval_core_z(k2_table* t, int n_cand, const uint4* cand, WeightArrays* ws, int nz,
+                       unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
+    const bool mixed = (flags & K2_MIXED) != 0;
+    const int head = mixed ? (int)mixed_head_zeros(ws->abs_blocks, (uint32_t)nz,
+                                                   t->tiling.mixed_head, t->tiling.mixed_tail)
+                           : nz;
+    const int n_groups = (n_cand + K2_Z_CZ - 1) / K2_Z_CZ;
+    int ch_h, zpc_h, ch_t, zpc_t;
+    z_chunks(head, n_groups, t->tiling.target_blocks, &ch_h, &zpc_h);
+    z_chunks(nz - head, n_groups, t->tiling.target_blocks, &ch_t, &zpc_t);
+    const int n_rows = std::max(1, ch_h + ch_t);
+    const size_t row = (size_t)n_groups * 3 * K2_Z_CZ;
+    int rc = ensure_bytes((void**)&t->partial, &t->partial_bytes, n_rows * row * sizeof(double));
+    if (rc) return rc;
+    LaunchArgs a;
+    fill_args(t, ws, cand, n_cand, flags, a);
+    ProfPair pp{nullptr, nullptr};
+    if (t->profile) {
+        pp.a = pool_event(t);
+        pp.b = pool_event(t);
+        HIPCHK(hipEventRecord(pp.a, st));
+    }
+    if (ch_h > 0) {
+        a.z_begin = 0; a.z_end = head; a.row_base = 0;
+        HIPCHK(launch_eval_z(a, false, n_groups, ch_h, zpc_h, st));
+    }
+    if (ch_t > 0) {
+        a.z_begin = head; a.z_end = nz; a.row_base = ch_h;
+        HIPCHK(launch_eval_z(a, true, n_groups, ch_t, zpc_t, st));
+    }
+    if (t->profile) {
+        HIPCHK(hipEventRecord(pp.b, st));
+        t->prof_pending.push_back(pp);
+    }
+    if (ch_h + ch_t == 0) HIPCHK(hipMemsetAsync(t->partial, 0, row * sizeof(double), st));
+    HIPCHK(launch_reduce_z(t->partial, n_rows, n_groups, n_cand, flags, K, dK, d2K, st));
+    return K2_OK;
+}
+
 // evaluation of one candidate list over zeros [0, nz) with weights ws; outputs on device
 static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* ws, int nz,
                      unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
+    const int zmax = t->tiling.z_max_cand == 0 ? K2_Z_MAX_CAND : t->tiling.z_max_cand;
+    if (n_cand > 0 && n_cand <= zmax)
+        return eval_core_z(t, n_cand, cand, ws, nz, flags, K, dK, d2K, st);
     const bool mixed = (flags & K2_MIXED) != 0;
     Plan p = make_plan(t, ws, n_cand, nz, mixed);
     const int n_rows = std::max(1, p.sp_head + p.sp_tail);

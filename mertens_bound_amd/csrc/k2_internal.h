@@ -27,6 +27,10 @@ struct WeightArrays {
 
 #define K2_ABS_BLOCK 4096   // zeros per weight-magnitude block (a multiple of the table padding)
 #define K2_MIXED_TAIL 0.16  // default FP32-tail magnitude of the mixed form
+#define K2_Z_CZ 8           // candidates per Z-tile workgroup of the one-launch path
+#define K2_Z_MAX_CAND 384   // default: candidate counts up to this use Z tiles (measured crossover)
+#define K2_Z_CHUNK 8192     // fewest zeros per Z-tile workgroup
+#define K2_MIN_SPLIT 256    // fewest zeros per C-tile split (one LDS tile)
 
 struct LaunchArgs {
     int nc = 4;
@@ -103,7 +107,7 @@ struct k2_table {
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;             // second stream for the FP64 head of the mixed form
     hipEvent_t fork = nullptr, join = nullptr;
-    k2_tiling tiling{2, 3072, 0, K2_MIXED_TAIL};
+    k2_tiling tiling{2, 3072, 0, K2_MIXED_TAIL, K2_Z_MAX_CAND};
     bool profile = false;
     std::vector<ProfPair> prof_pending;
     std::vector<hipEvent_t> ev_pool;
@@ -124,6 +128,12 @@ void free_enclose(k2_table* t);
 hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,
                           WeightArrays& out, hipStream_t st);
 hipError_t launch_abs_blocks(const double2* a, int n_pad, double* blk, hipStream_t st);
+// Z-tile one-launch path: workgroup (chunk, group) sums zeros [z_begin + chunk*zpc, ...) for
+// candidates [group*K2_Z_CZ, ...) into partial row (row_base + chunk) * n_groups + group
+hipError_t launch_eval_z(const LaunchArgs& a, bool f32, int n_groups, int n_chunks, int zpc,
+                         hipStream_t st);
+hipError_t launch_reduce_z(const double* partial, int n_rows, int n_groups, int n_cand,
+                           unsigned flags, double* K, double* dK, double* d2K, hipStream_t st);
 hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_cand_pad,
                          unsigned flags, double* K, double* dK, double* d2K, hipStream_t st);
 hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand, int n_cand,

@@ -107,19 +107,85 @@ __global__ __launch_bounds__(256, (NC <= 2 ? 4 : 2)) void k2_eval_kernel(
     }
 }
 
+// ---- one-launch evaluation for few candidates: Z tiles over (chunk, candidate group) ------------
+template <int FL, bool F32>
+__global__ __launch_bounds__(256) void k2_eval_z_kernel(
+    ZeroView zv, int z_begin, int z_end, int zpc, const uint4* __restrict__ cand, int n_cand,
+    int n_groups, double* __restrict__ partial, int row_base, const Tab64* __restrict__ tab64,
+    const Tab32* __restrict__ tab32) {
+    __shared__ ZTileSmem<K2_Z_CZ> sm;
+    __shared__ Tab64 s64[F32 ? 1 : K2_TAB];
+    __shared__ Tab32 s32[F32 ? K2_TAB : 1];
+    for (int i = threadIdx.x; i < K2_TAB; i += K2_ZT) {
+        if constexpr (F32) s32[i] = tab32[i];
+        else s64[i] = tab64[i];
+    }
+    // ztile's first barrier orders the table stores before any lookup
+    const int L = xcd_linear_block();
+    const int chunk = L / n_groups, g = L - chunk * n_groups;
+    const int z0 = z_begin + chunk * zpc;
+    const int z1 = min(z_end, z0 + zpc);
+    const int c_first = g * K2_Z_CZ;
+    double* out = partial + ((size_t)(row_base + chunk) * n_groups + g) * 3 * K2_Z_CZ;
+    ztile<K2_Z_CZ, FL, F32>(zv, z0, z1, cand, c_first, min(K2_Z_CZ, n_cand - c_first), out, sm,
+                            s64, s32);
+}
+
+// one wavefront per candidate: lanes stride over the rows, then a fixed butterfly
+__global__ __launch_bounds__(256) void k2_reduce_z_kernel(
+    const double* __restrict__ partial, int n_rows, int n_groups, int n_cand, unsigned flags,
+    double* __restrict__ K, double* __restrict__ dK, double* __restrict__ d2K) {
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= n_cand) return;   // whole wavefronts leave together
+    const int g = c / K2_Z_CZ, pos = c - g * K2_Z_CZ;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int r = lane; r < n_rows; r += 64) {
+        const double* row = partial + ((size_t)r * n_groups + g) * 3 * K2_Z_CZ;
+        s0 += row[pos];
+        s1 += row[K2_Z_CZ + pos];
+        s2 += row[2 * K2_Z_CZ + pos];
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        if (K) K[c] = s0;
+        if (dK) dK[c] = (flags & K2_D1) ? s1 : 0.0;
+        if (d2K) d2K[c] = (flags & K2_D2) ? s2 : 0.0;
+    }
+}
+
 // ---- deterministic split reduction ---------------------------------------------------------------
+// A workgroup covers 256/SG candidates; its SG thread rows sum the splits s = row (mod SG) in order
+// (consecutive candidates read consecutive doubles of a partial row), then row sums are added in
+// row order.  SG grows with the split count so that few candidates over many splits still spread.
+template <int SG>
 __global__ __launch_bounds__(256) void k2_reduce_kernel(
     const double* __restrict__ partial, int n_splits, int n_cand, int n_cand_pad, unsigned flags,
     double* __restrict__ K, double* __restrict__ dK, double* __restrict__ d2K) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_cand) return;
+    constexpr int CB = 256 / SG;
+    __shared__ double red[3][SG][CB];
+    const int cl = threadIdx.x % CB, sg = threadIdx.x / CB;
+    const int c = blockIdx.x * CB + cl;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int s = 0; s < n_splits; ++s) {
-        const double* row = partial + (size_t)s * 3 * n_cand_pad;
-        s0 += row[c];
-        if (flags & K2_D1) s1 += row[n_cand_pad + c];
-        if (flags & K2_D2) s2 += row[2 * n_cand_pad + c];
+    if (c < n_cand) {
+        for (int s = sg; s < n_splits; s += SG) {
+            const double* row = partial + (size_t)s * 3 * n_cand_pad;
+            s0 += row[c];
+            if (flags & K2_D1) s1 += row[n_cand_pad + c];
+            if (flags & K2_D2) s2 += row[2 * n_cand_pad + c];
+        }
     }
+    if constexpr (SG > 1) {
+        red[0][sg][cl] = s0; red[1][sg][cl] = s1; red[2][sg][cl] = s2;
+        __syncthreads();
+        if (sg != 0) return;
+        for (int g = 1; g < SG; ++g) {
+            s0 += red[0][g][cl]; s1 += red[1][g][cl]; s2 += red[2][g][cl];
+        }
+    }
+    if (c >= n_cand) return;
     if (K) K[c] = s0;
     if (dK) dK[c] = (flags & K2_D1) ? s1 : 0.0;
     if (d2K) d2K[c] = (flags & K2_D2) ? s2 : 0.0;
@@ -257,17 +323,62 @@ hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad
     return hipGetLastError();
 }
 
+template <int FL, bool F32>
+static hipError_t launch_eval_z_t(const LaunchArgs& a, int n_groups, int n_chunks, int zpc,
+                                  hipStream_t st) {
+    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef};
+    hipLaunchKernelGGL((k2_eval_z_kernel<FL, F32>), dim3(n_groups * n_chunks), dim3(K2_ZT), 0, st,
+                       zv, a.z_begin, a.z_end, zpc, a.cand, a.n_cand, n_groups, a.partial,
+                       a.row_base, a.tab64, a.tab32);
+    return hipGetLastError();
+}
+
+template <bool F32>
+static hipError_t launch_eval_z_f(const LaunchArgs& a, int n_groups, int n_chunks, int zpc,
+                                  hipStream_t st) {
+    switch (a.flags & (K2_D1 | K2_D2)) {
+        case 0: return launch_eval_z_t<0, F32>(a, n_groups, n_chunks, zpc, st);
+        case K2_D1: return launch_eval_z_t<K2_D1, F32>(a, n_groups, n_chunks, zpc, st);
+        case K2_D2: return launch_eval_z_t<K2_D2, F32>(a, n_groups, n_chunks, zpc, st);
+        default: return launch_eval_z_t<K2_D1 | K2_D2, F32>(a, n_groups, n_chunks, zpc, st);
+    }
+}
+
+hipError_t launch_eval_z(const LaunchArgs& a, bool f32, int n_groups, int n_chunks, int zpc,
+                         hipStream_t st) {
+    return f32 ? launch_eval_z_f<true>(a, n_groups, n_chunks, zpc, st)
+               : launch_eval_z_f<false>(a, n_groups, n_chunks, zpc, st);
+}
+
+hipError_t launch_reduce_z(const double* partial, int n_rows, int n_groups, int n_cand,
+                           unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
+    hipLaunchKernelGGL(k2_reduce_z_kernel, dim3((n_cand + 3) / 4), dim3(256), 0, st, partial,
+                       n_rows, n_groups, n_cand, flags, K, dK, d2K);
+    return hipGetLastError();
+}
+
 hipError_t launch_abs_blocks(const double2* a, int n_pad, double* blk, hipStream_t st) {
     hipLaunchKernelGGL(k2_abs_blocks_kernel, dim3(n_pad / K2_ABS_BLOCK), dim3(256), 0, st, a, blk);
     return hipGetLastError();
 }
 
+template <int SG>
+static hipError_t launch_reduce_t(const double* partial, int n_splits, int n_cand, int n_cand_pad,
+                                  unsigned flags, double* K, double* dK, double* d2K,
+                                  hipStream_t st) {
+    constexpr int CB = 256 / SG;
+    hipLaunchKernelGGL((k2_reduce_kernel<SG>), dim3((n_cand + CB - 1) / CB), dim3(256), 0, st,
+                       partial, n_splits, n_cand, n_cand_pad, flags, K, dK, d2K);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_cand_pad,
                          unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
-    const int nb = (n_cand + 255) / 256;
-    hipLaunchKernelGGL(k2_reduce_kernel, dim3(nb), dim3(256), 0, st, partial, n_splits, n_cand,
-                       n_cand_pad, flags, K, dK, d2K);
-    return hipGetLastError();
+    if (n_splits >= 256)
+        return launch_reduce_t<16>(partial, n_splits, n_cand, n_cand_pad, flags, K, dK, d2K, st);
+    if (n_splits >= 32)
+        return launch_reduce_t<4>(partial, n_splits, n_cand, n_cand_pad, flags, K, dK, d2K, st);
+    return launch_reduce_t<1>(partial, n_splits, n_cand, n_cand_pad, flags, K, dK, d2K, st);
 }
 
 hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand, int n_cand,

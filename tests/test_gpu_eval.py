@@ -175,3 +175,26 @@ def test_adaptive_mixed_head(mid_table, k):
     finally:
         t.set_tiling(mixed_head=0)
     assert t.get_tiling()["mixed_tail"] == pytest.approx(0.16)
+
+
+@pytest.mark.parametrize("n_cand", [1, 5, 8, 17, 64, 65, 200])
+def test_z_tile_path_matches_c_tiles(cfg1, n_cand):
+    """Few candidates run as Z tiles (lanes over zeros); forced C tiles must agree, and both
+    must match the oracle's golden values."""
+    tb, t, V, k = cfg1
+    g = golden("config1_oracle.npz")
+    Vs = V[:n_cand]
+    t.set_tiling(z_max_cand=1000)
+    try:
+        Kz, dKz, d2Kz = t.eval(Vs, k, flags=ALL)
+        Kzm, _, _ = t.eval(Vs, k, flags=ev.VALUE | ev.D1 | ev.MIXED)
+        t.set_tiling(z_max_cand=-1)
+        Kc, dKc, d2Kc = t.eval(Vs, k, flags=ALL)
+    finally:
+        t.set_tiling(z_max_cand=0)
+    s0, s1, s2 = _scales(tb, k)
+    assert np.max(np.abs(Kz - g["K"][:n_cand])) <= 1e-12
+    assert np.max(np.abs(Kz - Kc)) <= 1e-13
+    assert np.max(np.abs(dKz - dKc)) / s1 <= 1e-13
+    assert np.max(np.abs(d2Kz - d2Kc)) / s2 <= 1e-13
+    assert np.max(np.abs(Kzm - g["K"][:n_cand])) <= 1e-9
