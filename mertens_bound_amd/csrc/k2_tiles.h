@@ -272,7 +272,9 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
     const int lane = tid & 63, wid = tid >> 6;
     if (tid < CZ) sm.v[tid] = tid < c_count ? cand[c_first + tid] : make_uint4(0, 0, 0, 0);
     __syncthreads();
-    for (int cp = 0; cp < CZ; cp += K2_ZG) {
+    // passes with no live candidate are skipped (their outputs are written as zeros below)
+    const int cp_end = min(CZ, (c_count + K2_ZG - 1) / K2_ZG * K2_ZG);
+    for (int cp = 0; cp < cp_end; cp += K2_ZG) {
         Acc<F32> acc[K2_ZG];
         Limbs4 v[K2_ZG];
 #pragma unroll
@@ -320,7 +322,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
     __syncthreads();
     if (tid < 3 * CZ) {
         double s = 0.0;
-        if (tid < NQ * CZ) {
+        if (tid < NQ * CZ && tid % CZ < cp_end) {
 #pragma unroll
             for (int w = 0; w < K2_WG / 64; ++w) s += sm.red[w][tid];
         }
