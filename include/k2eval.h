@@ -140,6 +140,9 @@ int k2_batch_destroy(k2_batch *b);
 int k2_polish_device(k2_table *t, uint32_t n_cand, uint64_t *V_dev, double k, uint32_t n_zeros,
                      int n_steps, double clamp, uint32_t flags, double *K_dev, double *dK_dev,
                      double *d2K_dev, int32_t *steps_dev, void *stream);
+/* Candidate evaluations (K, K', K'' passes x candidates) of the last k2_polish_device call on t:
+ * after the first step only the candidates still moving are evaluated again. */
+int k2_polish_stats(k2_table *t, int64_t *cand_evals);
 
 /* Enclosure of K2 (and K2' with K2_D1) over the first n_zeros zeros (device buffers): the exact
  * sum over the table's definitions (gamma_j = 2 pi G_j / 2^110, w_j as stored, k and v as given)

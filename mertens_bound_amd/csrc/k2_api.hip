@@ -126,6 +126,7 @@ extern "C" int k2_table_destroy(k2_table* t) {
     (void)hipFree(t->g); (void)hipFree(t->gamma); (void)hipFree(t->w);
     (void)hipFree(t->tab64); (void)hipFree(t->tab32);
     (void)hipFree(t->partial); (void)hipFree(t->host_io); (void)hipFree(t->polish_active);
+    (void)hipFree(t->polish_buf);
     (void)hipFree(t->screen_params);
     free_enclose(t);
     if (t->aux) (void)hipStreamSynchronize(t->aux);
@@ -501,28 +502,75 @@ extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, d
         return fail(K2_E_ARG, "k2_polish_device: null pointer");
     if (n_steps < 0 || !(clamp > 0.0) || clamp > 0.25)
         return fail(K2_E_ARG, "k2_polish_device: n_steps >= 0 and 0 < clamp <= 0.25 required");
+    t->polish_evals = 0;
     if (n_cand == 0) return K2_OK;
     hipStream_t st = (hipStream_t)stream;   // NULL is the default stream
     const unsigned fl = (flags & K2_MIXED) | K2_VALUE | K2_D1 | K2_D2;
+    // compaction buffers: idx (int), gathered points (uint4), their K, K', K'' (double)
+    const size_t off_v = ((size_t)n_cand * 4 + 255) / 256 * 256;
+    const size_t off_k = off_v + (size_t)n_cand * 16;
+    int* idx_d = nullptr;
+    uint4* Vc = nullptr;
+    double* Kc = nullptr;
     {
         std::lock_guard<std::mutex> lk(t->mu);
         HIPCHK(hipSetDevice(t->device));
         int rc = ensure_bytes((void**)&t->polish_active, &t->polish_cap, (size_t)n_cand * 4);
+        if (!rc) rc = ensure_bytes(&t->polish_buf, &t->polish_buf_bytes,
+                                   off_k + (size_t)n_cand * 3 * sizeof(double));
         if (rc) return rc;
+        idx_d = (int*)t->polish_buf;
+        Vc = (uint4*)((char*)t->polish_buf + off_v);
+        Kc = (double*)((char*)t->polish_buf + off_k);
         std::vector<int> ones(n_cand, 1);
         HIPCHK(hipMemcpyAsync(t->polish_active, ones.data(), (size_t)n_cand * 4,
                               hipMemcpyHostToDevice, st));
         HIPCHK(hipMemsetAsync(steps_dev, 0, (size_t)n_cand * 4, st));
         HIPCHK(hipStreamSynchronize(st));
     }
+    std::vector<int> act(n_cand), idx;
+    int64_t evals = 0;
     for (int it = 0; it < n_steps; ++it) {
-        int rc = k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
-        if (rc) return rc;
+        int n_act = (int)n_cand;
+        if (it > 0) {
+            // only candidates still moving are evaluated again (the reference stops each peak
+            // once its step falls below 1e-13 or its K'' vanishes)
+            HIPCHK(hipMemcpyAsync(act.data(), t->polish_active, (size_t)n_cand * 4,
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            idx.clear();
+            for (uint32_t c = 0; c < n_cand; ++c)
+                if (act[c]) idx.push_back((int)c);
+            n_act = (int)idx.size();
+            if (n_act == 0) break;
+        }
+        if (n_act == (int)n_cand) {
+            int rc = k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipMemcpyAsync(idx_d, idx.data(), (size_t)n_act * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(launch_gather((const uint4*)V_dev, idx_d, n_act, Vc, st));
+            int rc = k2_eval_device(t, (uint32_t)n_act, (const uint64_t*)Vc, k, n_zeros, fl, Kc,
+                                    Kc + n_cand, Kc + 2 * (size_t)n_cand, st);
+            if (rc) return rc;
+            HIPCHK(launch_scatter3(idx_d, n_act, Kc, Kc + n_cand, Kc + 2 * (size_t)n_cand, K_dev,
+                                   dK_dev, d2K_dev, st));
+            HIPCHK(hipStreamSynchronize(st));   // idx is rebuilt on the host next step
+        }
+        evals += n_act;
         std::lock_guard<std::mutex> lk(t->mu);
         HIPCHK(launch_newton((uint4*)V_dev, dK_dev, d2K_dev, (int)n_cand, clamp, t->polish_active,
                              steps_dev, st));
     }
+    evals += n_cand;
+    t->polish_evals = evals;
     return k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
+}
+
+extern "C" int k2_polish_stats(k2_table* t, int64_t* cand_evals) {
+    if (!t || !cand_evals) return fail(K2_E_ARG, "k2_polish_stats: null pointer");
+    *cand_evals = t->polish_evals;
+    return K2_OK;
 }
 
 extern "C" int k2_mixed_head(k2_table* t, double k, uint32_t n_zeros, uint32_t* head) {

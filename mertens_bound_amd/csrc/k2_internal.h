@@ -94,6 +94,9 @@ struct k2_table {
     void* host_io = nullptr;   // device staging for host-buffer entry points
     size_t host_io_bytes = 0;
     int* polish_active = nullptr;
+    void* polish_buf = nullptr;          // compaction: indices, gathered points, their sums
+    size_t polish_buf_bytes = 0;
+    int64_t polish_evals = 0;            // candidate evaluations of the last polish
     int32_t* screen_params = nullptr;   // stage zero counts and thresholds of k2_screen_device
     size_t polish_cap = 0;
     // enclosure mode (k2_enclose_device), allocated on first use: weights with a stated error
@@ -128,6 +131,10 @@ void free_enclose(k2_table* t);
 hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,
                           WeightArrays& out, hipStream_t st);
 hipError_t launch_abs_blocks(const double2* a, int n_pad, double* blk, hipStream_t st);
+hipError_t launch_gather(const uint4* V, const int* idx, int n, uint4* Vc, hipStream_t st);
+hipError_t launch_scatter3(const int* idx, int n, const double* Kc, const double* dKc,
+                           const double* d2Kc, double* K, double* dK, double* d2K,
+                           hipStream_t st);
 // Z-tile one-launch path: workgroup (chunk, group) sums zeros [z_begin + chunk*zpc, ...) for
 // candidates [group*K2_Z_CZ, ...) into partial row (row_base + chunk) * n_groups + group
 hipError_t launch_eval_z(const LaunchArgs& a, bool f32, int n_groups, int n_chunks, int zpc,

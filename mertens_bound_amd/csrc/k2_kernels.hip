@@ -251,6 +251,26 @@ __global__ __launch_bounds__(256) void k2_newton_kernel(
     if (fabs(step) < 1e-13) active[c] = 0;
 }
 
+// ---- polish compaction: the candidates still moving, gathered and their sums scattered back -----
+__global__ __launch_bounds__(256) void k2_gather_kernel(const uint4* __restrict__ V,
+                                                        const int* __restrict__ idx, int n,
+                                                        uint4* __restrict__ Vc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Vc[i] = V[idx[i]];
+}
+
+__global__ __launch_bounds__(256) void k2_scatter3_kernel(
+    const int* __restrict__ idx, int n, const double* __restrict__ Kc,
+    const double* __restrict__ dKc, const double* __restrict__ d2Kc, double* __restrict__ K,
+    double* __restrict__ dK, double* __restrict__ d2K) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = idx[i];
+    K[c] = Kc[i];
+    dK[c] = dKc[i];
+    d2K[c] = d2Kc[i];
+}
+
 // ---- reference-format compute_peak_AB (search/path1_eval_fast.cpp:15-53) ---------------------
 // one lane per (peak, zero); ph = xh*ghi + xl*glo - psi mod 2^128; th = top64 * 2pi/2^64 + g dy0
 __global__ __launch_bounds__(256) void k2_ref_peak_ab_kernel(
@@ -354,6 +374,19 @@ hipError_t launch_reduce_z(const double* partial, int n_rows, int n_groups, int 
                            unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
     hipLaunchKernelGGL(k2_reduce_z_kernel, dim3((n_cand + 3) / 4), dim3(256), 0, st, partial,
                        n_rows, n_groups, n_cand, flags, K, dK, d2K);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const uint4* V, const int* idx, int n, uint4* Vc, hipStream_t st) {
+    hipLaunchKernelGGL(k2_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, st, V, idx, n, Vc);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter3(const int* idx, int n, const double* Kc, const double* dKc,
+                           const double* d2Kc, double* K, double* dK, double* d2K,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k2_scatter3_kernel, dim3((n + 255) / 256), dim3(256), 0, st, idx, n, Kc,
+                       dKc, d2Kc, K, dK, d2K);
     return hipGetLastError();
 }
 

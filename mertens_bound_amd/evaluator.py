@@ -88,6 +88,7 @@ def _declare(L):
     L.k2_enclose_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                     ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
     L.k2_exp_neg_host.argtypes = [ctypes.c_uint32, _f64p, _f64p]
+    L.k2_polish_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
     L.k2_mixed_head.argtypes = [_vp, ctypes.c_double, ctypes.c_uint32,
                                 ctypes.POINTER(ctypes.c_uint32)]
     L.k2_polish_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
@@ -256,6 +257,12 @@ class K2Table:
             int(flags), K.data_ptr(), dK.data_ptr(), d2K.data_ptr(), steps.data_ptr(), st),
             "k2_polish_device")
         return K, dK, d2K, steps
+
+    def polish_stats(self):
+        """Candidate evaluations of the last polish_device call (converged candidates drop out)."""
+        n = ctypes.c_int64()
+        _check(load_library().k2_polish_stats(self._h, ctypes.byref(n)), "k2_polish_stats")
+        return n.value
 
     def screen_device(self, V, k, shift_step=0.0014, n_shift=3,
                       stages=((64, 0.78), (128, 0.855), (256, 0.892)), stream=None):

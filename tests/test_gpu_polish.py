@@ -27,3 +27,27 @@ def test_polish_matches_oracle():
     dv = np.array([abs(a - b) for a, b in zip(synth.v_to_int(Vsel), synth.v_to_int(g["V"]))])
     assert np.all(dv <= 2 ** 64 * 1e-9)
     assert np.array_equal(steps.cpu().numpy(), g["steps"])
+
+
+def test_polish_compaction_counts_and_matches_full_passes():
+    """After each step only the candidates still moving are evaluated again; the result must not
+    depend on that (a per-candidate sum does not depend on which other candidates share a pass
+    beyond rounding), and the evaluation count must reflect the early stops."""
+    g = golden("polish_small.npz")
+    tb = synth.make_zero_table(3000, 11)
+    t = ev.K2Table(tb)
+    k = float(g["k"])
+    V0 = g["V0"]
+    n = V0.shape[0]
+    Vt = ev.v_tensor(V0)
+    K, dK, d2K, steps = t.polish_device(Vt, k, n_steps=8, clamp=1e-4)
+    evals = t.polish_stats()
+    st = steps.cpu().numpy()
+    print(f"polish: {n} candidates, mean steps {st.mean():.2f}, candidate evaluations {evals} "
+          f"(full passes would be {n * 9})")
+    assert 2 * n <= evals <= 9 * n
+    if st.max() < 8:
+        assert evals < 9 * n
+    # every candidate is evaluated once per step it took, and once more when it stopped because
+    # its step fell below 1e-13, plus the first and the final pass
+    assert evals >= n + int(st.sum())
