@@ -87,9 +87,11 @@ def _second_derivative_graph(delta, table, V0, k, n_zeros, mixed):
 
 
 def newton_polish(table: ev.K2Table, V0: torch.Tensor, k: float, n_steps: int = 8,
-                  clamp: float = 1e-4, n_zeros=None, mixed: bool = False):
-    """FP64 Newton on K2' = 0 driven through autograd.  Returns (V, K, K', steps) with the
-    start point kept where it was larger in |K2|."""
+                  clamp: float = 1e-4, n_zeros=None, mixed: bool = False, tol: float = 1e-13,
+                  keep_start: bool = True):
+    """FP64 Newton on K2' = 0 driven through autograd: step = K'/K'' clamped to +-clamp while
+    |K''| > 1e-12, stopping once |step| < tol (path1_eval_fast.cpp:105-129).  Returns
+    (V, K, K', steps); with keep_start the start point is kept where it was larger in |K2|."""
     n = V0.shape[0]
     delta = torch.zeros(n, dtype=torch.float64, device=V0.device)
     active = torch.ones(n, dtype=torch.bool, device=V0.device)
@@ -107,11 +109,13 @@ def newton_polish(table: ev.K2Table, V0: torch.Tensor, k: float, n_steps: int = 
         step = torch.where(ok, step, torch.zeros_like(step))
         delta = delta - step
         steps += ok.to(torch.int32)
-        active = ok & (step.abs() >= 1e-13)
+        active = ok & (step.abs() >= tol)
         if not bool(active.any()):
             K, g, h = _second_derivative_graph(delta, table, V0, k, n_zeros, mixed)
             break
     V = offset_points(V0, delta)
+    if not keep_start:
+        return V, K, g, steps
     keep = K0.abs() > K.abs()
     V = torch.where(keep[:, None], V0, V)
     Kf = torch.where(keep, K0, K)

@@ -12,9 +12,16 @@
 * :func:`cascade` -- the hierarchical grid polish of search/gpu_mertens_stream.cu:381-505: stages
   of (zeros, half-steps, step, threshold, centred on the previous best?); a candidate stops at the
   first stage whose best |h| is below its threshold.
+* :func:`polish_single_peak` -- search/path1_kernel_opt.py:94-147: a grid on all zeros, Newton
+  from 0 and from the best grid point, the largest |h| kept.
+* :func:`fast_polish_batch` -- search/path1_kernel_opt.py:150-221: a 121-point grid (evaluated
+  exactly here instead of through degree-16 Taylor moments), then 4 Newton steps for the 25
+  largest peaks.
+* :func:`scan` -- search/polish.py:51-73: the best |h| on the grid y0 + m step, |m| <= half/step.
 """
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import autograd as ag
@@ -113,3 +120,90 @@ def cascade(table: ev.K2Table, V: torch.Tensor, k: float, stages=REFERENCE_CASCA
         passed[alive[ok]] = s + 1
         alive = alive[ok]
     return best_h.abs(), best_h, best_d, passed
+
+
+def polish_single_peak(table: ev.K2Table, V: torch.Tensor, k: float, n_zeros=None,
+                       grid_hw: float = 4e-5, grid_steps: int = 40, newton_steps: int = 8,
+                       clamp: float = 2e-5, tol: float = 1e-14):
+    """search/path1_kernel_opt.py:94-147 on exact points, for a batch of peaks: h0 over the first
+    n_zeros zeros; a (2 grid_steps + 1)-point grid of +-grid_hw over the same zeros (first
+    maximum of |h|); Newton (clamp, stop once |step| < tol or |K''| <= 1e-12) from offset 0 and
+    from the best grid point; a Newton end point replaces the best so far (h0 first) only when
+    strictly larger in |h|.  Returns (h0, h_pol, V_pol, ddy)."""
+    n = V.shape[0]
+    nz = table.n if n_zeros is None else int(n_zeros)
+    h0, _, _ = table.eval_device(V, k, n_zeros=nz, flags=ev.VALUE)
+    h0 = h0.clone()
+    starts = [torch.zeros(n, dtype=torch.float64, device=V.device)]
+    if grid_steps > 0:
+        dd = torch.from_numpy(np.linspace(-grid_hw, grid_hw, 2 * grid_steps + 1)).to(V.device)
+        hs, _, _ = table.eval_device(shifted(V, dd), k, n_zeros=nz, flags=ev.VALUE)
+        j = torch.argmax(hs.view(n, -1).abs(), dim=1)
+        starts.append(dd[j])
+    best_h, best_V = h0, V
+    for d0 in starts:
+        Vs = ag.offset_points(V, d0)
+        Vf, Kf, _, _ = ag.newton_polish(table, Vs, k, n_steps=newton_steps, clamp=clamp,
+                                        n_zeros=nz, tol=tol, keep_start=False)
+        better = Kf.abs() > best_h.abs()
+        best_h = torch.where(better, Kf, best_h)
+        best_V = torch.where(better[:, None], Vf, best_V)
+    return h0, best_h, best_V, _delta(V, best_V)
+
+
+def fast_polish_batch(table: ev.K2Table, V: torch.Tensor, k: float, n_zeros=None,
+                      grid_hw: float = 3e-5, grid_points: int = 121, top: int = 25,
+                      newton_steps: int = 4, clamp: float = 1e-5, tol: float = 1e-13):
+    """search/path1_kernel_opt.py:150-221: h_unpol at offset 0; the grid value of largest |h| per
+    peak (every grid point evaluated exactly); for the `top` peaks of largest grid |h|, Newton
+    from their grid point, kept when larger in |h| than h_unpol, else h_unpol at offset 0.
+    Returns (h_unpol, h_pol, V_pol, ddy)."""
+    n = V.shape[0]
+    nz = table.n if n_zeros is None else int(n_zeros)
+    h_unpol, _, _ = table.eval_device(V, k, n_zeros=nz, flags=ev.VALUE)
+    h_unpol = h_unpol.clone()
+    grid = torch.from_numpy(np.linspace(-grid_hw, grid_hw, grid_points)).to(V.device)
+    vals, _, _ = table.eval_device(shifted(V, grid), k, n_zeros=nz, flags=ev.VALUE)
+    vals = vals.view(n, grid_points)
+    j = torch.argmax(vals.abs(), dim=1)
+    h_pol = vals.gather(1, j[:, None]).flatten()
+    ddy = grid[j].clone()
+    V_pol = ag.offset_points(V, ddy)
+    sel = torch.argsort(-h_pol.abs(), stable=True)[:top]
+    if sel.numel():
+        Vf, Kf, _, _ = ag.newton_polish(table, V_pol[sel], k, n_steps=newton_steps, clamp=clamp,
+                                        n_zeros=nz, tol=tol, keep_start=False)
+        take = Kf.abs() > h_unpol[sel].abs()
+        h_pol[sel] = torch.where(take, Kf, h_unpol[sel])
+        V_pol[sel] = torch.where(take[:, None], Vf, V[sel])
+        ddy = _delta(V, V_pol)
+    return h_unpol, h_pol, V_pol, ddy
+
+
+def scan(table: ev.K2Table, V0: torch.Tensor, k: float, half_width: float = 0.25,
+         step: float = 2e-5, n_zeros=None, chunk: int = 4000):
+    """search/polish.py:51-73 for a batch of seeds: K at V0 + rint(m step 2^64) / 2^64 for
+    |m| <= round(half_width / step), scanned in chunks of m; the first point of largest |h|.
+    Returns (best h, best V, best m) per seed."""
+    n = V0.shape[0]
+    nz = table.n if n_zeros is None else int(n_zeros)
+    m_half = int(round(half_width / step))
+    if m_half * step >= 0.5:
+        raise ValueError("scan: the grid must stay within +-0.5 of the seed")
+    best_h = torch.zeros(n, dtype=torch.float64, device=V0.device)
+    best_m = torch.zeros(n, dtype=torch.int64, device=V0.device)
+    have = torch.zeros(n, dtype=torch.bool, device=V0.device)
+    for m0 in range(-m_half, m_half + 1, chunk):
+        m1 = min(m0 + chunk, m_half + 1)
+        m = torch.arange(m0, m1, dtype=torch.int64, device=V0.device)
+        h, _, _ = table.eval_device(shifted(V0, m.to(torch.float64) * step), k, n_zeros=nz,
+                                    flags=ev.VALUE)
+        h = h.view(n, m1 - m0)
+        j = torch.argmax(h.abs(), dim=1)
+        v = h.gather(1, j[:, None]).flatten()
+        better = ~have | (v.abs() > best_h.abs())
+        best_h = torch.where(better, v, best_h)
+        best_m = torch.where(better, m[j], best_m)
+        have = torch.ones_like(have)
+    best_V = ag.offset_points(V0, best_m.to(torch.float64) * step)
+    return best_h, best_V, best_m

@@ -95,3 +95,50 @@ def test_cascade_stages(setup, oracle_lib):
         want = oracle_lib.eval_f64(tb, _shift_int(V[i:i + 1], 1, bd[i]), k, flags=1,
                                    n_zeros=nz)[0][0]
         assert abs(want - bh[i]) <= 1e-12
+
+
+def _near_peaks(t, tb, k, n, seed, torch):
+    """Candidates a few 1e-5 from local maxima of |K| (found by a fine GPU scan)."""
+    V = synth.make_candidates(n, 3.4e18, 3.6e18, seed)
+    hb, Vb, mb = pipeline.scan(t, ev.v_tensor(V), k, half_width=2e-3, step=2e-5, n_zeros=512)
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-3e-5, 3e-5, n)
+    return ev.v_from_tensor(pipeline.ag.offset_points(Vb, torch.tensor(d, device=Vb.device)))
+
+
+def test_polish_single_peak_vs_restatement(setup, oracle_lib):
+    from oracle import pipelines as P
+    tb, t, torch = setup
+    k = 3e-9
+    V = _near_peaks(t, tb, k, 12, 31, torch)
+    h0, h, Vp, d = pipeline.polish_single_peak(t, ev.v_tensor(V), k)
+    h0r, hr, dr = P.polish_single_peak(tb, V, k)
+    np.testing.assert_allclose(h0.cpu().numpy(), h0r, atol=1e-12)
+    np.testing.assert_allclose(h.cpu().numpy(), hr, atol=1e-11)
+    np.testing.assert_allclose(d.cpu().numpy(), dr, atol=1e-10)
+
+
+def test_fast_polish_batch_vs_restatement(setup, oracle_lib):
+    from oracle import pipelines as P
+    tb, t, torch = setup
+    k = 3e-9
+    V = _near_peaks(t, tb, k, 40, 32, torch)
+    hu, hp, Vp, d = pipeline.fast_polish_batch(t, ev.v_tensor(V), k, top=10)
+    hur, hpr, dr = P.fast_polish_batch(tb, V, k, top=10)
+    np.testing.assert_allclose(hu.cpu().numpy(), hur, atol=1e-12)
+    np.testing.assert_allclose(hp.cpu().numpy(), hpr, atol=1e-11)
+    np.testing.assert_allclose(d.cpu().numpy(), dr, atol=1e-10)
+
+
+def test_scan_vs_restatement(setup, oracle_lib):
+    from oracle import pipelines as P
+    tb, t, torch = setup
+    k = 3e-9
+    V = synth.make_candidates(6, 3.4e18, 3.6e18, 33)
+    hb, Vb, mb = pipeline.scan(t, ev.v_tensor(V), k, half_width=1e-3, step=2e-5, n_zeros=2000,
+                               chunk=37)
+    hr, mr = P.scan(tb, V, k, half_width=1e-3, step=2e-5, n_zeros=2000, chunk=37)
+    np.testing.assert_allclose(hb.cpu().numpy(), hr, atol=1e-12)
+    assert np.array_equal(mb.cpu().numpy(), mr)
+    want = P.offset(V, mr * 2e-5)
+    assert np.array_equal(ev.v_from_tensor(Vb), want)

@@ -197,3 +197,50 @@ def test_reference_format_matches_canonical(oracle_lib):
     K, _, _, _ = oracle_lib.eval_quad(tb, V, k)
     # dy0 < 2^-30 is applied in FP64 as g * dy0 (<= 1e-5 rad), psi carries 53 bits
     np.testing.assert_allclose(h0, K, rtol=0, atol=5e-12)
+
+
+def _one_zero_peak():
+    """One zero with gamma = 2 pi G / 2^110 ~ 1000.3 and w = 0.6 e^{0.4 i}: K2(v) =
+    2 e^{-k g^2} 0.6 cos(g v + 0.4), whose extrema sit exactly at g v + 0.4 = m pi."""
+    import mpmath
+    from mertens_bound_amd import synth
+    G = int(round(1000.3 / (2 * np.pi) * 2.0 ** 110))
+    with mpmath.workdps(50):
+        g = 2 * mpmath.pi * mpmath.mpf(G) / mpmath.mpf(2) ** 110
+        m = int(mpmath.nint((5 * g + mpmath.mpf("0.4")) / mpmath.pi))
+        vstar = (m * mpmath.pi - mpmath.mpf("0.4")) / g
+    w = np.array([0.6 * np.exp(0.4j)])
+    tb = synth.ZeroTable(np.array([[G & (2 ** 64 - 1), G >> 64]], dtype=np.uint64),
+                         np.array([float(g)]), w)
+    return tb, vstar, float(g)
+
+
+def test_pipeline_restatements_reach_the_one_zero_peak(oracle_lib):
+    import mpmath
+    from oracle import pipelines as P
+    from mertens_bound_amd import synth
+    tb, vstar, g = _one_zero_peak()
+    k = 1e-9
+    amp = 2 * np.exp(-k * g * g) * 0.6
+    offs = np.array([3e-6, -7e-6, 1.2e-5])                 # start points near the peak
+    V0 = synth.v_from_int([int(mpmath.nint((vstar + mpmath.mpf(float(o))) * 2 ** 64))
+                           for o in offs])
+    want_d = np.array([float(vstar - mpmath.mpf(v) / mpmath.mpf(2) ** 64)
+                       for v in synth.v_to_int(V0)])
+    h0, h, d = P.polish_single_peak(tb, V0, k)
+    assert np.allclose(np.abs(h), amp, rtol=0, atol=1e-15)
+    assert np.max(np.abs(d - want_d)) < 1e-12
+    assert np.all(np.abs(h) >= np.abs(h0))
+    hu, hp, dp = P.fast_polish_batch(tb, V0, k)
+    assert np.allclose(np.abs(hp), amp, rtol=0, atol=1e-15)
+    assert np.max(np.abs(dp - want_d)) < 1e-12
+    # scan: the grid point nearest the peak (cos is even about it) is the first maximum
+    step = 1e-4
+    hb, mb = P.scan(tb, V0, k, half_width=2e-3, step=step)
+    assert np.array_equal(mb, np.rint(want_d / step).astype(np.int64))
+    assert np.all(np.abs(hb) <= amp + 1e-15)
+    # the two Newton variants differ in clamp and tolerance only; a start too far for the clamp
+    # budget (8 x 2e-5) stays short of the peak
+    far = synth.v_from_int([int(mpmath.nint((vstar + mpmath.mpf("4e-4")) * 2 ** 64))])
+    d_far, _ = P.newton(tb, far, k, 1, 8, 2e-5, 1e-14)
+    assert abs(d_far[0] + 8 * 2e-5) < 1e-15
