@@ -29,11 +29,11 @@ __device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
 }
 
 // a*b + c over 64 bits with the carry out of bit 64 added to k (v_mad_u64_u32's carry-out,
-// then v_addc_co_u32).  The s_nop gives the VCC write two wait states before it is read.
+// then v_addc_co_u32; a VALU carry-out feeds the next VALU's carry-in without wait states, as in
+// the compiler's own 64-bit add sequences).
 __device__ __forceinline__ uint64_t mad_carry(uint32_t a, uint32_t b, uint64_t c, uint32_t& k) {
     uint64_t r;
     asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
-        "s_nop 1\n\t"
         "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
         : "=&v"(r), "+v"(k)
         : "v"(a), "v"(b), "v"(c)
@@ -45,7 +45,6 @@ __device__ __forceinline__ uint64_t mad_carry(uint32_t a, uint32_t b, uint64_t c
 __device__ __forceinline__ uint64_t mad_carry0(uint32_t a, uint32_t b, uint64_t c, uint32_t& k) {
     uint64_t r;
     asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
-        "s_nop 1\n\t"
         "v_cndmask_b32_e64 %1, 0, 1, vcc"
         : "=&v"(r), "=v"(k)
         : "v"(a), "v"(b), "v"(c)
