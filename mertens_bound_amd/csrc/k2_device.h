@@ -176,14 +176,31 @@ struct Tab32 {
     float cc, sc;     // the same times 2 pi / 2^32
 };
 
+// the remainder polynomials' coefficients (x = u 2 pi / 2^64 folded in); a loop that calls
+// sincos_tab64 many times takes them from sincos_coef() once, so that they stay in VGPRs (an FMA
+// reads at most one scalar operand)
+struct SinCosCoef {
+    double s1, s2, c1, c2;
+};
+
+__device__ __forceinline__ SinCosCoef sincos_coef() {
+    SinCosCoef k{-1.9336107030552212e-38, 1.121655105290912e-76, -5.800832109165663e-38,
+                 5.608275526454561e-76};
+    asm volatile("" : "+v"(k.s1), "+v"(k.s2), "+v"(k.c1), "+v"(k.c2));
+    return k;
+}
+
 __device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const Tab64* tab,
-                                             double& S, double& C) {
+                                             double& S, double& C,
+                                             const SinCosCoef& kc = SinCosCoef{
+                                                 -1.9336107030552212e-38, 1.121655105290912e-76,
+                                                 -5.800832109165663e-38, 5.608275526454561e-76}) {
     const uint32_t k = (whi + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT;   // nearest point
     const int32_t rhi = ((int32_t)(whi << (32 - K2_TAB_SHIFT))) >> (32 - K2_TAB_SHIFT);
     const double u = fma((double)rhi, 4294967296.0, (double)wlo);    // |u| <= 2^54
     const double z = u * u;
-    const double sp = fma(u * z, fma(z, 1.121655105290912e-76, -1.9336107030552212e-38), u);
-    const double cp = fma(z, fma(z, 5.608275526454561e-76, -5.800832109165663e-38), 1.0);
+    const double sp = fma(u * z, fma(z, kc.s2, kc.s1), u);
+    const double cp = fma(z, fma(z, kc.c2, kc.c1), 1.0);
     const Tab64 t = tab[k];
     C = fma(t.c, cp, -t.sc * sp);
     S = fma(t.s, cp, t.cc * sp);

@@ -162,6 +162,7 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
     };
     if (ntiles > 0) { fetch(0); stage(0); }
     __syncthreads();
+    const SinCosCoef kc = sincos_coef();
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
         if (t + 1 < ntiles) fetch(t + 1);
@@ -177,7 +178,7 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
             for (int i = 0; i < NC; ++i) {
                 const uint64_t W = phase_word(g, v[i]);
                 double S, C;
-                sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab, S, C);
+                sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab, S, C, kc);
                 accumulate<FL>(acc[i], S, C, aa, bb, ee);
             }
         }
@@ -274,6 +275,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
     __syncthreads();
     // passes with no live candidate are skipped (their outputs are written as zeros below)
     const int cp_end = min(CZ, (c_count + K2_ZG - 1) / K2_ZG * K2_ZG);
+    const SinCosCoef kc = sincos_coef();
     for (int cp = 0; cp < cp_end; cp += K2_ZG) {
         Acc<F32> acc[K2_ZG];
         Limbs4 v[K2_ZG];
@@ -305,7 +307,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                 for (int c = 0; c < K2_ZG; ++c) {
                     const uint64_t W = phase_word(g, v[c]);
                     double S, C;
-                    sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab64, S, C);
+                    sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab64, S, C, kc);
                     accumulate<FL>(acc[c], S, C, a, b, e);
                 }
             }
