@@ -33,7 +33,8 @@ WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
         "mixed": {"int32_mad": 15, "trans": 2, "fp32_fma": 4, "slots": 23},
         "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48}}
 # Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
-# published 78.6 TFLOP/s FP64 vector peak counted per FMA (2 flops).
+# published 78.6 TFLOP/s FP64 peak counted per FMA (2 flops).  The roofline reports the work in
+# FP64-FMA-equivalent flops (2 per slot) against that published peak.
 VALU_PEAK_TOPS = 39.3
 FP64_PEAK_TFLOPS = 78.6
 
@@ -127,9 +128,10 @@ def traffic_summary(workload):
 def roofline(form, evals, kern_ms, alg_bytes, workload):
     w = WORK[form]
     rate = evals / (kern_ms * 1e-3)
-    achieved = rate * w["slots"] / 1e12
-    r = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-         "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic_summary(workload),
+    achieved = rate * 2 * w["slots"] / 1e12
+    r = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_summary(workload),
+         "achieved_valu_slots_tops": rate * w["slots"] / 1e12, "valu_slot_peak_tops": VALU_PEAK_TOPS,
          "kernel_ms": kern_ms, "kernel_evals_per_s": rate, "work_per_eval": w,
          "int32_mad_tops": rate * w["int32_mad"] / 1e12,
          "algorithmic_bytes": alg_bytes,
