@@ -67,6 +67,7 @@ typedef struct k2_tiling {
     double mixed_tail;     /* 0 = 0.16 (FP32 tail error about 7e-10 per unit of magnitude) */
     int32_t z_max_cand;    /* candidate counts up to this run as Z tiles (lanes over zeros, 8
                               candidates per workgroup); 0 = library default, -1 = never */
+    int32_t wg_threads;    /* C-tile workgroup size: 256, or 512 (nc 1 or 2 only); 0 = 256 */
 } k2_tiling;
 
 const char *k2_last_error(void);

@@ -1,8 +1,9 @@
 """Tile autotuner for the evaluation launches.
 
 Tunable shapes:
-  one-launch evaluation  nc (candidates per lane; 256*nc candidates per workgroup) x
-                         target_blocks (workgroups the zero splits aim at, i.e. zeros per split)
+  one-launch evaluation  nc (candidates per lane) x wg_threads (256 or 512 lanes per workgroup;
+                         wg_threads*nc candidates per C tile) x target_blocks (workgroups the
+                         zero splits aim at, i.e. zeros per split)
   batched plan           nc x cz (candidates per Z tile) x chunk (zeros per work item) x
                          wg_per_cu (resident workgroups of the persistent grid)
 
@@ -25,7 +26,8 @@ import time
 PKG = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(PKG, "tuned_tiles.json")
 
-EVAL_GRID = {"nc": [1, 2, 4, 8], "target_blocks": [768, 1536, 2048, 3072, 4096, 6144]}
+EVAL_GRID = {"nc": [1, 2, 4], "target_blocks": [1536, 3072, 6144, 12288],
+             "wg_threads": [256, 512]}
 BATCH_GRID = {"nc": [2, 4], "cz": [8, 16], "chunk": [4096, 8192, 16384], "wg_per_cu": [0, 2]}
 
 _cache = None
@@ -69,7 +71,8 @@ def apply_tuned(table, n_cand: int, n_zeros: int, form: str = "plain") -> bool:
     ti = lookup_eval(n_cand, n_zeros, form)
     if ti is None:
         return False
-    table.set_tiling(nc=ti["nc"], target_blocks=ti["target_blocks"])
+    table.set_tiling(nc=ti["nc"], target_blocks=ti["target_blocks"],
+                     wg_threads=ti.get("wg_threads", 256))
     return True
 
 
@@ -100,11 +103,15 @@ def tune_eval(table, V, k, form, grid=EVAL_GRID, log=print):
     n = V.shape[0]
     out = tuple(torch.empty(n, dtype=torch.float64, device=V.device) for _ in range(3))
     results = []
-    for nc, tb in itertools.product(grid["nc"], grid["target_blocks"]):
-        table.set_tiling(nc=nc, target_blocks=tb)
+    for nc, tb, wg in itertools.product(grid["nc"], grid["target_blocks"],
+                                        grid.get("wg_threads", [256])):
+        if wg == 512 and nc > 2:
+            continue
+        table.set_tiling(nc=nc, target_blocks=tb, wg_threads=wg)
         ms = _time(lambda: table.eval_device(V, k, flags=flags, out=out), torch)
-        results.append({"tiling": {"nc": nc, "target_blocks": tb}, "ms": ms})
-        log(f"  eval {form} n={n} nc={nc} target_blocks={tb}: {ms:.3f} ms")
+        results.append({"tiling": {"nc": nc, "target_blocks": tb, "wg_threads": wg}, "ms": ms})
+        log(f"  eval {form} n={n} nc={nc} wg={wg} target_blocks={tb}: {ms:.3f} ms")
+    table.set_tiling(nc=2, target_blocks=3072, wg_threads=256)
     results.sort(key=lambda r: r["ms"])
     return results
 
@@ -145,7 +152,7 @@ def main():
     eg = dict(EVAL_GRID)
     bg = dict(BATCH_GRID)
     if args.quick:
-        eg = {"nc": [2, 4], "target_blocks": [1536, 2048, 3072]}
+        eg = {"nc": [2], "target_blocks": [3072, 6144, 12288], "wg_threads": [256, 512]}
         bg = {"nc": [4], "cz": [8, 16], "chunk": [4096, 8192], "wg_per_cu": [0]}
     torch.cuda.set_device(0)
     c2 = synth.CONFIG2

@@ -148,6 +148,9 @@ extern "C" int k2_set_tiling(k2_table* t, const k2_tiling* ti) {
     if (v.mixed_tail == 0.0) v.mixed_tail = K2_MIXED_TAIL;
     if (v.z_max_cand == 0) v.z_max_cand = K2_Z_MAX_CAND;
     if (v.z_max_cand < -1) return fail(K2_E_ARG, "k2_set_tiling: z_max_cand must be >= -1");
+    if (v.wg_threads == 0) v.wg_threads = 256;
+    if (v.wg_threads != 256 && !(v.wg_threads == 512 && v.nc <= 2))
+        return fail(K2_E_ARG, "k2_set_tiling: wg_threads must be 256, or 512 with nc 1 or 2");
     if (v.nc != 1 && v.nc != 2 && v.nc != 4 && v.nc != 8)
         return fail(K2_E_ARG, "k2_set_tiling: nc must be 1, 2, 4 or 8");
     if (v.target_blocks < 1 || v.mixed_head < 0 || !(v.mixed_tail > 0.0))
@@ -277,9 +280,9 @@ void k2impl::split_range(int nz, int n_ctiles, int target, int* n_splits, int* z
 
 static Plan make_plan(const k2_table* t, const WeightArrays* ws, int n_cand, int nz, bool mixed) {
     Plan p;
-    const int nc = t->tiling.nc;
-    p.n_ctiles = (n_cand + 256 * nc - 1) / (256 * nc);
-    p.n_cand_pad = p.n_ctiles * 256 * nc;
+    const int cw = t->tiling.wg_threads * t->tiling.nc;   // candidates per C tile
+    p.n_ctiles = (n_cand + cw - 1) / cw;
+    p.n_cand_pad = p.n_ctiles * cw;
     p.head_end = mixed ? (int)mixed_head_zeros(ws->abs_blocks, (uint32_t)nz, t->tiling.mixed_head,
                                                t->tiling.mixed_tail)
                        : nz;
@@ -378,6 +381,7 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
     if (rc) return rc;
     LaunchArgs a;
     a.nc = t->tiling.nc;
+    a.nt = t->tiling.wg_threads;
     a.flags = flags;
     a.zg = t->g;
     a.za = ws->a; a.zb = ws->b; a.ze = ws->e;

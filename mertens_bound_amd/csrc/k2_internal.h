@@ -34,6 +34,7 @@ struct WeightArrays {
 
 struct LaunchArgs {
     int nc = 4;
+    int nt = 256;                          // threads per workgroup of the C-tile kernel
     unsigned flags = 0;
     int n_blocks = 0;
     const uint4* zg = nullptr;
@@ -110,7 +111,7 @@ struct k2_table {
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;             // second stream for the FP64 head of the mixed form
     hipEvent_t fork = nullptr, join = nullptr;
-    k2_tiling tiling{2, 3072, 0, K2_MIXED_TAIL, K2_Z_MAX_CAND};
+    k2_tiling tiling{2, 3072, 0, K2_MIXED_TAIL, K2_Z_MAX_CAND, 256};
     bool profile = false;
     std::vector<ProfPair> prof_pending;
     std::vector<hipEvent_t> ev_pool;

@@ -68,9 +68,10 @@ __global__ __launch_bounds__(256) void k2_abs_blocks_kernel(const double2* __res
 // Split s covers zeros [z_begin + s*zps, ...) up to z_end and writes partial row row_base + s.
 // F32 selects the precision form (exact top-32-bit phase, FP32 sin/cos and weights, compensated
 // FP32 sums); the mixed form launches an FP64 head and an FP32 tail side by side.
-// NC <= 2 keeps to 128 VGPRs so that four workgroups (one wave per SIMD each) share a CU
-template <int NC, int FL, bool F32>
-__global__ __launch_bounds__(256, (NC <= 2 ? 4 : 2)) void k2_eval_kernel(
+// NC <= 2 keeps to 128 VGPRs so that four workgroups share a CU; NT = 512 puts twice the waves
+// on one staged tile and one sin/cos table (LDS is what limits the workgroups per CU)
+template <int NC, int FL, bool F32, int NT>
+__global__ __launch_bounds__(NT, (NC <= 2 ? 4 : 2)) void k2_eval_kernel(
     ZeroView zv, int z_begin, int z_end, int zps, const uint4* __restrict__ cand, int n_cand,
     int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
     const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(256, (NC <= 2 ? 4 : 2)) void k2_eval_kernel(
     const int z0 = z_begin + split * zps;
     const int z1 = min(z_end, z0 + zps);
     const int tid = threadIdx.x;
-    for (int i = tid; i < K2_TAB; i += K2_ZT) {
+    for (int i = tid; i < K2_TAB; i += NT) {
         if constexpr (F32) s_tab[i] = tab32[i];
         else s_tab[i] = tab64[i];
     }
@@ -89,18 +90,18 @@ __global__ __launch_bounds__(256, (NC <= 2 ? 4 : 2)) void k2_eval_kernel(
     Acc<F32> acc[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+        const int c = ct * (NT * NC) + i * NT + tid;
         uint4 cv = make_uint4(0, 0, 0, 0);
         if (c < n_cand) cv = cand[c];
         v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
         acc[i].zero();
     }
-    if constexpr (F32) ctile_f32<NC, FL>(zv, z0, z1, v, acc, sm, s_tab);
-    else ctile_f64<NC, FL>(zv, z0, z1, v, acc, sm, s_tab);
+    if constexpr (F32) ctile_f32<NC, FL, NT>(zv, z0, z1, v, acc, sm, s_tab);
+    else ctile_f64<NC, FL, Acc<false>, NT>(zv, z0, z1, v, acc, sm, s_tab);
     double* prow = partial + (size_t)(row_base + split) * 3 * n_cand_pad;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+        const int c = ct * (NT * NC) + i * NT + tid;
         prow[c] = acc[i].get(0);
         if (FL & K2_D1) prow[n_cand_pad + c] = acc[i].get(1);
         if (FL & K2_D2) prow[2 * n_cand_pad + c] = acc[i].get(2);
@@ -304,7 +305,16 @@ namespace k2impl {
 template <int NC, int FL, bool F32>
 static hipError_t launch_eval_t(const LaunchArgs& a, hipStream_t st) {
     const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef};
-    hipLaunchKernelGGL((k2_eval_kernel<NC, FL, F32>), dim3(a.n_blocks), dim3(K2_ZT), 0, st, zv,
+    if constexpr (NC <= 2) {
+        if (a.nt == 512) {
+            hipLaunchKernelGGL((k2_eval_kernel<NC, FL, F32, 512>), dim3(a.n_blocks), dim3(512), 0,
+                               st, zv, a.z_begin, a.z_end, a.zps, a.cand, a.n_cand, a.n_ctiles,
+                               a.partial, a.row_base, a.n_cand_pad, a.tab64, a.tab32);
+            return hipGetLastError();
+        }
+    }
+    if (a.nt != 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k2_eval_kernel<NC, FL, F32, 256>), dim3(a.n_blocks), dim3(256), 0, st, zv,
                        a.z_begin, a.z_end, a.zps, a.cand, a.n_cand, a.n_ctiles, a.partial,
                        a.row_base, a.n_cand_pad, a.tab64, a.tab32);
     return hipGetLastError();

@@ -135,30 +135,40 @@ constexpr size_t ctile_smem_bytes() {
 // Sums zeros [z0, z1) into acc for the NC candidates of this lane (ACC: Acc<false>, or AccDD
 // in the enclosure mode).  Enters and leaves with sm free (the last action is a barrier).
 // tab: the (cos, sin) table in LDS of the matching width.
-template <int NC, int FL, class ACC>
+// NT threads per workgroup (256, or 512: two threads per zero slot share the staging, the first
+// half g and a, the second half b and e, and twice the candidates read each staged record).
+template <int NC, int FL, class ACC, int NT = K2_WG>
 __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, const Limbs4 (&v)[NC],
                                           ACC (&acc)[NC], CTileSmem64<FL>& sm,
                                           const Tab64* tab) {
+    static_assert(NT == K2_WG || NT == 2 * K2_WG, "256 or 512 threads");
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
     const int tid = threadIdx.x;
+    const int zl = tid & (K2_WG - 1);
+    const bool lo_half = NT == K2_WG || tid < K2_WG;   // stages g, a
+    const bool hi_half = NT == K2_WG || tid >= K2_WG;  // stages b, e
     const int ntiles = z1 > z0 ? (z1 - z0 + K2_WG - 1) / K2_WG : 0;
     uint4 rg = make_uint4(0, 0, 0, 0);
     double2 ra = make_double2(0.0, 0.0), rb = ra, re = ra;
     auto fetch = [&](int t) {
-        const int j = z0 + t * K2_WG + tid;
+        const int j = z0 + t * K2_WG + zl;
         if (j < z1) {
-            rg = zv.g[j]; ra = zv.a[j];
-            if (D1) rb = zv.b[j];
-            if (D2) re = zv.e[j];
+            if (lo_half) { rg = zv.g[j]; ra = zv.a[j]; }
+            if (hi_half) {
+                if (D1) rb = zv.b[j];
+                if (D2) re = zv.e[j];
+            }
         } else {
             rg = make_uint4(0, 0, 0, 0); ra = make_double2(0.0, 0.0); rb = ra; re = ra;
         }
     };
     auto stage = [&](int buf) {
-        sm.g[buf][tid] = rg; sm.a(buf, tid) = ra;
-        if (D1) sm.b(buf, tid) = rb;
-        if (D2) sm.e(buf, tid) = re;
+        if (lo_half) { sm.g[buf][zl] = rg; sm.a(buf, zl) = ra; }
+        if (hi_half) {
+            if (D1) sm.b(buf, zl) = rb;
+            if (D2) sm.e(buf, zl) = re;
+        }
     };
     if (ntiles > 0) { fetch(0); stage(0); }
     __syncthreads();
@@ -187,34 +197,43 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
     }
 }
 
-template <int NC, int FL>
+template <int NC, int FL, int NT = K2_WG>
 __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, const Limbs4 (&v)[NC],
                                           Acc<true> (&acc)[NC], CTileSmem32<FL>& sm,
                                           const Tab32* tab) {
+    static_assert(NT == K2_WG || NT == 2 * K2_WG, "256 or 512 threads");
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
     const int tid = threadIdx.x;
+    const int zl = tid & (K2_WG - 1);
+    const bool lo_half = NT == K2_WG || tid < K2_WG;   // stages g
+    const bool hi_half = NT == K2_WG || tid >= K2_WG;  // stages a|b, e
     const int ntiles = z1 > z0 ? (z1 - z0 + K2_WG - 1) / K2_WG : 0;
     uint4 rg = make_uint4(0, 0, 0, 0);
     float4 rab = make_float4(0.f, 0.f, 0.f, 0.f);
     float2 re = make_float2(0.f, 0.f);
     auto fetch = [&](int t) {
-        const int j = z0 + t * K2_WG + tid;
+        const int j = z0 + t * K2_WG + zl;
         if (j < z1) {
-            rg = zv.g[j];
-            const float2 a = zv.af[j];
-            float2 b = make_float2(0.f, 0.f);
-            if (D1) b = zv.bf[j];
-            rab = make_float4(a.x, a.y, b.x, b.y);
-            if (D2) re = zv.ef[j];
+            if (lo_half) rg = zv.g[j];
+            if (hi_half) {
+                const float2 a = zv.af[j];
+                float2 b = make_float2(0.f, 0.f);
+                if (D1) b = zv.bf[j];
+                rab = make_float4(a.x, a.y, b.x, b.y);
+                if (D2) re = zv.ef[j];
+            }
         } else {
             rg = make_uint4(0, 0, 0, 0); rab = make_float4(0.f, 0.f, 0.f, 0.f);
             re = make_float2(0.f, 0.f);
         }
     };
     auto stage = [&](int buf) {
-        sm.g[buf][tid] = rg; sm.ab[buf][tid] = rab;
-        if (D2) sm.e(buf, tid) = re;
+        if (lo_half) sm.g[buf][zl] = rg;
+        if (hi_half) {
+            sm.ab[buf][zl] = rab;
+            if (D2) sm.e(buf, zl) = re;
+        }
     };
     if (ntiles > 0) { fetch(0); stage(0); }
     __syncthreads();

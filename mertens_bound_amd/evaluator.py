@@ -43,7 +43,7 @@ class k2_request(ctypes.Structure):
 class k2_tiling(ctypes.Structure):
     _fields_ = [("nc", ctypes.c_int32), ("target_blocks", ctypes.c_int32),
                 ("mixed_head", ctypes.c_int32), ("mixed_tail", ctypes.c_double),
-                ("z_max_cand", ctypes.c_int32)]
+                ("z_max_cand", ctypes.c_int32), ("wg_threads", ctypes.c_int32)]
 
 
 class k2_batch_tiling(ctypes.Structure):
@@ -171,22 +171,24 @@ class K2Table:
 
     # tiling -----------------------------------------------------------------------------
     def set_tiling(self, nc=None, target_blocks=None, mixed_head=None, mixed_tail=None,
-                   z_max_cand=None):
+                   z_max_cand=None, wg_threads=None):
         """mixed_head 0 chooses the FP64 head from the weight magnitudes (tail <= mixed_tail);
-        candidate counts <= z_max_cand run as Z tiles (-1: never)."""
+        candidate counts <= z_max_cand run as Z tiles (-1: never); wg_threads 256 or 512."""
         cur = self.get_tiling()
         t = k2_tiling(nc if nc is not None else cur["nc"],
                       target_blocks if target_blocks is not None else cur["target_blocks"],
                       mixed_head if mixed_head is not None else cur["mixed_head"],
                       mixed_tail if mixed_tail is not None else cur["mixed_tail"],
-                      z_max_cand if z_max_cand is not None else cur["z_max_cand"])
+                      z_max_cand if z_max_cand is not None else cur["z_max_cand"],
+                      wg_threads if wg_threads is not None else cur["wg_threads"])
         _check(load_library().k2_set_tiling(self._h, ctypes.byref(t)), "k2_set_tiling")
 
     def get_tiling(self):
         t = k2_tiling()
         _check(load_library().k2_get_tiling(self._h, ctypes.byref(t)), "k2_get_tiling")
         return {"nc": t.nc, "target_blocks": t.target_blocks, "mixed_head": t.mixed_head,
-                "mixed_tail": t.mixed_tail, "z_max_cand": t.z_max_cand}
+                "mixed_tail": t.mixed_tail, "z_max_cand": t.z_max_cand,
+                "wg_threads": t.wg_threads}
 
     def mixed_head(self, k, n_zeros=None):
         """FP64 head (zeros) the mixed form uses for k over the first n_zeros zeros."""

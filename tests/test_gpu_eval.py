@@ -63,17 +63,19 @@ def test_deterministic(cfg1):
         assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("nc", [1, 2, 4, 8])
+@pytest.mark.parametrize("nc,wg", [(1, 256), (2, 256), (4, 256), (8, 256), (1, 512), (2, 512)])
 @pytest.mark.parametrize("target", [37, 4096])
-def test_tilings_agree(cfg1, nc, target):
+def test_tilings_agree(cfg1, nc, wg, target):
     tb, t, V, k = cfg1
     g = golden("config1_oracle.npz")
-    t.set_tiling(nc=nc, target_blocks=target)
+    t.set_tiling(nc=nc, target_blocks=target, wg_threads=wg)
     try:
         K, dK, _ = t.eval(V, k, flags=ev.VALUE | ev.D1)
+        Km, _, _ = t.eval(V, k, flags=ev.VALUE | ev.D1 | ev.MIXED)
     finally:
-        t.set_tiling(nc=2, target_blocks=3072)
+        t.set_tiling(nc=2, target_blocks=3072, wg_threads=256)
     assert np.max(np.abs(K - g["K"])) <= 1e-12
+    assert np.max(np.abs(Km - g["K"])) <= 1e-9
 
 
 @pytest.mark.parametrize("n_cand,n_zeros", [(1, 1), (1, 10_000), (255, 257), (257, 3333),
@@ -108,6 +110,10 @@ def test_argument_errors(cfg1):
         t.eval(V[:4], -1.0)
     with pytest.raises(ev.K2Error):
         t.set_tiling(nc=3)
+    with pytest.raises(ev.K2Error):
+        t.set_tiling(nc=4, wg_threads=512)
+    with pytest.raises(ev.K2Error):
+        t.set_tiling(wg_threads=384)
     K, _, _ = t.eval(V[:0], k)
     assert K.shape == (0,)
 
