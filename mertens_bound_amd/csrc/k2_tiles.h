@@ -176,7 +176,7 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
         if (t + 1 < ntiles) fetch(t + 1);
-#pragma unroll 2
+#pragma unroll(NT == K2_WG ? 2 : 1)
         for (int z = 0; z < K2_WG; ++z) {
             const uint4 gg = sm.g[buf][z];
             const double2 aa = sm.a(buf, z);
