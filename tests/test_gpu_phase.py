@@ -80,3 +80,17 @@ def test_sincos_of_words():
     assert np.max(np.abs(s - np.sin(th))) < 2e-15
     s32, c32 = ev.sincos_words(W, mode=1)
     assert np.max(np.abs(s32 - np.sin(th))) < 3e-7 and np.max(np.abs(c32 - np.cos(th))) < 3e-7
+
+
+def test_phase_words_full_table(oracle_lib):
+    """All 2,325,572 zeros of the full-size table against candidates spread over
+    [3.4e18, 3.6e18]: every 64-bit word and every top-32-bit word bit-exact."""
+    c = synth.CONFIG2
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    t = ev.K2Table(tb)
+    V = np.concatenate([synth.v_from_int([int(3.4e18) << 64, (int(3.5e18) << 64) + 12345,
+                                          (int(3.6e18) << 64) - 1]),
+                        synth.make_candidates(1, c["v_lo"], c["v_hi"], 77)])
+    Wo = oracle_lib.phase_words(tb.G, V)
+    assert np.array_equal(t.phase_words(V, mode=0), Wo)
+    assert np.array_equal(t.phase_words(V, mode=2) >> np.uint64(32), Wo >> np.uint64(32))
