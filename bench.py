@@ -220,6 +220,23 @@ def run_eval(args, form):
     res["roofline"] = roofline(form, evals, k_ms / max(1, k_n), alg_bytes,
                                f"{args.workload}_{form}")
     res["wall_s"] = wall
+    if form == "mixed":
+        # the FP64 kernel on the same inputs and GPU: speed-up and |dK2| for every candidate
+        out64 = tuple(torch.empty(n_cand, dtype=torch.float64, device="cuda") for _ in range(3))
+        t64 = ev.K2Table(tb)
+        if not (args.nc or args.target_blocks):
+            autotune.apply_tuned(t64, n_cand, tb.n, "plain")
+
+        def step64():
+            t64.eval_device(Vd, c["k"], flags=ev.VALUE | ev.D1, out=out64, stream=st)
+
+        ms64, _ = timed(step64, args.steps, args.warmup, torch)
+        step()
+        torch.cuda.synchronize()
+        d = (out[0] - out64[0]).abs()
+        res["vs_plain"] = {"plain_ms_per_step": ms64, "speedup": ms64 / ms,
+                           "max_abs_dK2_vs_fp64_kernel_all_candidates": float(d.max().item()),
+                           "mean_abs_dK2_vs_fp64_kernel": float(d.mean().item())}
     if not args.no_cpu and args.workload == "metric":
         try:
             cb = cpu_baseline(tb, V, c["k"])

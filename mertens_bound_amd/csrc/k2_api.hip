@@ -399,7 +399,14 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
         pp.b = pool_event(t);
         HIPCHK(hipEventRecord(pp.a, st));
     }
-    if (p.sp_tail > 0 && p.sp_head > 0) {
+    const bool fused = p.sp_tail > 0 && p.sp_head > 0 && a.nt == 256 && a.nc <= 2 &&
+                       (double)n_cand * nz <= K2_FUSED_MIXED_EVALS;
+    if (fused) {
+        // small mixed problem: head and tail splits in one launch on st
+        a.z_begin = 0; a.z_end = nz; a.row_base = 0;
+        a.n_blocks = p.n_ctiles * (p.sp_head + p.sp_tail);
+        HIPCHK(launch_eval_mixed(a, p.head_end, p.zps_head, p.sp_head, p.zps_tail, st));
+    } else if (p.sp_tail > 0 && p.sp_head > 0) {
         // mixed form: FP64 head on the auxiliary stream, FP32 tail on st, joined before reducing
         if (!t->aux) {
             HIPCHK(hipStreamCreateWithFlags(&t->aux, hipStreamNonBlocking));

@@ -31,6 +31,7 @@ struct WeightArrays {
 #define K2_Z_MAX_CAND 384   // default: candidate counts up to this use Z tiles (measured crossover)
 #define K2_Z_CHUNK 8192     // fewest zeros per Z-tile workgroup
 #define K2_MIN_SPLIT 256    // fewest zeros per C-tile split (one LDS tile)
+#define K2_FUSED_MIXED_EVALS 1e9   // mixed-form problems up to this many evaluations: one launch
 
 struct LaunchArgs {
     int nc = 4;
@@ -124,6 +125,10 @@ struct k2_table {
 namespace k2impl {
 
 hipError_t launch_eval(const LaunchArgs& a, bool mixed, hipStream_t st);
+// mixed form in one launch: splits [0, sp_head) FP64 over [0, head_end) in zps_head steps, the
+// rest FP32 over [head_end, a.z_end) in zps_tail steps (nc 1 or 2, 256 threads)
+hipError_t launch_eval_mixed(const LaunchArgs& a, int head_end, int zps_head, int sp_head,
+                             int zps_tail, hipStream_t st);
 // zeros per split (a multiple of 256) and split count for nz zeros, n_ctiles candidate tiles
 // and a target workgroup count
 void split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps);

@@ -109,6 +109,65 @@ __global__ __launch_bounds__(NT, (NT == 512 ? 6 : (NC <= 2 ? 4 : 2))) void k2_ev
     }
 }
 
+// ---- mixed form in one launch (small problems, where a second stream costs more than it saves):
+// splits [0, sp_head) cover the FP64 head [0, head_end), the rest the FP32 tail [head_end, z_end)
+template <int NC, int FL>
+__global__ __launch_bounds__(256, 4) void k2_eval_mixed_kernel(
+    ZeroView zv, int head_end, int zps_head, int sp_head, int z_end, int zps_tail,
+    const uint4* __restrict__ cand, int n_cand, int n_ctiles, double* __restrict__ partial,
+    int n_cand_pad, const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
+    constexpr size_t TILE = ctile_smem_bytes<FL>();
+    __shared__ __align__(16) unsigned char sm_raw[TILE];
+    __shared__ Tab64 s_tab64[K2_TAB];   // the FP32 table of a tail block sits in its first half
+    const int L = xcd_linear_block();
+    const int split = L / n_ctiles, ct = L - split * n_ctiles;
+    const int tid = threadIdx.x;
+    const bool head = split < sp_head;
+    Tab32* s_tab32 = reinterpret_cast<Tab32*>(s_tab64);
+    for (int i = tid; i < K2_TAB; i += K2_ZT) {
+        if (head) s_tab64[i] = tab64[i];
+        else s_tab32[i] = tab32[i];
+    }
+    Limbs4 v[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+        uint4 cv = make_uint4(0, 0, 0, 0);
+        if (c < n_cand) cv = cand[c];
+        v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
+    }
+    double* prow = partial + (size_t)split * 3 * n_cand_pad;
+    if (head) {
+        const int z0 = split * zps_head, z1 = min(head_end, z0 + zps_head);
+        Acc<false> acc[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) acc[i].zero();
+        ctile_f64<NC, FL>(zv, z0, z1, v, acc, *reinterpret_cast<CTileSmem64<FL>*>(sm_raw),
+                          s_tab64);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+            prow[c] = acc[i].get(0);
+            if (FL & K2_D1) prow[n_cand_pad + c] = acc[i].get(1);
+            if (FL & K2_D2) prow[2 * n_cand_pad + c] = acc[i].get(2);
+        }
+    } else {
+        const int z0 = head_end + (split - sp_head) * zps_tail, z1 = min(z_end, z0 + zps_tail);
+        Acc<true> acc[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) acc[i].zero();
+        ctile_f32<NC, FL>(zv, z0, z1, v, acc, *reinterpret_cast<CTileSmem32<FL>*>(sm_raw),
+                          s_tab32);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const int c = ct * (K2_ZT * NC) + i * K2_ZT + tid;
+            prow[c] = acc[i].get(0);
+            if (FL & K2_D1) prow[n_cand_pad + c] = acc[i].get(1);
+            if (FL & K2_D2) prow[2 * n_cand_pad + c] = acc[i].get(2);
+        }
+    }
+}
+
 // ---- one-launch evaluation for few candidates: Z tiles over (chunk, candidate group) ------------
 template <int FL, bool F32>
 __global__ __launch_bounds__(256) void k2_eval_z_kernel(
@@ -352,6 +411,38 @@ hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad
     hipLaunchKernelGGL(k2_weights_kernel, dim3(nb), dim3(256), 0, st, gamma, (const double2*)w, n,
                        n_pad, k, out.a, out.b, out.e, out.af, out.bf, out.ef);
     return hipGetLastError();
+}
+
+template <int NC, int FL>
+static hipError_t launch_mixed_t(const LaunchArgs& a, int head_end, int zps_head, int sp_head,
+                                 int zps_tail, hipStream_t st) {
+    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef};
+    hipLaunchKernelGGL((k2_eval_mixed_kernel<NC, FL>), dim3(a.n_blocks), dim3(K2_ZT), 0, st, zv,
+                       head_end, zps_head, sp_head, a.z_end, zps_tail, a.cand, a.n_cand,
+                       a.n_ctiles, a.partial, a.n_cand_pad, a.tab64, a.tab32);
+    return hipGetLastError();
+}
+
+template <int NC>
+static hipError_t launch_mixed_fl(const LaunchArgs& a, int head_end, int zps_head, int sp_head,
+                                  int zps_tail, hipStream_t st) {
+    switch (a.flags & (K2_D1 | K2_D2)) {
+        case 0: return launch_mixed_t<NC, 0>(a, head_end, zps_head, sp_head, zps_tail, st);
+        case K2_D1: return launch_mixed_t<NC, K2_D1>(a, head_end, zps_head, sp_head, zps_tail, st);
+        case K2_D2: return launch_mixed_t<NC, K2_D2>(a, head_end, zps_head, sp_head, zps_tail, st);
+        default:
+            return launch_mixed_t<NC, K2_D1 | K2_D2>(a, head_end, zps_head, sp_head, zps_tail, st);
+    }
+}
+
+hipError_t launch_eval_mixed(const LaunchArgs& a, int head_end, int zps_head, int sp_head,
+                             int zps_tail, hipStream_t st) {
+    if (a.nt != 256) return hipErrorInvalidValue;
+    switch (a.nc) {
+        case 1: return launch_mixed_fl<1>(a, head_end, zps_head, sp_head, zps_tail, st);
+        case 2: return launch_mixed_fl<2>(a, head_end, zps_head, sp_head, zps_tail, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <int FL, bool F32>
