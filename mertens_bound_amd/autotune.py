@@ -28,7 +28,8 @@ TUNED = os.path.join(PKG, "tuned_tiles.json")
 
 EVAL_GRID = {"nc": [1, 2, 4], "target_blocks": [1536, 3072, 6144, 12288],
              "wg_threads": [256, 512]}
-BATCH_GRID = {"nc": [2, 4], "cz": [8, 16], "chunk": [4096, 8192, 16384], "wg_per_cu": [0, 2]}
+BATCH_GRID = {"nc": [2, 4], "cz": [8, 16], "chunk": [4096, 8192, 16384], "wg_per_cu": [0],
+              "c_min_fill": [128, 256, 384]}
 
 _cache = None
 
@@ -121,9 +122,12 @@ def tune_batch(table, reqs, V, form, grid=BATCH_GRID, log=print):
     from . import evaluator as ev
     flags = ev.VALUE | ev.D1 | (ev.MIXED if form == "mixed" else 0)
     results = []
-    for nc, cz, chunk, wpc in itertools.product(grid["nc"], grid["cz"], grid["chunk"],
-                                                grid["wg_per_cu"]):
-        ti = {"nc": nc, "cz": cz, "chunk": chunk, "wg_per_cu": wpc}
+    for nc, cz, chunk, wpc, fill in itertools.product(grid["nc"], grid["cz"], grid["chunk"],
+                                                      grid["wg_per_cu"],
+                                                      grid.get("c_min_fill", [0])):
+        if fill >= 256 * nc:
+            continue
+        ti = {"nc": nc, "cz": cz, "chunk": chunk, "wg_per_cu": wpc, "c_min_fill": fill}
         b = ev.K2Batch(table, reqs, V, flags=flags, tiling=ti)
         ms = _time(b.run, torch, reps=2)
         info = b.info()
@@ -153,7 +157,8 @@ def main():
     bg = dict(BATCH_GRID)
     if args.quick:
         eg = {"nc": [2], "target_blocks": [3072, 6144, 12288], "wg_threads": [256, 512]}
-        bg = {"nc": [4], "cz": [8, 16], "chunk": [4096, 8192], "wg_per_cu": [0]}
+        bg = {"nc": [2], "cz": [8], "chunk": [8192, 16384], "wg_per_cu": [0],
+              "c_min_fill": [128, 256]}
     torch.cuda.set_device(0)
     c2 = synth.CONFIG2
     big = synth.make_zero_table(c2["n_zeros"], c2["seed"])
