@@ -7,8 +7,14 @@
   rows (ghi_lo, ghi_hi, glo_lo, glo_hi, psi_lo, psi_hi).
 * words: glo = rint(g 2^98), ghi = rint(g 2^146) mod 2^128, psi = rint(frac(-arg w / 2pi) 2^128)
   with g = gamma / 2pi (search/path1_build_zeros_100k.py:56-58,102-104).
+* ``path1_zeros_100k.npz`` (search/path1_build_zeros_100k.py:145-159): the same words plus
+  gamma, re_c, im_c (w = re_c + i im_c), alpha and two_astar.
+* zero chunk text files, one zero per line ``gamma re im`` in decimal (read by
+  search/polish.py:23-38 and search/path1_build_zeros_100k.py:63-90).
 * candidates: ``CLUSTER`` log lines carry x (Q98.30 integer) and dy, y = x / 2^30 + dy
-  (search/eval_sub35_candidates.py:26-46, search/eval_tech1_600k.py:27-29,216-218).
+  (search/eval_sub35_candidates.py:26-46, search/eval_tech1_600k.py:27-29,216-218;
+  search/path1_kernel_opt.py:224-264); the peak summary CSV of
+  search/results/subrecord_peaks_summary.csv carries x_int, dy and y_exact_base.
 
 Canonical words: G = floor(g 2^110) (Q18.110) and V = v 2^64 (64.64).  A canonical table maps
 exactly onto reference words (ghi = G 2^36 mod 2^128, glo = round(G / 2^12)); a reference table
@@ -160,16 +166,122 @@ _CLUSTER = re.compile(
     r"h=(?P<h>[+\-0-9.]+).*?\(x=(?P<x>\d+),\s*dy=(?P<dy>[+\-0-9.eE]+)\)")
 
 
+_Y = re.compile(r"\sy=([0-9.]+)\s")
+
+
 def parse_cluster_lines(lines):
-    """Candidates from CLUSTER lines: list of dict(n, coset, absh, h, x, dy)."""
+    """Candidates from CLUSTER lines: list of dict(n, coset, absh, h, x, dy, y) (y the logged
+    value, None where the line has none)."""
     out = []
     for line in lines:
         m = _CLUSTER.search(line)
         if not m:
             continue
+        my = _Y.search(line)
         out.append({"n": int(m["n"]), "coset": m["coset"], "absh": float(m["absh"]),
-                    "h": float(m["h"]), "x": int(m["x"]), "dy": float(m["dy"])})
+                    "h": float(m["h"]), "x": int(m["x"]), "dy": float(m["dy"]),
+                    "y": float(my.group(1)) if my else None})
     return out
+
+
+# the two reference selections of CLUSTER candidates
+SUB35_SELECTION = {"y_max": 3.4849e18, "shift": 32, "min_abs_h": None, "y_from": "log"}
+KERNEL_OPT_SELECTION = {"y_max": 1.95718788505622e19, "shift": 18, "min_abs_h": 0.96,
+                        "y_from": "x_dy"}
+
+
+def select_candidates(cands, y_max, shift, min_abs_h=None, y_from="log"):
+    """Filter and deduplicate CLUSTER candidates like the reference drivers: keep
+    0 < y < y_max (y as logged, or x / 2^30 + dy) and |h| >= min_abs_h, then the largest |h| per
+    x >> shift, sorted by |h| (search/eval_sub35_candidates.py:26-46 with SUB35_SELECTION;
+    search/path1_kernel_opt.py:224-264 with KERNEL_OPT_SELECTION)."""
+    keep = []
+    for c in cands:
+        y = c.get("y") if y_from == "log" else c["x"] / 2.0 ** 30 + c["dy"]
+        if y is None or not (0 < y < y_max):
+            continue
+        if min_abs_h is not None and c["absh"] < min_abs_h:
+            continue
+        keep.append(c)
+    return dedupe_candidates(keep, shift=shift)
+
+
+def read_peaks_csv(path):
+    """Rows of search/results/subrecord_peaks_summary.csv as dicts; x (int), dy (float) and
+    y_exact (the decimal string of x / 2^30 + dy) are parsed, the other columns kept as text."""
+    import csv
+    out = []
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            d = dict(row)
+            d["x"] = int(row["x_int"])
+            d["dy"] = float(row["dy"])
+            d["y_exact"] = row["y_exact_base"]
+            out.append(d)
+    return out
+
+
+# ---- zero chunk text files and the .npz table ---------------------------------------------
+
+def read_zero_chunks(paths, gamma_max=None):
+    """Zeros from text files of ``gamma re im`` lines (decimal).  G is the 64.64-style word
+    round(gamma / 2pi 2^110) of the decimal gamma at 60 significant digits, so the table's gamma
+    (2 pi G / 2^110) is the decimal value to within 2^-110 relative, independent of any double
+    rounding; w = re + i im.  Zeros with gamma >= gamma_max are skipped (search/polish.py:31-33)."""
+    import mpmath
+    Gs, gam, w = [], [], []
+    with mpmath.workdps(60):
+        scale = mpmath.mpf(2) ** 110 / (2 * mpmath.pi)
+        for p in paths:
+            with open(p) as f:
+                for line in f:
+                    parts = line.split()
+                    if len(parts) < 3:
+                        continue
+                    g = mpmath.mpf(parts[0])
+                    if gamma_max is not None and g >= gamma_max:
+                        continue
+                    Gs.append(int(mpmath.nint(g * scale)))
+                    gam.append(float(g))
+                    w.append(complex(float(parts[1]), float(parts[2])))
+    G = np.zeros((len(Gs), 2), dtype=np.uint64)
+    for j, x in enumerate(Gs):
+        G[j, 0], G[j, 1] = _words(x)
+    return synth.ZeroTable(G=G, gamma=np.array(gam, dtype=np.float64),
+                           w=np.array(w, dtype=np.complex128))
+
+
+def write_zero_chunk(path, gamma_text, w):
+    """One text file of ``gamma re im`` lines (gamma as given decimal strings)."""
+    with open(path, "w") as f:
+        for g, c in zip(gamma_text, w):
+            f.write(f"{g} {c.real!r} {c.imag!r}\n")
+
+
+_NPZ_KEYS = ("gamma", "re_c", "im_c", "alpha", "two_astar", "ghi_lo", "ghi_hi", "glo_lo",
+             "glo_hi", "psi_lo", "psi_hi")
+
+
+def write_zeros_npz(path, table: synth.ZeroTable, weight_k: float = 3e-9):
+    """search/path1_build_zeros_100k.py:145-159 layout from a canonical table."""
+    ref = reference_words(table, weight_k=weight_k)
+    ref_a = reference_words(table)
+    np.savez(path, gamma=table.gamma, re_c=table.w.real, im_c=table.w.imag,
+             alpha=ref_a["alpha"], two_astar=ref["two_astar"],
+             **{k: ref[k] for k in ("ghi_lo", "ghi_hi", "glo_lo", "glo_hi", "psi_lo", "psi_hi")})
+
+
+def read_zeros_npz(path) -> dict:
+    with np.load(path, allow_pickle=False) as d:
+        return {k: np.array(d[k]) for k in _NPZ_KEYS if k in d.files}
+
+
+def table_from_npz(d: dict) -> synth.ZeroTable:
+    """Canonical table from .npz fields: G from the ghi/glo words (as table_from_reference),
+    w = re_c + i im_c (the stored weights, not their alpha/psi rounding)."""
+    t = table_from_reference(d["gamma"], d["alpha"], d["ghi_lo"], d["ghi_hi"], d["glo_lo"],
+                             d["glo_hi"], d["psi_lo"], d["psi_hi"])
+    return synth.ZeroTable(G=t.G, gamma=t.gamma, w=d["re_c"] + 1j * d["im_c"])
 
 
 def dedupe_candidates(cands, shift=32):

@@ -151,11 +151,16 @@ def candidates_from_x_dy(x, dy0) -> np.ndarray:
     """Reference candidate (x in Q98.30, dy0 float) -> 64.64 words, V = x 2^34 + rint(dy0 2^64).
 
     The reference evaluates at y = x / 2^30 + dy0 (search/eval_tech1_600k.py:216-218,
-    legacy/superseded/certify_peak2_600k.py:175-177).
+    legacy/superseded/certify_peak2_600k.py:175-177).  A 64.64 word holds 0 <= y < 2^64
+    (1.8447e19); candidates outside raise ValueError rather than wrap.
     """
     out = []
     for xi, d in zip(x, dy0):
-        out.append((int(xi) << 34) + int(round(float(d) * 2.0 ** 64)))
+        v = (int(xi) << 34) + int(round(float(d) * 2.0 ** 64))
+        if not 0 <= v < (1 << 128):
+            raise ValueError(f"candidate y = {int(xi) / 2.0 ** 30 + float(d):.6e} is outside the "
+                             "64.64 range [0, 2^64)")
+        out.append(v)
     return v_from_int(out)
 
 

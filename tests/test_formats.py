@@ -1,5 +1,8 @@
 """Reference table/candidate formats and their conversion to the canonical encoding."""
+import os
+
 import numpy as np
+import pytest
 
 from conftest import GOLDEN
 from mertens_bound_amd import formats, synth
@@ -62,3 +65,78 @@ def test_cluster_lines_from_reference_logs():
     V = synth.candidates_from_x_dy([c["x"] for c in cands if c["dy"] >= 0],
                                    [c["dy"] for c in cands if c["dy"] >= 0])
     assert V.shape[1] == 2
+
+
+def test_peaks_csv_candidate_encoding():
+    """The reference's own peak list (search/results/subrecord_peaks_summary.csv): each row's
+    (x_int, dy) maps to a 64.64 word whose value matches the row's y_exact_base, printed by the
+    reference to 10 decimals, within that rounding."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    rows = formats.read_peaks_csv(os.path.join(GOLDEN, "subrecord_peaks_summary.csv"))
+    assert len(rows) == 25
+    assert [int(r["rank_at_k3e9"]) for r in rows] == list(range(1, 26))
+    inside = [r for r in rows if Decimal(r["y_exact"]) < Decimal(2 ** 64)]
+    outside = [r for r in rows if Decimal(r["y_exact"]) >= Decimal(2 ** 64)]
+    assert len(inside) == 24 and len(outside) == 1       # one peak sits above y = 2^64
+    V = synth.candidates_from_x_dy([r["x"] for r in inside], [r["dy"] for r in inside])
+    for r, v in zip(inside, synth.v_to_int(V)):
+        val = Decimal(v) / Decimal(2 ** 64)
+        assert abs(val - Decimal(r["y_exact"])) <= Decimal("1e-10"), (r["rank_at_k3e9"], val)
+    with pytest.raises(ValueError):
+        synth.candidates_from_x_dy([outside[0]["x"]], [outside[0]["dy"]])
+
+
+def test_cluster_selection_presets():
+    lines = open(os.path.join(GOLDEN, "cluster_sample.txt")).read().splitlines()
+    cands = formats.parse_cluster_lines(lines)
+    assert cands and all(c["y"] is not None for c in cands)
+    for preset in (formats.SUB35_SELECTION, formats.KERNEL_OPT_SELECTION):
+        sel = formats.select_candidates(cands, **preset)
+        keys = [c["x"] >> preset["shift"] for c in sel]
+        assert len(keys) == len(set(keys))                      # one per bucket
+        assert all(a["absh"] >= b["absh"] for a, b in zip(sel, sel[1:]))
+        for c in sel:
+            y = c["y"] if preset["y_from"] == "log" else c["x"] / 2.0 ** 30 + c["dy"]
+            assert 0 < y < preset["y_max"]
+            if preset["min_abs_h"] is not None:
+                assert c["absh"] >= preset["min_abs_h"]
+            # the kept one is the largest |h| among the admissible candidates of its bucket
+            same = [d for d in cands if d["x"] >> preset["shift"] == c["x"] >> preset["shift"]]
+            adm = [d for d in same
+                   if 0 < (d["y"] if preset["y_from"] == "log" else d["x"] / 2.0 ** 30 + d["dy"])
+                   < preset["y_max"]
+                   and (preset["min_abs_h"] is None or d["absh"] >= preset["min_abs_h"])]
+            assert c["absh"] == max(d["absh"] for d in adm)
+
+
+def test_zero_chunk_text_exact_words(tmp_path):
+    """gamma from decimal text at 60 digits: G = round(gamma / 2pi 2^110) exactly (checked with
+    rational arithmetic at 100 digits), independent of the double rounding of gamma."""
+    import mpmath
+    gammas = ["14.134725141734693790457251983562470270784257115699",
+              "21.022039638771554992628479593896902777334340524903",
+              "1299999.9876543210987654321098765432109876543210"]
+    w = [0.01 - 0.02j, -0.03 + 0.001j, 1e-7 + 2e-7j]
+    p = tmp_path / "chunk_000.txt"
+    formats.write_zero_chunk(p, gammas, w)
+    tb = formats.read_zero_chunks([p])
+    with mpmath.workdps(100):
+        for j, g in enumerate(gammas):
+            want = int(mpmath.nint(mpmath.mpf(g) / (2 * mpmath.pi) * mpmath.mpf(2) ** 110))
+            assert synth.v_to_int(tb.G[j:j + 1])[0] == want
+            assert tb.gamma[j] == float(mpmath.mpf(g))
+    np.testing.assert_array_equal(tb.w, np.array(w))
+    assert formats.read_zero_chunks([p], gamma_max=100.0).n == 2
+
+
+def test_npz_round_trip(tmp_path):
+    tb = synth.make_zero_table(2000, 5)
+    p = tmp_path / "path1_zeros_100k.npz"
+    formats.write_zeros_npz(p, tb)
+    d = formats.read_zeros_npz(p)
+    assert set(d) >= {"gamma", "re_c", "im_c", "ghi_lo", "glo_lo", "psi_hi", "two_astar"}
+    back = formats.table_from_npz(d)
+    np.testing.assert_array_equal(back.G, tb.G)
+    np.testing.assert_array_equal(back.w, tb.w)
+    np.testing.assert_array_equal(back.gamma, tb.gamma)
