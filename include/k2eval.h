@@ -145,6 +145,16 @@ int k2_polish_device(k2_table *t, uint32_t n_cand, uint64_t *V_dev, double k, ui
  * after the first step only the candidates still moving are evaluated again. */
 int k2_polish_stats(k2_table *t, int64_t *cand_evals);
 
+/* Several real weight vectors in one pass (device buffers): for m < n_w (<= 8)
+ *   K_m = sum_{j < n_zeros} wt[m*ld + j] 2 Re(w_j e^{i g_j v})   (K2_D1: K_m', K2_D2: K_m'')
+ * written to K_dev[m*n_cand + c] (dK_dev, d2K_dev likewise).  The reference forms these as
+ * h = A @ wt per weight vector after compute_peak_AB (search/eval_tech1_600k.py:243-267: Gaussian
+ * at two k, the JP kernel, raw and a 10,000-zero truncation); here the exact phase and the sin/cos
+ * of each candidate and zero are shared by all the vectors. */
+int k2_eval_weights_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, uint32_t n_w,
+                           const double *wt_dev, uint32_t ld, uint32_t n_zeros, uint32_t flags,
+                           double *K_dev, double *dK_dev, double *d2K_dev, void *stream);
+
 /* Enclosure of K2 (and K2' with K2_D1) over the first n_zeros zeros (device buffers): the exact
  * sum over the table's definitions (gamma_j = 2 pi G_j / 2^110, w_j as stored, k and v as given)
  * lies in [K - K_rad, K + K_rad].  FP64 with error-free (TwoSum) accumulation and per-zero error

@@ -1,0 +1,237 @@
+// Several real weight vectors in one pass over the zeros:
+//
+//   K_m(v) = sum_{j < n_zeros} wt_m[j] * 2 Re(w_j e^{i g_j v}),   m < n_w <= 8
+//
+// (and K_m' = -sum wt_m g 2 Im(...), K_m'' = -sum wt_m g^2 2 Re(...)), the quantity the
+// reference forms as h = A @ wt per weight vector after compute_peak_AB (A_j = 2 Re(w_j e^{i g_j v}),
+// B_j = 2 Im(...); search/eval_tech1_600k.py:243-267 with the Gaussian, JP-kernel, raw and
+// truncated vectors).  The exact phase and the sin/cos are computed once per candidate and zero
+// and shared by all vectors: each further vector costs one FMA per requested sum.
+//
+// C tiles of 256 lanes, one candidate per lane; zero records of 128 zeros staged in LDS (double
+// buffered): G, w, g (derivatives only) and 2 wt_m; split partial rows [split][q*M + m][cand].
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <string>
+
+#include "k2_device.h"
+#include "k2_internal.h"
+#include "k2_tiles.h"
+
+using namespace k2dev;
+
+#define K2_MW_TZ 128     // zeros per LDS tile
+#define K2_MW_MAX 8      // weight vectors per call
+
+template <int M, bool DER>
+struct MwSmem {
+    uint4 g[2][K2_MW_TZ];
+    double2 w[2][K2_MW_TZ];
+    double gam[2][DER ? K2_MW_TZ : 1];
+    double wt[2][M][K2_MW_TZ];   // 2 wt_m[j]
+};
+
+template <int M, int FL>
+__global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
+    const uint4* __restrict__ zg, const double2* __restrict__ zw, const double* __restrict__ zgam,
+    const double* __restrict__ wt, int ld, int nz, int zps, const uint4* __restrict__ cand,
+    int n_cand, int n_ctiles, double* __restrict__ partial, int n_cand_pad,
+    const Tab64* __restrict__ tab64) {
+    constexpr bool D1 = (FL & K2_D1) != 0;
+    constexpr bool D2 = (FL & K2_D2) != 0;
+    constexpr bool DER = D1 || D2;
+    constexpr int NQ = 1 + (D1 ? 1 : 0) + (D2 ? 1 : 0);
+    __shared__ MwSmem<M, DER> sm;
+    __shared__ Tab64 s_tab[K2_TAB];
+    const int L = xcd_linear_block();
+    const int split = L / n_ctiles, ct = L - split * n_ctiles;
+    const int z0 = split * zps, z1 = min(nz, z0 + zps);
+    const int tid = threadIdx.x;
+    for (int i = tid; i < K2_TAB; i += 256) s_tab[i] = tab64[i];
+    const int c = ct * 256 + tid;
+    uint4 cv = make_uint4(0, 0, 0, 0);
+    if (c < n_cand) cv = cand[c];
+    const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
+    double acc[NQ][M];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+    // staging: lanes 0..127 carry G, w, g of one zero; lanes 128..255 its M weights
+    const int zl = tid & (K2_MW_TZ - 1);
+    const bool rec = tid < K2_MW_TZ;
+    uint4 rg = make_uint4(0, 0, 0, 0);
+    double2 rw = make_double2(0.0, 0.0);
+    double rgam = 0.0;
+    double rwt[M];
+    const int ntiles = z1 > z0 ? (z1 - z0 + K2_MW_TZ - 1) / K2_MW_TZ : 0;
+    auto fetch = [&](int t) {
+        const int j = z0 + t * K2_MW_TZ + zl;
+        const bool in = j < z1;
+        if (rec) {
+            rg = in ? zg[j] : make_uint4(0, 0, 0, 0);
+            rw = in ? zw[j] : make_double2(0.0, 0.0);
+            if (DER) rgam = in ? zgam[j] : 0.0;
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) rwt[m] = in ? 2.0 * wt[(size_t)m * ld + j] : 0.0;
+        }
+    };
+    auto stage = [&](int buf) {
+        if (rec) {
+            sm.g[buf][zl] = rg;
+            sm.w[buf][zl] = rw;
+            if (DER) sm.gam[buf][zl] = rgam;
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) sm.wt[buf][m][zl] = rwt[m];
+        }
+    };
+    if (ntiles > 0) { fetch(0); stage(0); }
+    __syncthreads();
+    const SinCosCoef kc = sincos_coef();
+    for (int t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        if (t + 1 < ntiles) fetch(t + 1);
+        for (int z = 0; z < K2_MW_TZ; ++z) {
+            const uint4 gg = sm.g[buf][z];
+            const double2 ww = sm.w[buf][z];
+            const uint64_t W = phase_word(Limbs4{gg.x, gg.y, gg.z, gg.w}, v);
+            double S, C;
+            sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), s_tab, S, C, kc);
+            const double A = fma(ww.x, C, -ww.y * S);          // Re(w e^{i th})
+#pragma unroll
+            for (int m = 0; m < M; ++m) acc[0][m] = fma(sm.wt[buf][m][z], A, acc[0][m]);
+            if constexpr (DER) {
+                const double g = sm.gam[buf][z];
+                if constexpr (D1) {
+                    const double gB = g * fma(ww.x, S, ww.y * C);   // g Im(w e^{i th})
+#pragma unroll
+                    for (int m = 0; m < M; ++m) acc[1][m] = fma(-sm.wt[buf][m][z], gB, acc[1][m]);
+                }
+                if constexpr (D2) {
+                    const double ggA = g * g * A;
+#pragma unroll
+                    for (int m = 0; m < M; ++m)
+                        acc[NQ - 1][m] = fma(-sm.wt[buf][m][z], ggA, acc[NQ - 1][m]);
+                }
+            }
+        }
+        if (t + 1 < ntiles) stage(buf ^ 1);
+        __syncthreads();
+    }
+    double* prow = partial + (size_t)split * NQ * M * n_cand_pad;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int m = 0; m < M; ++m) prow[(size_t)(q * M + m) * n_cand_pad + c] = acc[q][m];
+}
+
+// row q*M + m of the split partials, summed over splits in order, into K (q = 0), then K' and
+// K'' as requested; each output is [m][cand]
+__global__ __launch_bounds__(256) void k2_multiw_reduce_kernel(
+    const double* __restrict__ partial, int n_splits, int M, int nq, int has_d1, int n_cand,
+    int n_cand_pad, double* __restrict__ K, double* __restrict__ dK, double* __restrict__ d2K) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int rows = nq * M;
+    if (i >= (long long)rows * n_cand) return;
+    const int r = (int)(i / n_cand), c = (int)(i - (long long)r * n_cand);
+    double s = 0.0;
+    for (int sp = 0; sp < n_splits; ++sp) s += partial[((size_t)sp * rows + r) * n_cand_pad + c];
+    const int q = r / M, m = r - q * M;
+    double* dst = q == 0 ? K : ((q == 1 && has_d1) ? dK : d2K);
+    dst[(size_t)m * n_cand + c] = s;
+}
+
+namespace k2impl {
+
+template <int M, int FL>
+static hipError_t launch_mw_t(dim3 grid, const k2_table* t, const double* wt, int ld, int nz,
+                              int zps, const uint4* cand, int n_cand, int n_ctiles,
+                              double* partial, int n_cand_pad, hipStream_t st) {
+    hipLaunchKernelGGL((k2_multiw_kernel<M, FL>), grid, dim3(256), 0, st, t->g,
+                       (const double2*)t->w, t->gamma, wt, ld, nz, zps, cand, n_cand, n_ctiles,
+                       partial, n_cand_pad, t->tab64);
+    return hipGetLastError();
+}
+
+template <int M>
+static hipError_t launch_mw_fl(unsigned fl, dim3 grid, const k2_table* t, const double* wt,
+                               int ld, int nz, int zps, const uint4* cand, int n_cand,
+                               int n_ctiles, double* partial, int n_cand_pad, hipStream_t st) {
+    switch (fl & (K2_D1 | K2_D2)) {
+        case 0: return launch_mw_t<M, 0>(grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+        case K2_D1: return launch_mw_t<M, K2_D1>(grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+        case K2_D2: return launch_mw_t<M, K2_D2>(grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+        default: return launch_mw_t<M, K2_D1 | K2_D2>(grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+    }
+}
+
+static hipError_t launch_mw(int M, unsigned fl, dim3 grid, const k2_table* t, const double* wt,
+                            int ld, int nz, int zps, const uint4* cand, int n_cand, int n_ctiles,
+                            double* partial, int n_cand_pad, hipStream_t st) {
+    switch (M) {
+#define K2_MW(X) case X: return launch_mw_fl<X>(fl, grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+        K2_MW(1) K2_MW(2) K2_MW(3) K2_MW(4) K2_MW(5) K2_MW(6) K2_MW(7) K2_MW(8)
+#undef K2_MW
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace k2impl
+
+// ---- C ABI -------------------------------------------------------------------------------------
+
+using namespace k2impl;
+
+extern "C" int k2_eval_weights_device(k2_table* t, uint32_t n_cand, const uint64_t* V_dev,
+                                      uint32_t n_w, const double* wt_dev, uint32_t ld,
+                                      uint32_t n_zeros, uint32_t flags, double* K_dev,
+                                      double* dK_dev, double* d2K_dev, void* stream) {
+    if (!t) return set_error(K2_E_ARG, "k2_eval_weights_device: null table");
+    if (n_w < 1 || n_w > K2_MW_MAX)
+        return set_error(K2_E_ARG, "k2_eval_weights_device: 1..8 weight vectors");
+    if (flags & ~(K2_VALUE | K2_D1 | K2_D2))
+        return set_error(K2_E_ARG, "k2_eval_weights_device: flags are K2_VALUE, K2_D1, K2_D2");
+    if (n_zeros == 0) n_zeros = t->n;
+    if (n_zeros > t->n || ld < n_zeros)
+        return set_error(K2_E_ARG, "k2_eval_weights_device: n_zeros exceeds the table or ld");
+    if (n_cand && (!V_dev || !wt_dev || !K_dev || ((flags & K2_D1) && !dK_dev) ||
+                   ((flags & K2_D2) && !d2K_dev)))
+        return set_error(K2_E_ARG, "k2_eval_weights_device: null pointer");
+    if (n_cand > (1u << 30)) return set_error(K2_E_ARG, "k2_eval_weights_device: too many candidates");
+    if (n_cand == 0) return K2_OK;
+    std::lock_guard<std::mutex> lk(t->mu);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipSetDevice(t->device);
+    const int n_ctiles = (int)((n_cand + 255) / 256);
+    const int n_cand_pad = n_ctiles * 256;
+    int n_splits = 0, zps = 0;
+    split_range((int)n_zeros, n_ctiles, t->tiling.target_blocks, &n_splits, &zps);
+    const int nq = 1 + ((flags & K2_D1) ? 1 : 0) + ((flags & K2_D2) ? 1 : 0);
+    const int rows = nq * (int)n_w;
+    const size_t need = (size_t)std::max(1, n_splits) * rows * n_cand_pad * sizeof(double);
+    if (e == hipSuccess && need > t->partial_bytes) {
+        (void)hipFree(t->partial);
+        t->partial = nullptr;
+        t->partial_bytes = 0;
+        e = hipMalloc((void**)&t->partial, need);
+        if (e == hipSuccess) t->partial_bytes = need;
+    }
+    if (e == hipSuccess && n_splits > 0)
+        e = launch_mw((int)n_w, flags, dim3(n_ctiles * n_splits), t, wt_dev, (int)ld,
+                      (int)n_zeros, zps, (const uint4*)V_dev, (int)n_cand, n_ctiles, t->partial,
+                      n_cand_pad, st);
+    if (e == hipSuccess) {
+        const long long cnt = (long long)rows * n_cand;
+        hipLaunchKernelGGL(k2_multiw_reduce_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256),
+                           0, st, t->partial, n_splits, (int)n_w, nq, (flags & K2_D1) ? 1 : 0,
+                           (int)n_cand, n_cand_pad, K_dev, dK_dev, d2K_dev);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess)
+        return set_error(K2_E_HIP, std::string("k2_eval_weights_device: ") + hipGetErrorString(e));
+    return K2_OK;
+}

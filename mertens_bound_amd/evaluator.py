@@ -88,6 +88,9 @@ def _declare(L):
     L.k2_enclose_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                     ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
     L.k2_exp_neg_host.argtypes = [ctypes.c_uint32, _f64p, _f64p]
+    L.k2_eval_weights_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp,
+                                         _vp, _vp, _vp]
     L.k2_polish_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
     L.k2_mixed_head.argtypes = [_vp, ctypes.c_double, ctypes.c_uint32,
                                 ctypes.POINTER(ctypes.c_uint32)]
@@ -283,6 +286,24 @@ class K2Table:
                                                th, val.data_ptr(), best.data_ptr(),
                                                ok.data_ptr(), st), "k2_screen_device")
         return val, best, ok
+
+    def eval_weights_device(self, V, wt, n_zeros=None, flags=VALUE, stream=None):
+        """K_m = sum_j wt[m, j] 2 Re(w_j e^{i g_j v}) for the rows of wt ((n_w, >= n_zeros)
+        float64 CUDA tensor, n_w <= 8), sharing the phase and sin/cos.  Returns (K, dK, d2K),
+        each (n_w, n) or None where not requested."""
+        import torch
+        n = V.shape[0]
+        wt = wt.to(dtype=torch.float64).contiguous()
+        n_w, ld = wt.shape
+        mk = lambda f: (torch.empty((n_w, n), dtype=torch.float64, device=V.device)
+                        if flags & f else None)
+        K, dK, d2K = mk(VALUE | D1 | D2), mk(D1), mk(D2)
+        st = stream if stream is not None else torch.cuda.current_stream(V.device).cuda_stream
+        _check(load_library().k2_eval_weights_device(
+            self._h, n, V.data_ptr(), n_w, wt.data_ptr(), ld, int(n_zeros or 0),
+            int(flags) | VALUE, K.data_ptr(), dK.data_ptr() if dK is not None else None,
+            d2K.data_ptr() if d2K is not None else None, st), "k2_eval_weights_device")
+        return K, dK, d2K
 
     def enclose_device(self, V, k, n_zeros=None, d1=True, stream=None):
         """Enclosure mode (k2_enclose_device): the exact K2 (and K2') over the first n_zeros

@@ -207,3 +207,17 @@ def scan(table: ev.K2Table, V0: torch.Tensor, k: float, half_width: float = 0.25
         have = torch.ones_like(have)
     best_V = ag.offset_points(V0, best_m.to(torch.float64) * step)
     return best_h, best_V, best_m
+
+
+def reference_weight_vectors(gamma: np.ndarray, n_trunc: int = 10_000):
+    """The weight vectors of search/eval_tech1_600k.py:243-250, as rows of an array: the JP
+    kernel (1 - u) cos(pi u) + sin(pi u) / pi with u = gamma / gamma_max, the Gaussians at
+    k = 1.5e-10 and 1e-10, raw (all ones) and the first n_trunc zeros only.  Returns
+    (names, array (5, n))."""
+    g = np.asarray(gamma, dtype=np.float64)
+    u = g / g[-1]
+    trunc = np.zeros_like(g)
+    trunc[:n_trunc] = 1.0
+    rows = [(1.0 - u) * np.cos(np.pi * u) + np.sin(np.pi * u) / np.pi,
+            np.exp(-1.5e-10 * g * g), np.exp(-1.0e-10 * g * g), np.ones_like(g), trunc]
+    return ["jp", "g15", "g10", "raw", "trunc"], np.stack(rows)
