@@ -18,6 +18,8 @@
   exactly here instead of through degree-16 Taylor moments), then 4 Newton steps for the 25
   largest peaks.
 * :func:`scan` -- search/polish.py:51-73: the best |h| on the grid y0 + m step, |m| <= half/step.
+* :func:`polish_with_weights` -- polish_peaks_with_weights (search/path1_eval_fast.cpp:58-143) for
+  several real weight vectors at once, as search/eval_tech1_600k.py:252-271 runs it per vector.
 """
 from __future__ import annotations
 
@@ -221,3 +223,58 @@ def reference_weight_vectors(gamma: np.ndarray, n_trunc: int = 10_000):
     rows = [(1.0 - u) * np.cos(np.pi * u) + np.sin(np.pi * u) / np.pi,
             np.exp(-1.5e-10 * g * g), np.exp(-1.0e-10 * g * g), np.ones_like(g), trunc]
     return ["jp", "g15", "g10", "raw", "trunc"], np.stack(rows)
+
+
+def polish_with_weights(table: ev.K2Table, V: torch.Tensor, wt, half_width: float = 2.5e-4,
+                        steps: int = 40, grid_zeros: int = 4096, newton_steps: int = 6,
+                        clamp: float = 1e-4, n_zeros=None):
+    """search/path1_eval_fast.cpp:58-143 for each row of wt (weight vectors over the zeros,
+    n_w <= 8) on exact points: h0 over the first n_zeros zeros and the (2 steps + 1)-point grid
+    over the first grid_zeros, both for all rows in one pass each (k2_eval_weights_device); a
+    grid point replaces the start only when strictly larger in |h| (h0 included, scan order);
+    then per row Newton on K'/K'' (clamp, stop once |step| < 1e-13 or |K''| <= 1e-12) with the
+    offset accumulated in FP64 like the reference's ddy, and the larger of |h0| and |h_final|
+    kept.  Returns (h_unpol, h_pol, V_pol, ddy): (n_w, n), (n_w, n), (n_w, n, 2), (n_w, n)."""
+    n = V.shape[0]
+    nz = table.n if n_zeros is None else int(n_zeros)
+    wt = torch.as_tensor(np.asarray(wt, dtype=np.float64)).to(V.device)
+    if wt.dim() == 1:
+        wt = wt[None, :]
+    n_w = wt.shape[0]
+    h0 = table.eval_weights_device(V, wt, n_zeros=nz, flags=ev.VALUE)[0].clone()
+    best_abs = h0.abs()
+    best_d = torch.zeros((n_w, n), dtype=torch.float64, device=V.device)
+    if steps > 0:
+        mm = torch.arange(-steps, steps + 1, device=V.device, dtype=torch.float64)
+        dd = (half_width * mm) / steps
+        hs = table.eval_weights_device(shifted(V, dd), wt, n_zeros=min(grid_zeros, nz),
+                                       flags=ev.VALUE)[0].view(n_w, n, 2 * steps + 1).abs()
+        for j in range(2 * steps + 1):
+            better = hs[:, :, j] > best_abs
+            best_abs = torch.where(better, hs[:, :, j], best_abs)
+            best_d = torch.where(better, dd[j].expand(n_w, n), best_d)
+    h_pol = torch.empty_like(h0)
+    ddy = torch.empty_like(h0)
+    V_pol = torch.empty((n_w, n, 2), dtype=V.dtype, device=V.device)
+    for r in range(n_w):
+        d = best_d[r].clone()
+        active = torch.ones(n, dtype=torch.bool, device=V.device)
+        row = wt[r:r + 1]
+        for _ in range(newton_steps):
+            if not bool(active.any()):
+                break
+            _, fp, fpp = table.eval_weights_device(ag.offset_points(V, d), row, n_zeros=nz,
+                                                   flags=ev.VALUE | ev.D1 | ev.D2)
+            fp, fpp = fp[0], fpp[0]
+            ok = active & (fpp.abs() > 1e-12)
+            step = torch.clamp(fp / torch.where(ok, fpp, torch.ones_like(fpp)), -clamp, clamp)
+            step = torch.where(ok, step, torch.zeros_like(step))
+            d = d - step
+            active = ok & (step.abs() >= 1e-13)
+        Vf = ag.offset_points(V, d)
+        hf = table.eval_weights_device(Vf, row, n_zeros=nz, flags=ev.VALUE)[0][0]
+        keep = h0[r].abs() > hf.abs()
+        h_pol[r] = torch.where(keep, h0[r], hf)
+        ddy[r] = torch.where(keep, torch.zeros_like(d), d)
+        V_pol[r] = torch.where(keep[:, None], V, Vf)
+    return h0, h_pol, V_pol, ddy

@@ -65,3 +65,27 @@ def test_weight_argument_errors(setup):
     K, _, _ = t.eval_weights_device(V, torch.ones((2, tb.n), dtype=torch.float64, device="cuda"),
                                     n_zeros=100)
     assert K.shape == (2, 8)
+
+
+def test_polish_with_weights_vs_reference_polish(setup, oracle_lib):
+    """The reference's per-vector polish (restated polish_peaks_with_weights on compute_peak_AB
+    rows of the same table, reference-format words, dy0 = 0) for the JP-kernel and raw vectors,
+    against the fused-grid GPU pipeline on exact points."""
+    from mertens_bound_amd import formats
+    tb, t, torch = setup
+    names, wt = pipeline.reference_weight_vectors(tb.gamma, n_trunc=1000)
+    wt = wt[[0, 3]]
+    V = synth.make_candidates(16, 3.4e18, 3.6e18, 15)
+    h0, hp, Vp, ddy = pipeline.polish_with_weights(t, ev.v_tensor(V), wt)
+    rw = formats.reference_words(tb)
+    xs, dy = formats.candidates_to_x_dy(V)
+    xlo, xhi = formats.x_words(xs)
+    A, B = oracle_lib.ref_compute_peak_AB(xlo, xhi, dy, rw["alpha"], rw["gamma"], rw["ghi_lo"],
+                                          rw["ghi_hi"], rw["glo_lo"], rw["glo_hi"], rw["psi_lo"],
+                                          rw["psi_hi"])
+    for r in range(2):
+        hu_r, hp_r, dd_r = oracle_lib.ref_polish_peaks_with_weights(A, B, tb.gamma, wt[r],
+                                                                    2.5e-4, 40)
+        np.testing.assert_allclose(h0[r].cpu().numpy(), hu_r, atol=1e-11)
+        np.testing.assert_allclose(hp[r].cpu().numpy(), hp_r, atol=1e-9)
+        np.testing.assert_allclose(ddy[r].cpu().numpy(), dd_r, atol=1e-9)
