@@ -9,7 +9,8 @@ counts only with |dK2| <= 1e-9 against the multiprecision oracle, checked on the
 
 Other workloads (--workload): mixed (FP32 form of the same), batched (512 uneven requests in one
 persistent launch, graph replay), polish (4,096 candidates, 8 Newton steps), small (10,000 zeros x
-1,024 candidates), enclose (the default workload with rigorous radii), copy (a 2 GiB device copy, the byte-counter calibration for rocprofv3 --pmc).
+1,024 candidates), enclose (the default workload with rigorous radii), weights (five weight
+vectors in one pass), copy (a 2 GiB device copy, the byte-counter calibration for rocprofv3 --pmc).
 
 Prints one JSON line.
 """
@@ -381,6 +382,37 @@ def run_enclose(args):
     return res
 
 
+def run_weights(args):
+    """The reference's five weight vectors (JP kernel, Gaussians at 1.5e-10 and 1e-10, raw,
+    first 10,000 zeros) for the metric's candidates and table in one pass
+    (k2_eval_weights_device), against five one-vector passes of the same kernel."""
+    import torch
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import pipeline, synth
+    c = synth.CONFIG2
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    n_cand = args.n_cand or c["n_cand"]
+    V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
+    torch.cuda.set_device(0)
+    t = ev.K2Table(tb)
+    t.set_tiling(target_blocks=3072)
+    Vd = ev.v_tensor(V)
+    names, wt = pipeline.reference_weight_vectors(tb.gamma)
+    wtd = torch.tensor(wt, device="cuda")
+    ms, wall = timed(lambda: t.eval_weights_device(Vd, wtd, flags=ev.VALUE), args.steps,
+                     args.warmup, torch)
+    ms1, _ = timed(lambda: [t.eval_weights_device(Vd, wtd[m:m + 1], flags=ev.VALUE)
+                            for m in range(len(names))], max(1, args.steps // 2), 1, torch)
+    evals = float(n_cand) * tb.n * len(names)
+    res = base_record(args, "candidate-zero-weight evals/sec (5 weight vectors, 2.33M zeros)",
+                      evals / (ms * 1e-3), "candidate-zero-vector evals/s", ms, "f64",
+                      {"workload": f"K2 for {len(names)} weight vectors ({', '.join(names)}), "
+                                   f"{n_cand:,} candidates x {tb.n:,} zeros", "form": "weights"})
+    res["vs_separate_passes"] = {"ms": ms1, "speedup": ms1 / ms}
+    res["wall_s"] = wall
+    return res
+
+
 def run_copy(args):
     import torch
     n = 1 << 28                                  # 2 GiB of float64
@@ -397,7 +429,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["metric", "mixed", "batched", "polish", "small",
-                                           "enclose", "copy"], default="metric")
+                                           "enclose", "weights", "copy"], default="metric")
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
     ap.add_argument("--n-cand", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -420,6 +452,8 @@ def main():
         res = run_polish(args)
     elif args.workload == "enclose":
         res = run_enclose(args)
+    elif args.workload == "weights":
+        res = run_weights(args)
     else:
         res = run_copy(args)
     print(json.dumps(res))
