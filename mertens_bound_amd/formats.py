@@ -221,6 +221,35 @@ def read_peaks_csv(path):
     return out
 
 
+# ---- candidates beyond the 64.64 range ----------------------------------------------------
+
+def split_far_candidates(vals):
+    """Exact values V = v 2^64 (Python ints, v may exceed 2^64) -> {q: (indices, words)} with
+    v = 2^64 q + v', 0 <= v' < 2^64, words the 64.64 words of v'.  Evaluate group q against
+    rotate_table(table, q)."""
+    groups = {}
+    for i, x in enumerate(vals):
+        q, r = divmod(int(x), 1 << 128)
+        if q < 0:
+            raise ValueError("negative candidate")
+        groups.setdefault(q, ([], []))
+        groups[q][0].append(i)
+        groups[q][1].append(r)
+    return {q: (np.array(ix), synth.v_from_int(rs)) for q, (ix, rs) in groups.items()}
+
+
+def rotate_table(table: synth.ZeroTable, q: int) -> synth.ZeroTable:
+    """The table for candidates shifted down by 2^64 q: e^{i g (2^64 q + v')} = e^{i g 2^64 q}
+    e^{i g v'}, and g 2^64 q / 2pi = G q / 2^46 exactly, so w_j becomes w_j e^{2 pi i f_j} with
+    f_j = ((G_j q) mod 2^46) / 2^46 (a 46-bit fraction, exact in FP64).  K2 and its derivatives
+    at v are those of the rotated table at v', up to the FP64 rounding of the rotated weights."""
+    if q == 0:
+        return table
+    f = np.array([((_u128(lo, hi) * q) & ((1 << 46) - 1)) / 2.0 ** 46 for lo, hi in table.G])
+    rot = np.exp(2j * np.pi * f)
+    return synth.ZeroTable(G=table.G, gamma=table.gamma, w=table.w * rot)
+
+
 # ---- zero chunk text files and the .npz table ---------------------------------------------
 
 def read_zero_chunks(paths, gamma_max=None):

@@ -152,7 +152,8 @@ def candidates_from_x_dy(x, dy0) -> np.ndarray:
 
     The reference evaluates at y = x / 2^30 + dy0 (search/eval_tech1_600k.py:216-218,
     legacy/superseded/certify_peak2_600k.py:175-177).  A 64.64 word holds 0 <= y < 2^64
-    (1.8447e19); candidates outside raise ValueError rather than wrap.
+    (1.8447e19); candidates outside raise ValueError rather than wrap (formats.split_far_candidates
+    and formats.rotate_table evaluate larger y).
     """
     out = []
     for xi, d in zip(x, dy0):

@@ -140,3 +140,39 @@ def test_npz_round_trip(tmp_path):
     np.testing.assert_array_equal(back.G, tb.G)
     np.testing.assert_array_equal(back.w, tb.w)
     np.testing.assert_array_equal(back.gamma, tb.gamma)
+
+
+def _mp_k2(G, w, gamma, Vint, k):
+    """K2 from exact big-integer phases (frac(G V / 2^174) turns) and mpmath at 40 digits."""
+    import mpmath
+    out = []
+    with mpmath.workdps(40):
+        for V in Vint:
+            s = mpmath.mpf(0)
+            for (lo, hi), wj, g in zip(G, w, gamma):
+                Gi = (int(hi) << 64) | int(lo)
+                fr = mpmath.mpf((Gi * V) % (1 << 174)) / mpmath.mpf(2) ** 174
+                th = 2 * mpmath.pi * fr
+                gg = 2 * mpmath.pi * mpmath.mpf(Gi) / mpmath.mpf(2) ** 110
+                a = 2 * mpmath.exp(-mpmath.mpf(k) * gg * gg)
+                s += a * (mpmath.mpf(wj.real) * mpmath.cos(th) - mpmath.mpf(wj.imag) * mpmath.sin(th))
+            out.append(float(s))
+    return np.array(out)
+
+
+def test_far_candidates_by_rotated_table(oracle_lib):
+    """y beyond 2^64 (the reference's peak at 1.8789e19 and its search bound 1.957e19): the
+    rotated table at v' = v - 2^64 q reproduces K2 from exact big-integer phases."""
+    tb = synth.make_zero_table(150, 21)
+    rows = formats.read_peaks_csv(os.path.join(GOLDEN, "subrecord_peaks_summary.csv"))
+    far = [r for r in rows if r["x"] >= 1 << 94][0]
+    vals = [(far["x"] << 34) + int(round(far["dy"] * 2.0 ** 64)),
+            int(1.957e19) << 64, (int(1.2e19) << 64) + 12345]
+    k = 3e-9
+    want = _mp_k2(tb.G, tb.w, tb.gamma, vals, k)
+    got = np.zeros(len(vals))
+    groups = formats.split_far_candidates(vals)
+    assert set(groups) == {0, 1}
+    for q, (ix, words) in groups.items():
+        got[ix] = oracle_lib.eval_quad(formats.rotate_table(tb, q), words, k)[0]
+    assert np.max(np.abs(got - want)) < 1e-14

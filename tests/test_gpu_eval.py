@@ -204,3 +204,29 @@ def test_z_tile_path_matches_c_tiles(cfg1, n_cand):
     assert np.max(np.abs(dKz - dKc)) / s1 <= 1e-13
     assert np.max(np.abs(d2Kz - d2Kc)) / s2 <= 1e-13
     assert np.max(np.abs(Kzm - g["K"][:n_cand])) <= 1e-9
+
+
+def test_far_candidates_rotated_table():
+    """Candidates above y = 2^64 evaluate exactly through formats.rotate_table: the GPU at
+    v' = v - 2^64 q on the rotated table against exact big-integer phases with mpmath."""
+    import mpmath
+    from mertens_bound_amd import formats
+    tb = synth.make_zero_table(120, 22)
+    vals = [(int(1.8788774099540933515e19) << 64) + 7, int(1.957e19) << 64,
+            (int(3.5e18) << 64) + 99]
+    k = 3e-9
+    want = []
+    with mpmath.workdps(40):
+        for V in vals:
+            s = mpmath.mpf(0)
+            for (lo, hi), wj in zip(tb.G, tb.w):
+                Gi = (int(hi) << 64) | int(lo)
+                th = 2 * mpmath.pi * mpmath.mpf((Gi * V) % (1 << 174)) / mpmath.mpf(2) ** 174
+                gg = 2 * mpmath.pi * mpmath.mpf(Gi) / mpmath.mpf(2) ** 110
+                s += 2 * mpmath.exp(-k * gg * gg) * (mpmath.mpf(wj.real) * mpmath.cos(th)
+                                                     - mpmath.mpf(wj.imag) * mpmath.sin(th))
+            want.append(float(s))
+    got = np.zeros(len(vals))
+    for q, (ix, words) in formats.split_far_candidates(vals).items():
+        got[ix] = ev.K2Table(formats.rotate_table(tb, q)).eval(words, k, flags=ev.VALUE)[0]
+    assert np.max(np.abs(got - np.array(want))) < 1e-13
