@@ -230,3 +230,35 @@ def test_far_candidates_rotated_table():
     for q, (ix, words) in formats.split_far_candidates(vals).items():
         got[ix] = ev.K2Table(formats.rotate_table(tb, q)).eval(words, k, flags=ev.VALUE)[0]
     assert np.max(np.abs(got - np.array(want))) < 1e-13
+
+
+def test_handles_from_threads_and_caller_stream(cfg1):
+    """Distinct table handles used from two threads at once give the sequential results, and
+    the device entry points run on the caller's stream (a side stream here)."""
+    import threading
+
+    import torch
+    tb, t, V, k = cfg1
+    want = t.eval(V, k, flags=ALL)
+    tables = [ev.K2Table(tb), ev.K2Table(tb)]
+    got = [None, None]
+
+    def work(i):
+        for _ in range(3):
+            got[i] = tables[i].eval(V, k, flags=ALL)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    for g in got:
+        for a, b in zip(g, want):
+            assert np.array_equal(a, b)
+    side = torch.cuda.Stream()
+    Vd = ev.v_tensor(V)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        K, dK, d2K = t.eval_device(Vd, k, flags=ALL, stream=side.cuda_stream)
+    side.synchronize()
+    assert np.array_equal(K.cpu().numpy(), want[0])
