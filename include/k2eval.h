@@ -178,7 +178,7 @@ int k2_mixed_head(k2_table *t, double k, uint32_t n_zeros, uint32_t *head);
  * (stage_zeros[s] zeros, increasing, <= 1024) continues only if its value reaches
  * stage_thresholds[s].  val_dev: [n_cand][4] stage values (-1 where not reached), best_shift_dev:
  * argmax m of the last completed stage, pass_dev: 1 if all stages passed.  FP32 sums after the
- * exact top-32-bit phase.  Replaces the leaf screen of search/gpu_mertens_stream.cu:186-250
+ * integer phase reduction of the FP32 form.  Replaces the leaf screen of search/gpu_mertens_stream.cu:186-250
  * (64/128/256 zeros, 7 shifts of 0.0014, thresholds 0.78/0.855/0.892). */
 int k2_screen_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double k,
                      double shift_step, int n_shift, int n_stages, const int32_t *stage_zeros,
@@ -186,13 +186,14 @@ int k2_screen_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double
                      int32_t *pass_dev, void *stream);
 
 /* Debug: phase words floor(G_j V_c / 2^110) mod 2^64 for zeros [z0, z0+nz) (host buffers,
- * out[c*nz + j]).  mode 0 = evaluator path, 1 = schoolbook product, 2 = the FP32 form's top-32
- * word in the high half. */
+ * out[c*nz + j]).  mode 0 = evaluator path, 1 = schoolbook product, 2 = the FP32 form's phase
+ * window: bits [128,174) of the product's columns 3..5, i.e. floor(W / 2^18) or up to 4 below
+ * it (mod 2^46). */
 int k2_phase_words(k2_table *t, uint32_t n_cand, const uint64_t *V, uint32_t z0, uint32_t nz,
                    int mode, uint64_t *out);
 
-/* Debug: the evaluator's sin/cos of 2 pi W / 2^64 (mode 0, FP64) or of the top 32 bits
- * (mode 1, FP32 widened); out[2i] = sin, out[2i+1] = cos.  Host buffers. */
+/* Debug: the evaluator's sin/cos of 2 pi W / 2^64 (mode 0, FP64) or the FP32 form's of the
+ * window floor(W / 2^18) (mode 1, widened); out[2i] = sin, out[2i+1] = cos.  Host buffers. */
 int k2_sincos_words(uint32_t n, const uint64_t *W, int mode, double *out);
 
 /* Timing of the evaluation kernels on the stream they run on (HIP events around each launch).

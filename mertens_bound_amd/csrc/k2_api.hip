@@ -19,32 +19,32 @@ using namespace k2impl;
 #define K2_VERSION 1
 #define K2_PAD 4096            // zero arrays are padded to a multiple of this
 #define K2_MAX_WEIGHT_SETS 4   // cached per-k weight arrays per table
-#define K2_TAB_N K2_TAB        // entries of the (cos, sin) table the kernels stage in LDS
 
 static int fail(int code, const std::string& msg);
 
-void k2impl::sincos_tables(int n, k2dev::Tab64* t64, k2dev::Tab32* t32) {
+void k2impl::sincos_tables(k2dev::Tab64* t64, k2dev::Tab32* t32) {
     const long double two_pi = 6.283185307179586476925286766559005768L;
     const long double c64 = two_pi / 18446744073709551616.0L;   // 2 pi / 2^64
-    const long double c32 = two_pi / 4294967296.0L;             // 2 pi / 2^32
-    for (int k = 0; k < n; ++k) {
-        const long double a = two_pi * (long double)k / (long double)n;
+    for (int k = 0; k < K2_TAB; ++k) {
+        const long double a = two_pi * (long double)k / (long double)K2_TAB;
         const long double c = cosl(a), s = sinl(a);
         t64[k] = k2dev::Tab64{(double)c, (double)s, (double)(c * c64), (double)(s * c64)};
-        t32[k] = k2dev::Tab32{(float)c, (float)s, (float)(c * c32), (float)(s * c32)};
+    }
+    for (int k = 0; k < K2_TAB32; ++k) {
+        const long double a = two_pi * (long double)k / (long double)K2_TAB32;
+        t32[k] = k2dev::Tab32{(float)cosl(a), (float)sinl(a)};
     }
 }
 
 int k2impl::upload_tables(k2dev::Tab64** t64, k2dev::Tab32** t32, hipStream_t st) {
-    std::vector<k2dev::Tab64> h64(K2_TAB_N);
-    std::vector<k2dev::Tab32> h32(K2_TAB_N);
-    sincos_tables(K2_TAB_N, h64.data(), h32.data());
-    hipError_t e = hipMalloc((void**)t64, K2_TAB_N * sizeof(k2dev::Tab64));
-    if (e == hipSuccess) e = hipMalloc((void**)t32, K2_TAB_N * sizeof(k2dev::Tab32));
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(*t64, h64.data(), K2_TAB_N * sizeof(k2dev::Tab64), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(*t32, h32.data(), K2_TAB_N * sizeof(k2dev::Tab32), hipMemcpyHostToDevice, st);
+    std::vector<k2dev::Tab64> h64(K2_TAB);
+    std::vector<k2dev::Tab32> h32(K2_TAB32);
+    sincos_tables(h64.data(), h32.data());
+    const size_t b64 = K2_TAB * sizeof(k2dev::Tab64), b32 = K2_TAB32 * sizeof(k2dev::Tab32);
+    hipError_t e = hipMalloc((void**)t64, b64);
+    if (e == hipSuccess) e = hipMalloc((void**)t32, b32);
+    if (e == hipSuccess) e = hipMemcpyAsync(*t64, h64.data(), b64, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(*t32, h32.data(), b32, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e == hipSuccess ? K2_OK : fail(K2_E_HIP, std::string("sin/cos table: ") + hipGetErrorString(e));
 }

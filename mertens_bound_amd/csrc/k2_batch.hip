@@ -65,13 +65,15 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
     auto& sm32 = *reinterpret_cast<CTileSmem32<FL>*>(sm_raw);
     auto& zsm = *reinterpret_cast<ZTileSmem<CZ>*>(sm_raw);
     int& s_item = *reinterpret_cast<int*>(sm_raw);
+    // the FP32 items use a 1,024-point table (every other entry of the global one), which keeps
+    // the LDS per workgroup at three workgroups per CU
+    constexpr int TB = K2_TAB32_BITS - 1;
     __shared__ Tab64 s_tab64[K2_TAB];
-    __shared__ Tab32 s_tab32[ALLOW32 ? K2_TAB : 1];
+    __shared__ Tab32 s_tab32[ALLOW32 ? (1 << TB) : 1];
     const int tid = threadIdx.x;
-    for (int i = tid; i < K2_TAB; i += K2_WG) {
-        s_tab64[i] = tab64[i];
-        if (ALLOW32) s_tab32[i] = tab32[i];
-    }
+    for (int i = tid; i < K2_TAB; i += K2_WG) s_tab64[i] = tab64[i];
+    if (ALLOW32)
+        for (int i = tid; i < (1 << TB); i += K2_WG) s_tab32[i] = tab32[i << (K2_TAB32_BITS - TB)];
     __syncthreads();
     constexpr int CW = K2_WG * NC;
     for (;;) {
@@ -88,8 +90,8 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
             double* out = partial + I.poff;
             if (f32) {
                 if constexpr (ALLOW32)
-                    ztile<CZ, FL, true>(zv, I.z0, I.z1, cand, I.c_first, I.c_count, out, zsm,
-                                        s_tab64, s_tab32);
+                    ztile<CZ, FL, true, TB>(zv, I.z0, I.z1, cand, I.c_first, I.c_count, out,
+                                            zsm, s_tab64, s_tab32);
             } else {
                 ztile<CZ, FL, false>(zv, I.z0, I.z1, cand, I.c_first, I.c_count, out, zsm, s_tab64,
                                      s_tab32);
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
                 Acc<true> acc[NC];
 #pragma unroll
                 for (int i = 0; i < NC; ++i) acc[i].zero();
-                ctile_f32<NC, FL>(zv, I.z0, I.z1, v, acc, sm32, s_tab32);
+                ctile_f32<NC, FL, K2_WG, TB>(zv, I.z0, I.z1, v, acc, sm32, s_tab32);
 #pragma unroll
                 for (int i = 0; i < NC; ++i)
 #pragma unroll

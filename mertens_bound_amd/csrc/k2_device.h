@@ -104,28 +104,42 @@ __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
     return W;
 }
 
-// top-32-bit phase word floor(G V / 2^142) mod 2^32 (for the FP32 form), exact.
-// Columns <= 2 sum to < 2^130 and only matter through a carry into bit 142, which needs bits
-// [130,142) of the rest to be all ones.
-__device__ __forceinline__ uint32_t phase_word_hi32(Limbs4 g, Limbs4 v) {
+// ---- FP32 form: the phase window ------------------------------------------------------------
+// The FP32 kernels need the phase to about 2^-40 turn, so they form only the product columns
+// 3..5 (bit weights 2^96..2^160).  Columns 0..2 sum to less than 2^130 and move the phase
+// (bits [0,174) of G V, in units of 2^-174 turn) by less than 2^-44 turn; the integer turns are
+// still removed exactly, because the window keeps bits [128,174) of the partial product, which is
+// the phase mod 1.  hi:lo = bits [160,192) : [128,160) of the partial product plus 2^(173-TB),
+// half a step of a 2^TB-point table, so that the top TB phase bits of the window round to the
+// nearest table point.  Only hi bits [0,24) are formed (the column-5 terms are 24x24-bit
+// products, exact mod 2^24).
+// hs = window_half<TB>(), taken once outside the loop: it is the addend of the first column-4
+// multiply-add and stays in a VGPR pair.
+template <int TB>
+__device__ __forceinline__ uint64_t window_half() {
+    static_assert(TB >= 9 && TB <= 11, "table of 512..2048 points");
+    uint64_t h = 1ull << (45 - TB);   // 2^(173-TB) at weight 2^128
+    asm volatile("" : "+v"(h));
+    return h;
+}
+
+template <int TB>
+__device__ __forceinline__ void phase_window(Limbs4 g, Limbs4 v, uint64_t hs, uint32_t& hi,
+                                             uint32_t& lo) {
     uint64_t s3 = mul32(g.x0, v.x3);
     uint32_t k3;
     s3 = mad_carry0(g.x1, v.x2, s3, k3);
     s3 = mad_carry(g.x2, v.x1, s3, k3);
     s3 = mad_carry(g.x3, v.x0, s3, k3);
-    uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
-    s4 = mul32(g.x1, v.x3) + s4;
+    uint64_t s4 = mul32(g.x1, v.x3) + hs;
     s4 = mul32(g.x2, v.x2) + s4;
     s4 = mul32(g.x3, v.x1) + s4;
+    s4 += (s3 >> 32) | ((uint64_t)k3 << 32);
     uint32_t s5 = (uint32_t)(s4 >> 32);
     s5 = mad24(g.x2, v.x3, s5);
     s5 = mad24(g.x3, v.x2, s5);
-    const uint32_t s4lo = (uint32_t)s4;
-    uint32_t whi = __builtin_amdgcn_alignbit(s5, s4lo, 14);
-    // bits [130,142) of the partial sum are bits [2,14) of s4lo
-    if (__builtin_expect(((s4lo >> 2) & 0xFFFu) == 0xFFFu, 0))
-        whi = (uint32_t)(phase_word_exact(g, v) >> 32);
-    return whi;
+    hi = s5;
+    lo = (uint32_t)s4;
 }
 
 // ---- FP64 sin/cos of 2 pi W / 2^64 ---------------------------------------------------------
@@ -171,9 +185,13 @@ struct Tab64 {
     double cc, sc;    // the same times 2 pi / 2^64
 };
 
+// FP32 table: cos, sin of 2 pi k / K2_TAB32 (8 bytes, so that the byte offset of entry k is
+// window bits shifted in place)
+#define K2_TAB32_BITS 11
+#define K2_TAB32 (1 << K2_TAB32_BITS)
+
 struct Tab32 {
-    float c, s;       // cos, sin of 2 pi k / K2_TAB
-    float cc, sc;     // the same times 2 pi / 2^32
+    float c, s;
 };
 
 // the remainder polynomials' coefficients (x = u 2 pi / 2^64 folded in); a loop that calls
@@ -206,18 +224,28 @@ __device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const T
     S = fma(t.s, cp, t.cc * sp);
 }
 
-// FP32 form: W = top 32 bits of the phase word, |r| <= 2^22 (exact in FP32); x - x^3/6 and
-// 1 - x^2/2 are exact to FP32 rounding at |x| <= pi/512 (next terms < 2e-11 relative)
-__device__ __forceinline__ void sincos_tab32(uint32_t W, const Tab32* tab, float& S, float& C) {
-    const uint32_t k = (W + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT;
-    const int32_t r = ((int32_t)(W << (32 - K2_TAB_SHIFT))) >> (32 - K2_TAB_SHIFT);
-    const float u = (float)r;
-    const float z = u * u;
-    const float sp = fmaf(u * z, -3.566882158968672e-19f, u);
-    const float cp = fmaf(z, -1.0700646218412075e-18f, 1.0f);
-    const Tab32 t = tab[k];
-    C = fmaf(t.c, cp, -t.sc * sp);
-    S = fmaf(t.s, cp, t.cc * sp);
+// FP32 sin/cos of the window's phase with a 2^TB-point table (every 2^(11-TB)-th entry of the
+// K2_TAB32-point table, as staged in LDS).  Window bits [46-TB,46) are the index of the nearest
+// point (hi bits [14-TB,14)); the next 32 bits R give the remainder
+// x = 2 pi (R - 2^31) 2^-(32+TB), |x| <= pi/2^TB, with R rounded to FP32 (error <= 2^-(24+TB)
+// turn).  cos x = 1 - x^2/2 and sin x = x hold to x^4/24 and |x|^3/6 (<= 6.0e-10 at TB = 11,
+// 4.8e-9 at TB = 10, against FP32 rounding of 6e-8); the x^3 error is odd in x, so it averages
+// out over the zeros instead of biasing the sum.  cos(t + x) = c - x (s + c x/2) and
+// sin(t + x) = s + x (c - s x/2) take four FMAs.
+template <int TB>
+__device__ __forceinline__ void sincos_window(uint32_t hi, uint32_t lo, const Tab32* tab,
+                                              float& S, float& C) {
+    constexpr uint32_t mask = ((1u << TB) - 1u) << 3;
+    const uint32_t off = (TB == 11 ? hi : hi >> (11 - TB)) & mask;   // 8 * index
+    const Tab32 t = *reinterpret_cast<const Tab32*>(reinterpret_cast<const char*>(tab) + off);
+    const uint32_t R = __builtin_amdgcn_alignbit(hi, lo, 14 - TB);
+    constexpr float step = 6.28318530717958647692f / (float)(1ull << (32 + TB));
+    constexpr float half = step * 2147483648.0f;
+    const float u = (float)R;
+    const float x = fmaf(u, step, -half);
+    const float h = fmaf(u, 0.5f * step, -0.5f * half);   // x / 2
+    C = fmaf(-x, fmaf(h, t.c, t.s), t.c);
+    S = fmaf(x, fmaf(-h, t.s, t.c), t.s);
 }
 
 // ---- general FP64 sincos (|x| < 2^20 pi/2) for the reference-format entry points -----------

@@ -54,9 +54,10 @@ struct LaunchArgs {
     const k2dev::Tab32* tab32 = nullptr;
 };
 
-// host-side sin/cos tables of 2 pi k / n (plain and scaled entries), rounded from long double
-void sincos_tables(int n, k2dev::Tab64* t64, k2dev::Tab32* t32);
-// device copies of those tables (K2_TAB entries)
+// host-side sin/cos tables, rounded from long double: K2_TAB FP64 entries of 2 pi k / K2_TAB
+// (plain and scaled) and K2_TAB32 FP32 entries of 2 pi k / K2_TAB32
+void sincos_tables(k2dev::Tab64* t64, k2dev::Tab32* t32);
+// device copies of those tables
 int upload_tables(k2dev::Tab64** t64, k2dev::Tab32** t32, hipStream_t st);
 // records the calling thread's last error (k2_last_error) and returns code
 int set_error(int code, const std::string& msg);

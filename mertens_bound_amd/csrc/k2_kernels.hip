@@ -66,8 +66,8 @@ __global__ __launch_bounds__(256) void k2_abs_blocks_kernel(const double2* __res
 
 // ---- one-launch evaluation: C tiles over (split, candidate tile) workgroups -----------------------
 // Split s covers zeros [z_begin + s*zps, ...) up to z_end and writes partial row row_base + s.
-// F32 selects the precision form (exact top-32-bit phase, FP32 sin/cos and weights, compensated
-// FP32 sums); the mixed form launches an FP64 head and an FP32 tail side by side.
+// F32 selects the precision form (phase window, FP32 sin/cos and weights, compensated FP32
+// sums); the mixed form launches an FP64 head and an FP32 tail side by side.
 // NC <= 2 keeps to 128 VGPRs so that four workgroups share a CU; NT = 512 puts twice the waves
 // on one staged tile and one sin/cos table (LDS is what limits the workgroups per CU), walks the
 // zeros without unrolling and stays within 80 VGPRs (six waves per SIMD)
@@ -77,13 +77,14 @@ __global__ __launch_bounds__(NT, (NT == 512 ? 6 : (NC <= 2 ? 4 : 2))) void k2_ev
     int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
     const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
     __shared__ typename std::conditional<F32, CTileSmem32<FL>, CTileSmem64<FL>>::type sm;
-    __shared__ typename std::conditional<F32, Tab32, Tab64>::type s_tab[K2_TAB];
+    constexpr int NTAB = F32 ? K2_TAB32 : K2_TAB;
+    __shared__ typename std::conditional<F32, Tab32, Tab64>::type s_tab[NTAB];
     const int L = xcd_linear_block();
     const int split = L / n_ctiles, ct = L - split * n_ctiles;
     const int z0 = z_begin + split * zps;
     const int z1 = min(z_end, z0 + zps);
     const int tid = threadIdx.x;
-    for (int i = tid; i < K2_TAB; i += NT) {
+    for (int i = tid; i < NTAB; i += NT) {
         if constexpr (F32) s_tab[i] = tab32[i];
         else s_tab[i] = tab64[i];
     }
@@ -118,15 +119,17 @@ __global__ __launch_bounds__(256, 4) void k2_eval_mixed_kernel(
     int n_cand_pad, const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
     constexpr size_t TILE = ctile_smem_bytes<FL>();
     __shared__ __align__(16) unsigned char sm_raw[TILE];
-    __shared__ Tab64 s_tab64[K2_TAB];   // the FP32 table of a tail block sits in its first half
+    __shared__ Tab64 s_tab64[K2_TAB];   // a tail block's FP32 table takes the same 16 KiB
     const int L = xcd_linear_block();
     const int split = L / n_ctiles, ct = L - split * n_ctiles;
     const int tid = threadIdx.x;
     const bool head = split < sp_head;
+    static_assert(sizeof(Tab32) * K2_TAB32 <= sizeof(Tab64) * K2_TAB, "FP32 table fits");
     Tab32* s_tab32 = reinterpret_cast<Tab32*>(s_tab64);
-    for (int i = tid; i < K2_TAB; i += K2_ZT) {
-        if (head) s_tab64[i] = tab64[i];
-        else s_tab32[i] = tab32[i];
+    if (head) {
+        for (int i = tid; i < K2_TAB; i += K2_ZT) s_tab64[i] = tab64[i];
+    } else {
+        for (int i = tid; i < K2_TAB32; i += K2_ZT) s_tab32[i] = tab32[i];
     }
     Limbs4 v[NC];
 #pragma unroll
@@ -176,8 +179,8 @@ __global__ __launch_bounds__(256) void k2_eval_z_kernel(
     const Tab32* __restrict__ tab32) {
     __shared__ ZTileSmem<K2_Z_CZ> sm;
     __shared__ Tab64 s64[F32 ? 1 : K2_TAB];
-    __shared__ Tab32 s32[F32 ? K2_TAB : 1];
-    for (int i = threadIdx.x; i < K2_TAB; i += K2_ZT) {
+    __shared__ Tab32 s32[F32 ? K2_TAB32 : 1];
+    for (int i = threadIdx.x; i < (F32 ? K2_TAB32 : K2_TAB); i += K2_ZT) {
         if constexpr (F32) s32[i] = tab32[i];
         else s64[i] = tab64[i];
     }
@@ -253,7 +256,8 @@ __global__ __launch_bounds__(256) void k2_reduce_kernel(
 }
 
 // ---- debug: phase words --------------------------------------------------------------------------
-// mode 0: fast W (skip + fix-up), 1: schoolbook W, 2: fast top-32 word (zero-extended)
+// mode 0: fast W (skip + fix-up), 1: schoolbook W, 2: the FP32 form's phase window, bits
+// [128,174) of the partial product (the phase floored to 2^-46 turn, low by at most 4 units)
 __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     const uint4* __restrict__ zg, int z0, int nz, const uint4* __restrict__ cand, int n_cand,
     int mode, uint64_t* __restrict__ out) {
@@ -265,12 +269,17 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     uint64_t W;
     if (mode == 0) W = phase_word(g, v);
     else if (mode == 1) W = phase_word_exact(g, v);
-    else W = (uint64_t)phase_word_hi32(g, v) << 32;
+    else {
+        uint32_t hi, lo;
+        phase_window<K2_TAB32_BITS>(g, v, window_half<K2_TAB32_BITS>(), hi, lo);
+        W = ((((uint64_t)hi << 32) | lo) - (1ull << (45 - K2_TAB32_BITS))) & ((1ull << 46) - 1);
+    }
     out[idx] = W;
 }
 
 // ---- debug: the evaluators' sin/cos of phase words ---------------------------------------------
-// mode 0: FP64 table path of the plain kernel, 1: FP32 table path of the mixed kernel (top 32 bits)
+// mode 0: FP64 table path of the plain kernel, 1: FP32 table path of the mixed kernel (the
+// window of W: bits [18,64) plus half a table step)
 __global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __restrict__ W,
                                                               int n, int mode,
                                                               const Tab64* __restrict__ tab64,
@@ -284,7 +293,8 @@ __global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __
         out[2 * i] = s; out[2 * i + 1] = c;
     } else {
         float s, c;
-        sincos_tab32((uint32_t)(W[i] >> 32), tab32, s, c);
+        const uint64_t win = (W[i] >> 18) + (1ull << (45 - K2_TAB32_BITS));
+        sincos_window<K2_TAB32_BITS>((uint32_t)(win >> 32), (uint32_t)win, tab32, s, c);
         out[2 * i] = s; out[2 * i + 1] = c;
     }
 }

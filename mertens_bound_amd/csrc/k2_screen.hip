@@ -4,9 +4,10 @@
 // candidate v, the sums h_m = sum_{j<n} 2 e^{-k g_j^2} Re(w_j e^{i g_j (v + m d)}) over the first
 // n zeros at 2M+1 shifts m = -M..M (reference: M = 3, d = 0.0014), the stage value max_m |h_m|,
 // and staged cutoffs: stage s continues only if the value after n_s zeros reaches thr_s
-// (reference: 64/128/256 zeros, 0.78/0.855/0.892).  Here the phase is the exact top-32-bit word
-// (not a wrapped 64-bit approximation) and sin/cos come from the FP32 LDS table (not a 12-bit
-// cos lookup); the shifted terms use one rotation per shift step, e^{i g_j d}, per zero.
+// (reference: 64/128/256 zeros, 0.78/0.855/0.892).  Here the phase is the integer-reduced window
+// of the FP32 form (to 2^-44 turn, not a wrapped 64-bit approximation) and sin/cos come from the
+// FP32 LDS table (not a 12-bit cos lookup); the shifted terms use one rotation per shift step,
+// e^{i g_j d}, per zero.
 //
 // One lane per candidate; the zeros of all stages (<= K2_SCREEN_MAX) are staged in LDS once per
 // workgroup.  A lane that fails a stage stops accumulating; its later stage values stay -1.
@@ -41,9 +42,9 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
     const Tab32* __restrict__ tab, float* __restrict__ out_val, int* __restrict__ out_shift,
     int* __restrict__ out_pass) {
     __shared__ ScreenZero sz[K2_SCREEN_MAX];
-    __shared__ Tab32 s_tab[K2_TAB];
+    __shared__ Tab32 s_tab[K2_TAB32];
     const int tid = threadIdx.x;
-    for (int i = tid; i < K2_TAB; i += blockDim.x) s_tab[i] = tab[i];
+    for (int i = tid; i < K2_TAB32; i += blockDim.x) s_tab[i] = tab[i];
     for (int j = tid; j < n_zeros; j += blockDim.x) {
         const double g = gamma[j];
         const double f = 2.0 * exp(-k * g * g);
@@ -62,13 +63,15 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
 #pragma unroll
     for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
     int j = 0, best = M, passed = 1;
+    const uint64_t hs = window_half<K2_TAB32_BITS>();
     for (int s = 0; s < n_stages; ++s) {
         const int nz = stage_nz[s];
         for (; j < nz; ++j) {
             const ScreenZero z = sz[j];
-            const uint32_t W = phase_word_hi32(Limbs4{z.g.x, z.g.y, z.g.z, z.g.w}, v);
+            uint32_t hi, lo;
+            phase_window<K2_TAB32_BITS>(Limbs4{z.g.x, z.g.y, z.g.z, z.g.w}, v, hs, hi, lo);
             float S, C;
-            sincos_tab32(W, s_tab, S, C);
+            sincos_window<K2_TAB32_BITS>(hi, lo, s_tab, S, C);
             // q = a e^{i th}
             const float qr = fmaf(z.a.x, C, -z.a.y * S);
             const float qi = fmaf(z.a.x, S, z.a.y * C);

@@ -95,12 +95,22 @@ __device__ __forceinline__ void accumulate(AccDD& A, double S, double C, double2
     if (FL & K2_D1) two_sum_acc(A.h1, A.l1, -fma(b.x, S, b.y * C));
 }
 
+// the FP32 terms (K, K', K'') of one zero; the unrequested ones are left 0
+template <int FL>
+__device__ __forceinline__ float3 terms(float S, float C, float2 a, float2 b, float2 e) {
+    float3 t = make_float3(fmaf(a.x, C, -a.y * S), 0.f, 0.f);
+    if (FL & K2_D1) t.y = -fmaf(b.x, S, b.y * C);
+    if (FL & K2_D2) t.z = fmaf(e.y, S, -e.x * C);
+    return t;
+}
+
 template <int FL>
 __device__ __forceinline__ void accumulate(Acc<true>& A, float S, float C, float2 a, float2 b,
                                            float2 e) {
-    kahan_add(A.s0, A.c0, fmaf(a.x, C, -a.y * S));
-    if (FL & K2_D1) kahan_add(A.s1, A.c1, -fmaf(b.x, S, b.y * C));
-    if (FL & K2_D2) kahan_add(A.s2, A.c2, fmaf(e.y, S, -e.x * C));
+    const float3 t = terms<FL>(S, C, a, b, e);
+    kahan_add(A.s0, A.c0, t.x);
+    if (FL & K2_D1) kahan_add(A.s1, A.c1, t.y);
+    if (FL & K2_D2) kahan_add(A.s2, A.c2, t.z);
 }
 
 // ---- C tile ---------------------------------------------------------------------------------
@@ -197,7 +207,11 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
     }
 }
 
-template <int NC, int FL, int NT = K2_WG>
+#define K2_F32_GROUP 8   // zeros per compensated add in the FP32 C tile (divides K2_WG)
+
+// FP32 form of ctile_f64: the phase window, a 2^TB-point FP32 table (tab), FP32 weights and
+// compensated FP32 sums
+template <int NC, int FL, int NT = K2_WG, int TB = K2_TAB32_BITS>
 __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, const Limbs4 (&v)[NC],
                                           Acc<true> (&acc)[NC], CTileSmem32<FL>& sm,
                                           const Tab32* tab) {
@@ -237,23 +251,45 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
     };
     if (ntiles > 0) { fetch(0); stage(0); }
     __syncthreads();
+    const uint64_t hs = window_half<TB>();
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
         if (t + 1 < ntiles) fetch(t + 1);
-#pragma unroll 2
-        for (int z = 0; z < K2_WG; ++z) {
-            const uint4 gg = sm.g[buf][z];
-            const float4 ab = sm.ab[buf][z];
-            float2 ee = make_float2(0.f, 0.f);
-            if (D2) ee = sm.e(buf, z);
-            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+        // K2_F32_GROUP zeros per step: their terms are summed in FP32, then the group sum enters
+        // the compensated sum (blocked compensated summation: the plain adds inside a group
+        // cost about two roundings per term, against the four adds per term of compensating
+        // every term)
+        for (int z = 0; z < K2_WG; z += K2_F32_GROUP) {
+            float3 tp[NC];
+#pragma unroll
+            for (int h = 0; h < K2_F32_GROUP; ++h) {
+                const uint4 gg = sm.g[buf][z + h];
+                const float4 ab = sm.ab[buf][z + h];
+                float2 ee = make_float2(0.f, 0.f);
+                if (D2) ee = sm.e(buf, z + h);
+                const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    uint32_t hi, lo;
+                    phase_window<TB>(g, v[i], hs, hi, lo);
+                    float S, C;
+                    sincos_window<TB>(hi, lo, tab, S, C);
+                    const float3 tm = terms<FL>(S, C, make_float2(ab.x, ab.y),
+                                                make_float2(ab.z, ab.w), ee);
+                    if (h == 0) {
+                        tp[i] = tm;
+                    } else {
+                        tp[i].x += tm.x;
+                        if (D1) tp[i].y += tm.y;
+                        if (D2) tp[i].z += tm.z;
+                    }
+                }
+            }
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                const uint32_t W = phase_word_hi32(g, v[i]);
-                float S, C;
-                sincos_tab32(W, tab, S, C);
-                accumulate<FL>(acc[i], S, C, make_float2(ab.x, ab.y), make_float2(ab.z, ab.w),
-                               ee);
+                kahan_add(acc[i].s0, acc[i].c0, tp[i].x);
+                if (D1) kahan_add(acc[i].s1, acc[i].c1, tp[i].y);
+                if (D2) kahan_add(acc[i].s2, acc[i].c2, tp[i].z);
             }
         }
         if (t + 1 < ntiles) stage(buf ^ 1);
@@ -281,7 +317,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 // cache for each group, which bounds the register footprint).  Writes
 // out[q * CZ + c] (q = value, K', K'') after a fixed-order reduction: butterfly within each
 // wave, then the four wave sums in order.
-template <int CZ, int FL, bool F32>
+template <int CZ, int FL, bool F32, int TB = K2_TAB32_BITS>
 __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const uint4* cand,
                                       int c_first, int c_count, double* out, ZTileSmem<CZ>& sm,
                                       const Tab64* tab64, const Tab32* tab32) {
@@ -295,6 +331,8 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
     // passes with no live candidate are skipped (their outputs are written as zeros below)
     const int cp_end = min(CZ, (c_count + K2_ZG - 1) / K2_ZG * K2_ZG);
     const SinCosCoef kc = sincos_coef();
+    uint64_t hs = 0;
+    if constexpr (F32) hs = window_half<TB>();
     for (int cp = 0; cp < cp_end; cp += K2_ZG) {
         Acc<F32> acc[K2_ZG];
         Limbs4 v[K2_ZG];
@@ -313,9 +351,10 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                 const float2 e = D2 ? zv.ef[j] : make_float2(0.f, 0.f);
 #pragma unroll
                 for (int c = 0; c < K2_ZG; ++c) {
-                    const uint32_t W = phase_word_hi32(g, v[c]);
+                    uint32_t hi, lo;
+                    phase_window<TB>(g, v[c], hs, hi, lo);
                     float S, C;
-                    sincos_tab32(W, tab32, S, C);
+                    sincos_window<TB>(hi, lo, tab32, S, C);
                     accumulate<FL>(acc[c], S, C, a, b, e);
                 }
             } else {
