@@ -52,7 +52,7 @@ __global__ void k2_counter_reset_kernel(int* counter) {
 }
 
 template <int NC, int CZ, int FL, bool ALLOW32>
-__global__ __launch_bounds__(256) void k2_batch_kernel(
+__global__ __launch_bounds__(256, 4) void k2_batch_kernel(
     BatchViews views, const BatchItem* __restrict__ items, int n_items, int* counter,
     const uint4* __restrict__ cand, double* __restrict__ partial,
     const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
@@ -65,16 +65,14 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
     auto& sm32 = *reinterpret_cast<CTileSmem32<FL>*>(sm_raw);
     auto& zsm = *reinterpret_cast<ZTileSmem<CZ>*>(sm_raw);
     int& s_item = *reinterpret_cast<int*>(sm_raw);
-    // the FP32 items use a 1,024-point table (every other entry of the global one), which keeps
-    // the LDS per workgroup at three workgroups per CU
-    constexpr int TB = K2_TAB32_BITS - 1;
+    // one 16 KiB table region holds the FP64 table or, while FP32 items run, the FP32 one; it is
+    // reloaded when the kind of item changes (a few times per workgroup: microseconds against
+    // milliseconds per item), which leaves room for four workgroups per CU
+    static_assert(sizeof(Tab32) * K2_TAB32 <= sizeof(Tab64) * K2_TAB, "FP32 table fits");
     __shared__ Tab64 s_tab64[K2_TAB];
-    __shared__ Tab32 s_tab32[ALLOW32 ? (1 << TB) : 1];
+    Tab32* s_tab32 = reinterpret_cast<Tab32*>(s_tab64);
     const int tid = threadIdx.x;
-    for (int i = tid; i < K2_TAB; i += K2_WG) s_tab64[i] = tab64[i];
-    if (ALLOW32)
-        for (int i = tid; i < (1 << TB); i += K2_WG) s_tab32[i] = tab32[i << (K2_TAB32_BITS - TB)];
-    __syncthreads();
+    int loaded = -1;   // kind of table in s_tab64: 0 FP64, 1 FP32 (uniform over the workgroup)
     constexpr int CW = K2_WG * NC;
     for (;;) {
         __syncthreads();   // every wave is done with the previous item's LDS before s_item lands
@@ -86,12 +84,22 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
         const BatchItem I = items[it];
         const ZeroView zv = views.zv[(I.flags >> 8) & 3];
         const bool f32 = ALLOW32 && (I.flags & 2);
+        if ((int)f32 != loaded) {
+            // every lookup of the previous item is done: tiles end with a barrier
+            if (f32) {
+                for (int i = tid; i < K2_TAB32; i += K2_WG) s_tab32[i] = tab32[i];
+            } else {
+                for (int i = tid; i < K2_TAB; i += K2_WG) s_tab64[i] = tab64[i];
+            }
+            __syncthreads();
+            loaded = (int)f32;
+        }
         if (I.flags & 1) {
             double* out = partial + I.poff;
             if (f32) {
                 if constexpr (ALLOW32)
-                    ztile<CZ, FL, true, TB>(zv, I.z0, I.z1, cand, I.c_first, I.c_count, out,
-                                            zsm, s_tab64, s_tab32);
+                    ztile<CZ, FL, true>(zv, I.z0, I.z1, cand, I.c_first, I.c_count, out, zsm,
+                                        s_tab64, s_tab32);
             } else {
                 ztile<CZ, FL, false>(zv, I.z0, I.z1, cand, I.c_first, I.c_count, out, zsm, s_tab64,
                                      s_tab32);
@@ -112,7 +120,7 @@ __global__ __launch_bounds__(256) void k2_batch_kernel(
                 Acc<true> acc[NC];
 #pragma unroll
                 for (int i = 0; i < NC; ++i) acc[i].zero();
-                ctile_f32<NC, FL, K2_WG, TB>(zv, I.z0, I.z1, v, acc, sm32, s_tab32);
+                ctile_f32<NC, FL>(zv, I.z0, I.z1, v, acc, sm32, s_tab32);
 #pragma unroll
                 for (int i = 0; i < NC; ++i)
 #pragma unroll

@@ -299,6 +299,7 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 
 // ---- Z tile ---------------------------------------------------------------------------------
 #define K2_ZG 4   // candidates a Z-tile lane evaluates per pass over its zeros
+#define K2_ZF32_GROUP 4   // zeros per lane per compensated add in the FP32 Z tile
 
 template <int CZ>
 struct ZTileSmem {
@@ -342,22 +343,45 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
             const uint4 cv = sm.v[cp + c];
             v[c] = Limbs4{cv.x, cv.y, cv.z, cv.w};
         }
-        for (int j = z0 + tid; j < z1; j += K2_WG) {
-            const uint4 gg = zv.g[j];
-            const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
-            if constexpr (F32) {
-                const float2 a = zv.af[j];
-                const float2 b = D1 ? zv.bf[j] : make_float2(0.f, 0.f);
-                const float2 e = D2 ? zv.ef[j] : make_float2(0.f, 0.f);
+        if constexpr (F32) {
+            // K2_ZF32_GROUP zeros per lane per compensated add, as in ctile_f32
+            for (int jb = z0 + tid; jb < z1; jb += K2_WG * K2_ZF32_GROUP) {
+                float3 tp[K2_ZG];
+#pragma unroll
+                for (int c = 0; c < K2_ZG; ++c) tp[c] = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+                for (int h = 0; h < K2_ZF32_GROUP; ++h) {
+                    const int j = jb + h * K2_WG;
+                    if (j < z1) {
+                        const uint4 gg = zv.g[j];
+                        const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+                        const float2 a = zv.af[j];
+                        const float2 b = D1 ? zv.bf[j] : make_float2(0.f, 0.f);
+                        const float2 e = D2 ? zv.ef[j] : make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int c = 0; c < K2_ZG; ++c) {
+                            uint32_t hi, lo;
+                            phase_window<TB>(g, v[c], hs, hi, lo);
+                            float S, C;
+                            sincos_window<TB>(hi, lo, tab32, S, C);
+                            const float3 tm = terms<FL>(S, C, a, b, e);
+                            tp[c].x += tm.x;
+                            if (D1) tp[c].y += tm.y;
+                            if (D2) tp[c].z += tm.z;
+                        }
+                    }
+                }
 #pragma unroll
                 for (int c = 0; c < K2_ZG; ++c) {
-                    uint32_t hi, lo;
-                    phase_window<TB>(g, v[c], hs, hi, lo);
-                    float S, C;
-                    sincos_window<TB>(hi, lo, tab32, S, C);
-                    accumulate<FL>(acc[c], S, C, a, b, e);
+                    kahan_add(acc[c].s0, acc[c].c0, tp[c].x);
+                    if (D1) kahan_add(acc[c].s1, acc[c].c1, tp[c].y);
+                    if (D2) kahan_add(acc[c].s2, acc[c].c2, tp[c].z);
                 }
-            } else {
+            }
+        } else {
+            for (int j = z0 + tid; j < z1; j += K2_WG) {
+                const uint4 gg = zv.g[j];
+                const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
                 const double2 a = zv.a[j];
                 const double2 b = D1 ? zv.b[j] : make_double2(0.0, 0.0);
                 const double2 e = D2 ? zv.e[j] : make_double2(0.0, 0.0);
