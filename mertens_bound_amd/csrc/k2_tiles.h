@@ -314,7 +314,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 // Sums zeros [z0, z1) for candidates cand[c_first .. c_first + c_count) (c_count <= CZ); lanes
-// stride over the zeros, K2_ZG candidates at a time (the lane's zero records are re-read from
+// stride over the zeros, ZG candidates at a time (the lane's zero records are re-read from
 // cache for each group, which bounds the register footprint).  Writes
 // out[q * CZ + c] (q = value, K', K'') after a fixed-order reduction: butterfly within each
 // wave, then the four wave sums in order.
@@ -327,18 +327,21 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
     constexpr int NQ = D2 ? 3 : (D1 ? 2 : 1);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
+    // candidates per pass: the FP32 path keeps 8 in registers (each zero record loaded once
+    // per 8 candidates), the FP64 path 4
+    constexpr int ZG = (F32 && CZ % 8 == 0) ? 8 : K2_ZG;
     if (tid < CZ) sm.v[tid] = tid < c_count ? cand[c_first + tid] : make_uint4(0, 0, 0, 0);
     __syncthreads();
     // passes with no live candidate are skipped (their outputs are written as zeros below)
-    const int cp_end = min(CZ, (c_count + K2_ZG - 1) / K2_ZG * K2_ZG);
+    const int cp_end = min(CZ, (c_count + ZG - 1) / ZG * ZG);
     const SinCosCoef kc = sincos_coef();
     uint64_t hs = 0;
     if constexpr (F32) hs = window_half<TB>();
-    for (int cp = 0; cp < cp_end; cp += K2_ZG) {
-        Acc<F32> acc[K2_ZG];
-        Limbs4 v[K2_ZG];
+    for (int cp = 0; cp < cp_end; cp += ZG) {
+        Acc<F32> acc[ZG];
+        Limbs4 v[ZG];
 #pragma unroll
-        for (int c = 0; c < K2_ZG; ++c) {
+        for (int c = 0; c < ZG; ++c) {
             acc[c].zero();
             const uint4 cv = sm.v[cp + c];
             v[c] = Limbs4{cv.x, cv.y, cv.z, cv.w};
@@ -346,9 +349,9 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
         if constexpr (F32) {
             // K2_ZF32_GROUP zeros per lane per compensated add, as in ctile_f32
             for (int jb = z0 + tid; jb < z1; jb += K2_WG * K2_ZF32_GROUP) {
-                float3 tp[K2_ZG];
+                float3 tp[ZG];
 #pragma unroll
-                for (int c = 0; c < K2_ZG; ++c) tp[c] = make_float3(0.f, 0.f, 0.f);
+                for (int c = 0; c < ZG; ++c) tp[c] = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
                 for (int h = 0; h < K2_ZF32_GROUP; ++h) {
                     const int j = jb + h * K2_WG;
@@ -359,7 +362,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                         const float2 b = D1 ? zv.bf[j] : make_float2(0.f, 0.f);
                         const float2 e = D2 ? zv.ef[j] : make_float2(0.f, 0.f);
 #pragma unroll
-                        for (int c = 0; c < K2_ZG; ++c) {
+                        for (int c = 0; c < ZG; ++c) {
                             uint32_t hi, lo;
                             phase_window<TB>(g, v[c], hs, hi, lo);
                             float S, C;
@@ -372,7 +375,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < K2_ZG; ++c) {
+                for (int c = 0; c < ZG; ++c) {
                     kahan_add(acc[c].s0, acc[c].c0, tp[c].x);
                     if (D1) kahan_add(acc[c].s1, acc[c].c1, tp[c].y);
                     if (D2) kahan_add(acc[c].s2, acc[c].c2, tp[c].z);
@@ -386,7 +389,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                 const double2 b = D1 ? zv.b[j] : make_double2(0.0, 0.0);
                 const double2 e = D2 ? zv.e[j] : make_double2(0.0, 0.0);
 #pragma unroll
-                for (int c = 0; c < K2_ZG; ++c) {
+                for (int c = 0; c < ZG; ++c) {
                     const uint64_t W = phase_word(g, v[c]);
                     double S, C;
                     sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), tab64, S, C, kc);
@@ -395,7 +398,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
             }
         }
 #pragma unroll
-        for (int c = 0; c < K2_ZG; ++c) {
+        for (int c = 0; c < ZG; ++c) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const double x = wave_sum(acc[c].get(q));
