@@ -72,8 +72,14 @@ __device__ __noinline__ uint64_t phase_word_exact(Limbs4 g, Limbs4 v) {
     return lo | hi;
 }
 
-// W with columns 0-1 skipped and the carry case fixed up exactly
-__device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
+// byte offset of the 32-byte FP64 table entry nearest to the phase word W (K2_TAB points)
+__device__ __forceinline__ uint32_t tab64_offset(uint64_t W) {
+    return ((((uint32_t)(W >> 32)) + (1u << 22)) >> 18) & 0x3FE0u;
+}
+
+// W with columns 0-1 skipped and the carry case fixed up exactly; off = tab64_offset(W), taken
+// from the column-5 word (W bits [55,64) are its bits [5,14), and rounding adds 2^54 = 16 there)
+__device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v, uint32_t& off) {
     // column 2 (weight 2^64): g0 v2 + g1 v1 + g2 v0; a carry out of a 64-bit add shows as the
     // new sum being below the old one
     uint32_t k2;
@@ -100,8 +106,17 @@ __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
     uint32_t wlo = __builtin_amdgcn_alignbit(s4lo, s3lo, 14);
     uint32_t whi = __builtin_amdgcn_alignbit(s5, s4lo, 14);
     uint64_t W = ((uint64_t)whi << 32) | wlo;
-    if (__builtin_expect(((s3lo >> 2) & 0xFFFu) == 0xFFFu, 0)) W = phase_word_exact(g, v);
+    off = (s5 + 16u) & 0x3FE0u;
+    if (__builtin_expect(((s3lo >> 2) & 0xFFFu) == 0xFFFu, 0)) {
+        W = phase_word_exact(g, v);
+        off = tab64_offset(W);
+    }
     return W;
+}
+
+__device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
+    uint32_t off;
+    return phase_word(g, v, off);
 }
 
 // ---- FP32 form: the phase window ------------------------------------------------------------
@@ -208,20 +223,30 @@ __device__ __forceinline__ SinCosCoef sincos_coef() {
     return k;
 }
 
+// off: tab64_offset of the word (phase_word hands it over; the kernels' loops save its
+// computation that way)
+__device__ __forceinline__ void sincos_tab64_at(uint32_t wlo, uint32_t whi, uint32_t off,
+                                                const Tab64* tab, double& S, double& C,
+                                                const SinCosCoef& kc) {
+    // sign-extended low K2_TAB_SHIFT bits of whi in one v_bfe_i32 (as asm: the compiler would
+    // otherwise rebuild it from the 64-bit word as a funnel shift plus an arithmetic shift)
+    int32_t rhi;
+    asm("v_bfe_i32 %0, %1, 0, %2" : "=v"(rhi) : "v"(whi), "i"(K2_TAB_SHIFT));
+    const double u = fma((double)rhi, 4294967296.0, (double)wlo);    // |u| <= 2^54
+    const double z = u * u;
+    const double sp = fma(u * z, fma(z, kc.s2, kc.s1), u);
+    const double cp = fma(z, fma(z, kc.c2, kc.c1), 1.0);
+    const Tab64 t = *reinterpret_cast<const Tab64*>(reinterpret_cast<const char*>(tab) + off);
+    C = fma(t.c, cp, -t.sc * sp);
+    S = fma(t.s, cp, t.cc * sp);
+}
+
 __device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const Tab64* tab,
                                              double& S, double& C,
                                              const SinCosCoef& kc = SinCosCoef{
                                                  -1.9336107030552212e-38, 1.121655105290912e-76,
                                                  -5.800832109165663e-38, 5.608275526454561e-76}) {
-    const uint32_t k = (whi + (1u << (K2_TAB_SHIFT - 1))) >> K2_TAB_SHIFT;   // nearest point
-    const int32_t rhi = ((int32_t)(whi << (32 - K2_TAB_SHIFT))) >> (32 - K2_TAB_SHIFT);
-    const double u = fma((double)rhi, 4294967296.0, (double)wlo);    // |u| <= 2^54
-    const double z = u * u;
-    const double sp = fma(u * z, fma(z, kc.s2, kc.s1), u);
-    const double cp = fma(z, fma(z, kc.c2, kc.c1), 1.0);
-    const Tab64 t = tab[k];
-    C = fma(t.c, cp, -t.sc * sp);
-    S = fma(t.s, cp, t.cc * sp);
+    sincos_tab64_at(wlo, whi, tab64_offset(((uint64_t)whi << 32) | wlo), tab, S, C, kc);
 }
 
 // FP32 sin/cos of the window's phase with a 2^TB-point table (every 2^(11-TB)-th entry of the

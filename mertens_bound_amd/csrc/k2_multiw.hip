@@ -98,9 +98,10 @@ __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
         for (int z = 0; z < K2_MW_TZ; ++z) {
             const uint4 gg = sm.g[buf][z];
             const double2 ww = sm.w[buf][z];
-            const uint64_t W = phase_word(Limbs4{gg.x, gg.y, gg.z, gg.w}, v);
+            uint32_t off;
+            const uint64_t W = phase_word(Limbs4{gg.x, gg.y, gg.z, gg.w}, v, off);
             double S, C;
-            sincos_tab64((uint32_t)W, (uint32_t)(W >> 32), s_tab, S, C, kc);
+            sincos_tab64_at((uint32_t)W, (uint32_t)(W >> 32), off, s_tab, S, C, kc);
             const double A = fma(ww.x, C, -ww.y * S);          // Re(w e^{i th})
 #pragma unroll
             for (int m = 0; m < M; ++m) acc[0][m] = fma(sm.wt[buf][m][z], A, acc[0][m]);
