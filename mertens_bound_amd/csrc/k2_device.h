@@ -52,6 +52,28 @@ __device__ __forceinline__ uint64_t mad_carry0(uint32_t a, uint32_t b, uint64_t 
     return r;
 }
 
+// a*b + c mod 2^64 (v_mad_u64_u32; as asm so that a chain of them keeps its order: the compiler
+// would sum the products apart and add the incoming word with one more 64-bit add)
+__device__ __forceinline__ uint64_t mad64(uint32_t a, uint32_t b, uint64_t c) {
+    uint64_t r;
+    asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c) : "vcc");
+    return r;
+}
+
+// as mad_carry0, with the carry count starting at B (an inline constant, 0..63): k = B + carry
+template <int B>
+__device__ __forceinline__ uint64_t mad_carry0_from(uint32_t a, uint32_t b, uint64_t c,
+                                                    uint32_t& k) {
+    static_assert(B >= 0 && B < 64, "inline constant");
+    uint64_t r;
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+        "v_cndmask_b32_e64 %1, %5, %6, vcc"
+        : "=&v"(r), "=v"(k)
+        : "v"(a), "v"(b), "v"(c), "i"(B), "i"(B + 1)
+        : "vcc");
+    return r;
+}
+
 // exact W via a straightforward 256-bit schoolbook product (rare fix-up path and debug use)
 __device__ __noinline__ uint64_t phase_word_exact(Limbs4 g, Limbs4 v) {
     const uint32_t G[4] = {g.x0, g.x1, g.x2, g.x3};
@@ -93,11 +115,12 @@ __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v, uint32_t& off
     s3 = mad_carry(g.x1, v.x2, s3, k3);
     s3 = mad_carry(g.x2, v.x1, s3, k3);
     s3 = mad_carry(g.x3, v.x0, s3, k3);
-    // column 4 (2^128): only bits [0,46) are needed, so wrap mod 2^64
+    // column 4 (2^128): only bits [0,46) are needed, so wrap mod 2^64; chained onto the
+    // column-3 carry word
     uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
-    s4 = mul32(g.x1, v.x3) + s4;
-    s4 = mul32(g.x2, v.x2) + s4;
-    s4 = mul32(g.x3, v.x1) + s4;
+    s4 = mad64(g.x1, v.x3, s4);
+    s4 = mad64(g.x2, v.x2, s4);
+    s4 = mad64(g.x3, v.x1, s4);
     // column 5 (2^160): only bits [0,14) are needed; 24x24-bit products are exact mod 2^24
     uint32_t s5 = (uint32_t)(s4 >> 32);
     s5 = mad24(g.x2, v.x3, s5);
@@ -128,28 +151,18 @@ __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
 // half a step of a 2^TB-point table, so that the top TB phase bits of the window round to the
 // nearest table point.  Only hi bits [0,24) are formed (the column-5 terms are 24x24-bit
 // products, exact mod 2^24).
-// hs = window_half<TB>(), taken once outside the loop: it is the addend of the first column-4
-// multiply-add and stays in a VGPR pair.
 template <int TB>
-__device__ __forceinline__ uint64_t window_half() {
+__device__ __forceinline__ void phase_window(Limbs4 g, Limbs4 v, uint32_t& hi, uint32_t& lo) {
     static_assert(TB >= 9 && TB <= 11, "table of 512..2048 points");
-    uint64_t h = 1ull << (45 - TB);   // 2^(173-TB) at weight 2^128
-    asm volatile("" : "+v"(h));
-    return h;
-}
-
-template <int TB>
-__device__ __forceinline__ void phase_window(Limbs4 g, Limbs4 v, uint64_t hs, uint32_t& hi,
-                                             uint32_t& lo) {
     uint64_t s3 = mul32(g.x0, v.x3);
-    uint32_t k3;
-    s3 = mad_carry0(g.x1, v.x2, s3, k3);
+    uint32_t k3;   // column-3 carries, starting at half a table step: 2^(173-TB) = 2^(13-TB) 2^160
+    s3 = mad_carry0_from<(1 << (13 - TB))>(g.x1, v.x2, s3, k3);
     s3 = mad_carry(g.x2, v.x1, s3, k3);
     s3 = mad_carry(g.x3, v.x0, s3, k3);
-    uint64_t s4 = mul32(g.x1, v.x3) + hs;
-    s4 = mul32(g.x2, v.x2) + s4;
-    s4 = mul32(g.x3, v.x1) + s4;
-    s4 += (s3 >> 32) | ((uint64_t)k3 << 32);
+    uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
+    s4 = mad64(g.x1, v.x3, s4);
+    s4 = mad64(g.x2, v.x2, s4);
+    s4 = mad64(g.x3, v.x1, s4);
     uint32_t s5 = (uint32_t)(s4 >> 32);
     s5 = mad24(g.x2, v.x3, s5);
     s5 = mad24(g.x3, v.x2, s5);

@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     else if (mode == 1) W = phase_word_exact(g, v);
     else {
         uint32_t hi, lo;
-        phase_window<K2_TAB32_BITS>(g, v, window_half<K2_TAB32_BITS>(), hi, lo);
+        phase_window<K2_TAB32_BITS>(g, v, hi, lo);
         W = ((((uint64_t)hi << 32) | lo) - (1ull << (45 - K2_TAB32_BITS))) & ((1ull << 46) - 1);
     }
     out[idx] = W;

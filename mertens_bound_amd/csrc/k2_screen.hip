@@ -63,13 +63,12 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
 #pragma unroll
     for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
     int j = 0, best = M, passed = 1;
-    const uint64_t hs = window_half<K2_TAB32_BITS>();
     for (int s = 0; s < n_stages; ++s) {
         const int nz = stage_nz[s];
         for (; j < nz; ++j) {
             const ScreenZero z = sz[j];
             uint32_t hi, lo;
-            phase_window<K2_TAB32_BITS>(Limbs4{z.g.x, z.g.y, z.g.z, z.g.w}, v, hs, hi, lo);
+            phase_window<K2_TAB32_BITS>(Limbs4{z.g.x, z.g.y, z.g.z, z.g.w}, v, hi, lo);
             float S, C;
             sincos_window<K2_TAB32_BITS>(hi, lo, s_tab, S, C);
             // q = a e^{i th}

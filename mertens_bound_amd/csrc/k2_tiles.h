@@ -252,7 +252,6 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
     };
     if (ntiles > 0) { fetch(0); stage(0); }
     __syncthreads();
-    const uint64_t hs = window_half<TB>();
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
         if (t + 1 < ntiles) fetch(t + 1);
@@ -272,7 +271,7 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
                     uint32_t hi, lo;
-                    phase_window<TB>(g, v[i], hs, hi, lo);
+                    phase_window<TB>(g, v[i], hi, lo);
                     float S, C;
                     sincos_window<TB>(hi, lo, tab, S, C);
                     const float3 tm = terms<FL>(S, C, make_float2(ab.x, ab.y),
@@ -336,8 +335,6 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
     // passes with no live candidate are skipped (their outputs are written as zeros below)
     const int cp_end = min(CZ, (c_count + ZG - 1) / ZG * ZG);
     const SinCosCoef kc = sincos_coef();
-    uint64_t hs = 0;
-    if constexpr (F32) hs = window_half<TB>();
     for (int cp = 0; cp < cp_end; cp += ZG) {
         Acc<F32> acc[ZG];
         Limbs4 v[ZG];
@@ -365,7 +362,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
 #pragma unroll
                         for (int c = 0; c < ZG; ++c) {
                             uint32_t hi, lo;
-                            phase_window<TB>(g, v[c], hs, hi, lo);
+                            phase_window<TB>(g, v[c], hi, lo);
                             float S, C;
                             sincos_window<TB>(hi, lo, tab32, S, C);
                             const float3 tm = terms<FL>(S, C, a, b, e);
