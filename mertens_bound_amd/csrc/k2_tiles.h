@@ -119,11 +119,19 @@ __device__ __forceinline__ void accumulate(Acc<true>& A, float S, float C, float
 template <int FL>
 struct CTileSmem64 {
     static constexpr int NB = (FL & K2_D1) ? 1 : 0, NE = (FL & K2_D2) ? 1 : 0;
-    uint4 g[2][K2_WG];
-    double2 w[2][(1 + NB + NE) * K2_WG];   // a, then b, then e
-    __device__ __forceinline__ double2& a(int buf, int z) { return w[buf][z]; }
-    __device__ __forceinline__ double2& b(int buf, int z) { return w[buf][K2_WG + z]; }
-    __device__ __forceinline__ double2& e(int buf, int z) { return w[buf][(1 + NB) * K2_WG + z]; }
+    // one buffer's records together, so that a zero's fields sit at fixed offsets from one
+    // address (one address add per zero in the loop)
+    struct Buf {
+        uint4 g[K2_WG];
+        double2 w[(1 + NB + NE) * K2_WG];   // a, then b, then e
+    };
+    Buf bf[2];
+    __device__ __forceinline__ uint4& gz(int buf, int z) { return bf[buf].g[z]; }
+    __device__ __forceinline__ double2& a(int buf, int z) { return bf[buf].w[z]; }
+    __device__ __forceinline__ double2& b(int buf, int z) { return bf[buf].w[K2_WG + z]; }
+    __device__ __forceinline__ double2& e(int buf, int z) {
+        return bf[buf].w[(1 + NB) * K2_WG + z];
+    }
 };
 
 template <int FL>
@@ -174,7 +182,7 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
         }
     };
     auto stage = [&](int buf) {
-        if (lo_half) { sm.g[buf][zl] = rg; sm.a(buf, zl) = ra; }
+        if (lo_half) { sm.gz(buf, zl) = rg; sm.a(buf, zl) = ra; }
         if (hi_half) {
             if (D1) sm.b(buf, zl) = rb;
             if (D2) sm.e(buf, zl) = re;
@@ -188,7 +196,7 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
         if (t + 1 < ntiles) fetch(t + 1);
 #pragma unroll(NT == K2_WG ? 2 : 1)
         for (int z = 0; z < K2_WG; ++z) {
-            const uint4 gg = sm.g[buf][z];
+            const uint4 gg = sm.gz(buf, z);
             const double2 aa = sm.a(buf, z);
             double2 bb = make_double2(0.0, 0.0), ee = bb;
             if (D1) bb = sm.b(buf, z);
