@@ -313,25 +313,47 @@ def run_polish(args):
     V0t = ev.v_tensor(V0)
     k = c["k"]
 
+    mixed = args.form == "mixed"
+
     def step():
-        ag.polish(t, V0t, k, n_steps=8)
+        ag.polish(t, V0t, k, n_steps=8, mixed=mixed)
 
     ms, wall = timed(step, args.steps, args.warmup, torch)
-    V, K, dK, steps = ag.polish(t, V0t, k, n_steps=8)
+    V, K, dK, steps = ag.polish(t, V0t, k, n_steps=8, mixed=mixed)
     torch.cuda.synchronize()
     # candidate evaluations per polish: the start value, then the Newton passes (converged
     # candidates drop out after each step) and the final value
     cand_evals = V0.shape[0] + t.polish_stats()
     evals = float(cand_evals) * tb.n
     res = base_record(args, "candidate-zero evals/sec, Newton polish (K2, K2', K2'')",
-                      evals / (ms * 1e-3), "candidate-zero evals/s", ms, "f64",
+                      evals / (ms * 1e-3), "candidate-zero evals/s", ms,
+                      "f32+f64" if mixed else "f64",
                       {"workload": "4,096 candidates near synthetic peaks (v0 + N(0,1e-3)) x "
                                    "2,325,572 zeros, 8 Newton steps",
+                       "form": "mixed" if mixed else "plain",
                        "candidate_evals_per_polish": cand_evals,
                        "passes_equivalent": cand_evals / V0.shape[0]})
     res["polish"] = {"mean_steps": float(steps.float().mean().item()),
                      "max_abs_dK_final": float(dK.abs().max().item()),
                      "mean_abs_K": float(K.abs().mean().item())}
+    if mixed and not args.no_check:
+        # the FP64 polish on the same candidates: time and agreement of the final values
+        def step64():
+            ag.polish(t, V0t, k, n_steps=8)
+
+        ms64, _ = timed(step64, args.steps, args.warmup, torch)
+        _, K64, _, steps64 = ag.polish(t, V0t, k, n_steps=8)
+        torch.cuda.synchronize()
+        conv = steps64 < 8   # stopped on the step rule in FP64
+        d = (K - K64).abs()
+        res["vs_plain"] = {"plain_ms_per_step": ms64, "speedup": ms64 / ms,
+                           "fp64_converged": int(conv.sum().item()),
+                           "max_abs_dK2_converged_vs_fp64_polish":
+                               float(d[conv].max().item()) if bool(conv.any()) else None,
+                           "max_abs_dK2_all_vs_fp64_polish": float(d.max().item()),
+                           "note": "candidates still moving after 8 steps end at different "
+                                   "points in the two forms (their Newton steps amplify K2' "
+                                   "differences); converged ones agree"}
     res["wall_s"] = wall
     return res
 

@@ -51,3 +51,32 @@ def test_polish_compaction_counts_and_matches_full_passes():
     # every candidate is evaluated once per step it took, and once more when it stopped because
     # its step fell below 1e-13, plus the first and the final pass
     assert evals >= n + int(st.sum())
+
+
+def test_mixed_polish_matches_fp64_polish():
+    """The polish in the precision form (FP32 tail after the integer phase reduction, FP64 head
+    and Newton step) lands on the same peaks as the FP64 polish: K within 1e-9 at the final
+    points, the same step counts, and the same points to within K2' error / K2''."""
+    import torch
+    from mertens_bound_amd import autograd as ag
+    from mertens_bound_amd import workloads
+    tb = synth.make_zero_table(200_000, 5)
+    t = ev.K2Table(tb)
+    k = 1.5e-10
+    V0 = workloads.polish_candidates(t, 256, seed=3, n_peaks=8, n_screen=4096, k=k)
+    V0t = ev.v_tensor(V0)
+    Va, Ka, dKa, sa = ag.polish(t, V0t, k, n_steps=8)
+    Vb, Kb, dKb, sb = ag.polish(t, V0t, k, n_steps=8, mixed=True)
+    torch.cuda.synchronize()
+    Ka, Kb = Ka.cpu().numpy(), Kb.cpu().numpy()
+    sa, sb = sa.cpu().numpy(), sb.cpu().numpy()
+    dv = np.array([abs(a - b) / 2.0 ** 64 for a, b in
+                   zip(synth.v_to_int(ev.v_from_tensor(Va)), synth.v_to_int(ev.v_from_tensor(Vb)))])
+    conv = sa < 8   # FP64 polish stopped on the step rule
+    dK = np.abs(Ka - Kb)
+    print(f"mixed vs FP64 polish: {conv.sum()} of {len(sa)} converged in FP64; converged: max |dK| "
+          f"{dK[conv].max():.3e}, max |dv| {dv[conv].max():.3e}; all: max |dK| {dK.max():.3e}, "
+          f"max |dv| {dv.max():.3e}; mixed steps {np.bincount(sb, minlength=9)}; FP64 steps "
+          f"{np.bincount(sa, minlength=9)}")
+    assert conv.sum() >= len(sa) // 2
+    assert dK[conv].max() <= 1e-9
