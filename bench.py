@@ -30,8 +30,13 @@ sys.path.insert(0, ROOT)
 #   mixed: 15 INT32 multiply-adds + 2 transcendental-equivalents (counted as 2 slots each,
 #          half rate) + 4 FP32 FMA-equivalents (8 flops)
 #   enclose: the plain work + 2 x 6 FP64 additions (TwoSum error-free accumulation of K2, K2')
+# Work per candidate-zero evaluation in full-rate VALU slots (one wave64 instruction per 4 cycles
+# per SIMD).  FP32 instructions issue in half that (2 cycles per wave64 on gfx950; the FP32 tail
+# issues 130% of the full-rate instruction rate, profiles/k2_eval_mixed_config2_trace_counters.txt),
+# so an FP32 FMA is half a slot.  Mixed: the sin and cos of the reduced angle count as the three
+# FMAs of a table rotation each, plus four FMAs for the weights: 10 FP32 FMAs = 5 slots.
 WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
-        "mixed": {"int32_mad": 15, "trans": 2, "fp32_fma": 4, "slots": 23},
+        "mixed": {"int32_mad": 15, "fp32_fma": 10, "fp32_fma_slot": 0.5, "slots": 20},
         "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48}}
 # Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
 # published 78.6 TFLOP/s FP64 peak counted per FMA (2 flops).  The roofline reports the work in

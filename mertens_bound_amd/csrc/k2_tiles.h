@@ -95,22 +95,21 @@ __device__ __forceinline__ void accumulate(AccDD& A, double S, double C, double2
     if (FL & K2_D1) two_sum_acc(A.h1, A.l1, -fma(b.x, S, b.y * C));
 }
 
-// the FP32 terms (K, K', K'') of one zero; the unrequested ones are left 0
+// one zero's FP32 terms (K, K', K'') added into a group sum with two FMAs each
 template <int FL>
-__device__ __forceinline__ float3 terms(float S, float C, float2 a, float2 b, float2 e) {
-    float3 t = make_float3(fmaf(a.x, C, -a.y * S), 0.f, 0.f);
-    if (FL & K2_D1) t.y = -fmaf(b.x, S, b.y * C);
-    if (FL & K2_D2) t.z = fmaf(e.y, S, -e.x * C);
-    return t;
+__device__ __forceinline__ void add_terms(float3& p, float S, float C, float2 a, float2 b,
+                                          float2 e) {
+    p.x = fmaf(-a.y, S, fmaf(a.x, C, p.x));
+    if (FL & K2_D1) p.y = fmaf(-b.y, C, fmaf(-b.x, S, p.y));
+    if (FL & K2_D2) p.z = fmaf(-e.x, C, fmaf(e.y, S, p.z));
 }
 
+// a group sum into the compensated per-lane sums
 template <int FL>
-__device__ __forceinline__ void accumulate(Acc<true>& A, float S, float C, float2 a, float2 b,
-                                           float2 e) {
-    const float3 t = terms<FL>(S, C, a, b, e);
-    kahan_add(A.s0, A.c0, t.x);
-    if (FL & K2_D1) kahan_add(A.s1, A.c1, t.y);
-    if (FL & K2_D2) kahan_add(A.s2, A.c2, t.z);
+__device__ __forceinline__ void add_group(Acc<true>& A, const float3& p) {
+    kahan_add(A.s0, A.c0, p.x);
+    if (FL & K2_D1) kahan_add(A.s1, A.c1, p.y);
+    if (FL & K2_D2) kahan_add(A.s2, A.c2, p.z);
 }
 
 // ---- C tile ---------------------------------------------------------------------------------
@@ -263,12 +262,14 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
         if (t + 1 < ntiles) fetch(t + 1);
-        // K2_F32_GROUP zeros per step: their terms are summed in FP32, then the group sum enters
-        // the compensated sum (blocked compensated summation: the plain adds inside a group
-        // cost about two roundings per term, against the four adds per term of compensating
-        // every term)
+        // K2_F32_GROUP zeros per step: their terms are accumulated in FP32 by FMAs, then the
+        // group sum enters the compensated sum (blocked compensated summation: about two
+        // roundings per term inside a group, against four adds per term to compensate every
+        // term)
         for (int z = 0; z < K2_WG; z += K2_F32_GROUP) {
             float3 tp[NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) tp[i] = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
             for (int h = 0; h < K2_F32_GROUP; ++h) {
                 const uint4 gg = sm.g[buf][z + h];
@@ -282,23 +283,12 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
                     phase_window<TB>(g, v[i], hi, lo);
                     float S, C;
                     sincos_window<TB>(hi, lo, tab, S, C);
-                    const float3 tm = terms<FL>(S, C, make_float2(ab.x, ab.y),
-                                                make_float2(ab.z, ab.w), ee);
-                    if (h == 0) {
-                        tp[i] = tm;
-                    } else {
-                        tp[i].x += tm.x;
-                        if (D1) tp[i].y += tm.y;
-                        if (D2) tp[i].z += tm.z;
-                    }
+                    add_terms<FL>(tp[i], S, C, make_float2(ab.x, ab.y), make_float2(ab.z, ab.w),
+                                  ee);
                 }
             }
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                kahan_add(acc[i].s0, acc[i].c0, tp[i].x);
-                if (D1) kahan_add(acc[i].s1, acc[i].c1, tp[i].y);
-                if (D2) kahan_add(acc[i].s2, acc[i].c2, tp[i].z);
-            }
+            for (int i = 0; i < NC; ++i) add_group<FL>(acc[i], tp[i]);
         }
         if (t + 1 < ntiles) stage(buf ^ 1);
         __syncthreads();
@@ -373,19 +363,12 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                             phase_window<TB>(g, v[c], hi, lo);
                             float S, C;
                             sincos_window<TB>(hi, lo, tab32, S, C);
-                            const float3 tm = terms<FL>(S, C, a, b, e);
-                            tp[c].x += tm.x;
-                            if (D1) tp[c].y += tm.y;
-                            if (D2) tp[c].z += tm.z;
+                            add_terms<FL>(tp[c], S, C, a, b, e);
                         }
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < ZG; ++c) {
-                    kahan_add(acc[c].s0, acc[c].c0, tp[c].x);
-                    if (D1) kahan_add(acc[c].s1, acc[c].c1, tp[c].y);
-                    if (D2) kahan_add(acc[c].s2, acc[c].c2, tp[c].z);
-                }
+                for (int c = 0; c < ZG; ++c) add_group<FL>(acc[c], tp[c]);
             }
         } else {
             for (int j = z0 + tid; j < z1; j += K2_WG) {
