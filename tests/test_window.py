@@ -1,0 +1,76 @@
+"""The FP32 form's phase window and table sin/cos, restated in exact integers and numpy float32
+on the CPU and checked against the exact phase: the bounds the kernels rely on (the window is
+floor(phase 2^46) minus 0..4; sin/cos within FP32 rounding of the exact values)."""
+import numpy as np
+
+from mertens_bound_amd import synth
+
+M64 = (1 << 64) - 1
+
+
+def _limbs(x):
+    return [(x >> (32 * i)) & 0xFFFFFFFF for i in range(4)]
+
+
+def _window(G, V, tb=11):
+    """Columns 3..5 of the limb product plus half a table step, bits [128, 192) (as the kernels
+    form them: columns 0..2 are skipped)."""
+    g, v = _limbs(G), _limbs(V)
+    P = sum(g[i] * v[j] << (32 * (i + j)) for i in range(4) for j in range(4) if i + j >= 3)
+    P += 1 << (173 - tb)
+    return (P >> 128) & M64
+
+
+def _cases(n, seed):
+    rng = np.random.default_rng(seed)
+    Gs = [int(x) for x in synth.v_to_int(synth.make_zero_table(n, seed).G)]
+    Vs = [int(x) for x in synth.v_to_int(synth.make_candidates(n, 3.4e18, 3.6e18, seed))]
+    edge = [(1 << 128) - 1, (1 << 127) | ((1 << 96) - 1), 1 << 110, (1 << 96) - 1, 0xFFFFFFFF << 64]
+    pairs = list(zip(Gs, Vs)) + [(a, b) for a in edge for b in edge]
+    pairs += [(int(rng.integers(0, 2**63)) << 65 | int(rng.integers(0, 2**63)),
+               int(rng.integers(0, 2**63)) << 65 | int(rng.integers(0, 2**63))) for _ in range(n)]
+    return pairs
+
+
+def test_window_is_floor_minus_at_most_4():
+    m46 = (1 << 46) - 1
+    diffs = []
+    for G, V in _cases(2000, 1):
+        exact = ((G * V) >> 128) & m46
+        win = (_window(G, V) - (1 << 34)) & m46
+        diffs.append((exact - win) & m46)
+    diffs = np.array(diffs)
+    assert diffs.max() <= 4
+    assert diffs.min() == 0 and diffs.max() >= 2   # the skipped columns do carry
+
+
+def test_window_sincos_float32_within_rounding():
+    """Index = top 11 bits (rounded), remainder R -> x = 2 pi (R - 2^31) 2^-43 in FP32,
+    cos(t + x) = c - x (s + c x/2), sin(t + x) = s + x (c - s x/2) in FP32; compared with
+    sin/cos of the exact phase (to 2^-64 turn, evaluated in float64)."""
+    f32 = np.float32
+    two_pi = 2 * np.pi
+    k = np.arange(2048)
+    tab_c = np.cos(two_pi * k / 2048).astype(f32)
+    tab_s = np.sin(two_pi * k / 2048).astype(f32)
+    step = f32(two_pi) / f32(2.0 ** 43)
+    half = step * f32(2.0 ** 31)
+    errs = []
+    for G, V in _cases(3000, 2):
+        w = _window(G, V)
+        hi, lo = w >> 32, w & 0xFFFFFFFF
+        idx = (hi >> 3) & 0x7FF
+        R = ((hi << 32 | lo) >> 3) & 0xFFFFFFFF
+        u = f32(R)
+        x = f32(u * step - half)        # the kernels fuse these two roundings into one FMA
+        h = f32(u * f32(0.5 * step) - f32(0.5 * half))
+        c, s = tab_c[idx], tab_s[idx]
+        C = f32(c - x * f32(s + h * c))
+        S = f32(s + x * f32(c - h * s))
+        phase = ((G * V) >> 110) & M64          # exact to 2^-64 turn
+        th = two_pi * (phase / 2.0 ** 64)
+        errs.append(max(abs(float(C) - np.cos(th)), abs(float(S) - np.sin(th))))
+    errs = np.array(errs)
+    # table rounding (<= 2^-25 each), two FP32 roundings of the rotation, |x|^3/6 <= 6e-10:
+    # well inside 2.5e-7; a wrong sign, index or half step misses by ~1e-3 or more
+    assert errs.max() < 2.5e-7, errs.max()
