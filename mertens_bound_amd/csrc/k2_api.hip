@@ -269,11 +269,15 @@ struct Plan {
 
 void k2impl::split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps) {
     if (nz <= 0) { *n_splits = 0; *zps = 0; return; }
-    const int max_sp = (nz + K2_MIN_SPLIT - 1) / K2_MIN_SPLIT;
-    int sp = (target + n_ctiles - 1) / n_ctiles;
-    sp = std::max(1, std::min(sp, max_sp));
+    // splits of whole 256-zero LDS tiles, about target workgroups in all; when that leaves the
+    // grid under K2_FILL_BLOCKS workgroups (few zeros), shorter splits (down to K2_MIN_SPLIT
+    // zeros) fill the GPU instead
+    int sp = std::max(1, std::min((target + n_ctiles - 1) / n_ctiles, (nz + 255) / 256));
+    if ((int64_t)sp * n_ctiles < K2_FILL_BLOCKS)
+        sp = std::max(sp, std::min((K2_FILL_BLOCKS + n_ctiles - 1) / n_ctiles,
+                                   (nz + K2_MIN_SPLIT - 1) / K2_MIN_SPLIT));
     int z = (nz + sp - 1) / sp;
-    z = (z + 255) / 256 * 256;
+    z = z >= 256 ? (z + 255) / 256 * 256 : (z + K2_MIN_SPLIT - 1) / K2_MIN_SPLIT * K2_MIN_SPLIT;
     *zps = z;
     *n_splits = (nz + z - 1) / z;
 }

@@ -193,8 +193,9 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
         if (t + 1 < ntiles) fetch(t + 1);
+        const int zn = min(K2_WG, z1 - z0 - t * K2_WG);   // zeros of this tile (short splits)
 #pragma unroll(NT == K2_WG ? 2 : 1)
-        for (int z = 0; z < K2_WG; ++z) {
+        for (int z = 0; z < zn; ++z) {
             const uint4 gg = sm.gz(buf, z);
             const double2 aa = sm.a(buf, z);
             double2 bb = make_double2(0.0, 0.0), ee = bb;
@@ -262,11 +263,14 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
         if (t + 1 < ntiles) fetch(t + 1);
+        // zeros of this tile rounded up to whole groups (the staged records past z1 are zeros)
+        const int zn = min(K2_WG, (z1 - z0 - t * K2_WG + K2_F32_GROUP - 1) / K2_F32_GROUP *
+                                      K2_F32_GROUP);
         // K2_F32_GROUP zeros per step: their terms are accumulated in FP32 by FMAs, then the
         // group sum enters the compensated sum (blocked compensated summation: about two
         // roundings per term inside a group, against four adds per term to compensate every
         // term)
-        for (int z = 0; z < K2_WG; z += K2_F32_GROUP) {
+        for (int z = 0; z < zn; z += K2_F32_GROUP) {
             float3 tp[NC];
 #pragma unroll
             for (int i = 0; i < NC; ++i) tp[i] = make_float3(0.f, 0.f, 0.f);
