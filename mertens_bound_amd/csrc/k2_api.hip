@@ -306,11 +306,17 @@ static Plan make_plan(const k2_table* t, const WeightArrays* ws, int n_cand, int
     return p;
 }
 
-// zero chunks of the Z-tile path: about target / n_groups chunks, at least K2_Z_CHUNK zeros each
+// zero chunks of the Z-tile path: about target / n_groups chunks of at least K2_Z_CHUNK zeros;
+// when that leaves the grid under K2_FILL_BLOCKS workgroups (one or two candidate groups),
+// shorter chunks (down to K2_Z_MIN_CHUNK zeros) fill the GPU instead
 static void z_chunks(int len, int n_groups, int target, int* n_ch, int* zpc) {
     if (len <= 0) { *n_ch = 0; *zpc = 0; return; }
     const int want = std::max(1, target / n_groups);
     int z = std::max(K2_Z_CHUNK, (len + want - 1) / want);
+    if ((int64_t)((len + z - 1) / z) * n_groups < K2_FILL_BLOCKS) {
+        const int fill = (K2_FILL_BLOCKS + n_groups - 1) / n_groups;
+        z = std::max(K2_Z_MIN_CHUNK, std::min(z, (len + fill - 1) / fill));
+    }
     z = (z + 255) / 256 * 256;
     *zpc = z;
     *n_ch = (len + z - 1) / z;

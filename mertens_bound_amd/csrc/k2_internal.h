@@ -29,7 +29,8 @@ struct WeightArrays {
 #define K2_MIXED_TAIL 0.16  // default FP32-tail magnitude of the mixed form
 #define K2_Z_CZ 8           // candidates per Z-tile workgroup of the one-launch path
 #define K2_Z_MAX_CAND 384   // default: candidate counts up to this use Z tiles (measured crossover)
-#define K2_Z_CHUNK 8192     // fewest zeros per Z-tile workgroup
+#define K2_Z_CHUNK 8192     // fewest zeros per Z-tile workgroup on a full grid
+#define K2_Z_MIN_CHUNK 2048 // fewest zeros per Z-tile workgroup for one or two candidate groups
 #define K2_MIN_SPLIT 32     // fewest zeros per C-tile split (short splits for small problems)
 #define K2_FILL_BLOCKS 2048 // a C-tile grid below this many workgroups gets shorter splits
 #define K2_FUSED_MIXED_EVALS 1e9   // mixed-form problems up to this many evaluations: one launch
