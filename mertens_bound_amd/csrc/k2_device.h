@@ -19,58 +19,79 @@ struct Limbs4 { uint32_t x0, x1, x2, x3; };
 
 __device__ __forceinline__ uint64_t mul32(uint32_t a, uint32_t b) { return (uint64_t)a * b; }
 
+// The multiply-add helpers below take b from an SGPR when SB is set: the Z tiles' candidate limbs
+// are the same in every lane and live in SGPRs there, which frees VGPRs for more waves (a VALU
+// instruction reads at most one SGPR, and these read only b).
+
 // (a mod 2^24)(b mod 2^24) + c mod 2^32 (v_mad_u32_u24, full rate).  Written as asm: the
 // compiler otherwise drops the operand masks when only low result bits are used and falls back
 // to the quarter-rate v_mul_lo_u32.
+template <bool SB = false>
 __device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
     uint32_t r;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    if constexpr (SB) asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    else asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
     return r;
 }
 
 // a*b + c over 64 bits with the carry out of bit 64 added to k (v_mad_u64_u32's carry-out,
 // then v_addc_co_u32; a VALU carry-out feeds the next VALU's carry-in without wait states, as in
 // the compiler's own 64-bit add sequences).
+template <bool SB = false>
 __device__ __forceinline__ uint64_t mad_carry(uint32_t a, uint32_t b, uint64_t c, uint32_t& k) {
     uint64_t r;
-    asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
-        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
-        : "=&v"(r), "+v"(k)
-        : "v"(a), "v"(b), "v"(c)
-        : "vcc");
+    if constexpr (SB)
+        asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+            "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+            : "=&v"(r), "+v"(k) : "v"(a), "s"(b), "v"(c) : "vcc");
+    else
+        asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+            "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+            : "=&v"(r), "+v"(k) : "v"(a), "v"(b), "v"(c) : "vcc");
     return r;
 }
 
 // a*b + c over 64 bits, returning the carry out of bit 64 in k (first sum of a column)
+template <bool SB = false>
 __device__ __forceinline__ uint64_t mad_carry0(uint32_t a, uint32_t b, uint64_t c, uint32_t& k) {
     uint64_t r;
-    asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
-        "v_cndmask_b32_e64 %1, 0, 1, vcc"
-        : "=&v"(r), "=v"(k)
-        : "v"(a), "v"(b), "v"(c)
-        : "vcc");
+    if constexpr (SB)
+        asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+            "v_cndmask_b32_e64 %1, 0, 1, vcc"
+            : "=&v"(r), "=v"(k) : "v"(a), "s"(b), "v"(c) : "vcc");
+    else
+        asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+            "v_cndmask_b32_e64 %1, 0, 1, vcc"
+            : "=&v"(r), "=v"(k) : "v"(a), "v"(b), "v"(c) : "vcc");
     return r;
 }
 
 // a*b + c mod 2^64 (v_mad_u64_u32; as asm so that a chain of them keeps its order: the compiler
 // would sum the products apart and add the incoming word with one more 64-bit add)
+template <bool SB = false>
 __device__ __forceinline__ uint64_t mad64(uint32_t a, uint32_t b, uint64_t c) {
     uint64_t r;
-    asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c) : "vcc");
+    if constexpr (SB)
+        asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c) : "vcc");
+    else
+        asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c) : "vcc");
     return r;
 }
 
 // as mad_carry0, with the carry count starting at B (an inline constant, 0..63): k = B + carry
-template <int B>
+template <int B, bool SB = false>
 __device__ __forceinline__ uint64_t mad_carry0_from(uint32_t a, uint32_t b, uint64_t c,
                                                     uint32_t& k) {
     static_assert(B >= 0 && B < 64, "inline constant");
     uint64_t r;
-    asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
-        "v_cndmask_b32_e64 %1, %5, %6, vcc"
-        : "=&v"(r), "=v"(k)
-        : "v"(a), "v"(b), "v"(c), "i"(B), "i"(B + 1)
-        : "vcc");
+    if constexpr (SB)
+        asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+            "v_cndmask_b32_e64 %1, %5, %6, vcc"
+            : "=&v"(r), "=v"(k) : "v"(a), "s"(b), "v"(c), "i"(B), "i"(B + 1) : "vcc");
+    else
+        asm("v_mad_u64_u32 %0, vcc, %2, %3, %4\n\t"
+            "v_cndmask_b32_e64 %1, %5, %6, vcc"
+            : "=&v"(r), "=v"(k) : "v"(a), "v"(b), "v"(c), "i"(B), "i"(B + 1) : "vcc");
     return r;
 }
 
@@ -101,30 +122,31 @@ __device__ __forceinline__ uint32_t tab64_offset(uint64_t W) {
 
 // W with columns 0-1 skipped and the carry case fixed up exactly; off = tab64_offset(W), taken
 // from the column-5 word (W bits [55,64) are its bits [5,14), and rounding adds 2^54 = 16 there)
+template <bool SB = false>
 __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v, uint32_t& off) {
     // column 2 (weight 2^64): g0 v2 + g1 v1 + g2 v0; a carry out of a 64-bit add shows as the
     // new sum being below the old one
     uint32_t k2;
     uint64_t t = mul32(g.x0, v.x2);
-    t = mad_carry0(g.x1, v.x1, t, k2);
-    t = mad_carry(g.x2, v.x0, t, k2);
+    t = mad_carry0<SB>(g.x1, v.x1, t, k2);
+    t = mad_carry<SB>(g.x2, v.x0, t, k2);
     // column 3 (2^96): carry-in hi32(t) + k2 2^32
     uint64_t s3 = (t >> 32) | ((uint64_t)k2 << 32);
     uint32_t k3;
-    s3 = mad_carry0(g.x0, v.x3, s3, k3);
-    s3 = mad_carry(g.x1, v.x2, s3, k3);
-    s3 = mad_carry(g.x2, v.x1, s3, k3);
-    s3 = mad_carry(g.x3, v.x0, s3, k3);
+    s3 = mad_carry0<SB>(g.x0, v.x3, s3, k3);
+    s3 = mad_carry<SB>(g.x1, v.x2, s3, k3);
+    s3 = mad_carry<SB>(g.x2, v.x1, s3, k3);
+    s3 = mad_carry<SB>(g.x3, v.x0, s3, k3);
     // column 4 (2^128): only bits [0,46) are needed, so wrap mod 2^64; chained onto the
     // column-3 carry word
     uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
-    s4 = mad64(g.x1, v.x3, s4);
-    s4 = mad64(g.x2, v.x2, s4);
-    s4 = mad64(g.x3, v.x1, s4);
+    s4 = mad64<SB>(g.x1, v.x3, s4);
+    s4 = mad64<SB>(g.x2, v.x2, s4);
+    s4 = mad64<SB>(g.x3, v.x1, s4);
     // column 5 (2^160): only bits [0,14) are needed; 24x24-bit products are exact mod 2^24
     uint32_t s5 = (uint32_t)(s4 >> 32);
-    s5 = mad24(g.x2, v.x3, s5);
-    s5 = mad24(g.x3, v.x2, s5);
+    s5 = mad24<SB>(g.x2, v.x3, s5);
+    s5 = mad24<SB>(g.x3, v.x2, s5);
     const uint32_t s3lo = (uint32_t)s3, s4lo = (uint32_t)s4;
     uint32_t wlo = __builtin_amdgcn_alignbit(s4lo, s3lo, 14);
     uint32_t whi = __builtin_amdgcn_alignbit(s5, s4lo, 14);
@@ -151,21 +173,21 @@ __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
 // half a step of a 2^TB-point table, so that the top TB phase bits of the window round to the
 // nearest table point.  Only hi bits [0,24) are formed (the column-5 terms are 24x24-bit
 // products, exact mod 2^24).
-template <int TB>
+template <int TB, bool SB = false>
 __device__ __forceinline__ void phase_window(Limbs4 g, Limbs4 v, uint32_t& hi, uint32_t& lo) {
     static_assert(TB >= 9 && TB <= 11, "table of 512..2048 points");
     uint64_t s3 = mul32(g.x0, v.x3);
     uint32_t k3;   // column-3 carries, starting at half a table step: 2^(173-TB) = 2^(13-TB) 2^160
-    s3 = mad_carry0_from<(1 << (13 - TB))>(g.x1, v.x2, s3, k3);
-    s3 = mad_carry(g.x2, v.x1, s3, k3);
-    s3 = mad_carry(g.x3, v.x0, s3, k3);
+    s3 = mad_carry0_from<(1 << (13 - TB)), SB>(g.x1, v.x2, s3, k3);
+    s3 = mad_carry<SB>(g.x2, v.x1, s3, k3);
+    s3 = mad_carry<SB>(g.x3, v.x0, s3, k3);
     uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
-    s4 = mad64(g.x1, v.x3, s4);
-    s4 = mad64(g.x2, v.x2, s4);
-    s4 = mad64(g.x3, v.x1, s4);
+    s4 = mad64<SB>(g.x1, v.x3, s4);
+    s4 = mad64<SB>(g.x2, v.x2, s4);
+    s4 = mad64<SB>(g.x3, v.x1, s4);
     uint32_t s5 = (uint32_t)(s4 >> 32);
-    s5 = mad24(g.x2, v.x3, s5);
-    s5 = mad24(g.x3, v.x2, s5);
+    s5 = mad24<SB>(g.x2, v.x3, s5);
+    s5 = mad24<SB>(g.x3, v.x2, s5);
     hi = s5;
     lo = (uint32_t)s4;
 }

@@ -303,6 +303,11 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 #define K2_ZG 4   // candidates a Z-tile lane evaluates per pass over its zeros
 #define K2_ZF32_GROUP 4   // zeros per lane per compensated add in the FP32 Z tile
 
+// a value the same in every lane, as an SGPR
+__device__ __forceinline__ uint32_t uniform(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+}
+
 template <int CZ>
 struct ZTileSmem {
     uint4 v[CZ];
@@ -343,8 +348,9 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
 #pragma unroll
         for (int c = 0; c < ZG; ++c) {
             acc[c].zero();
+            // the same candidates in every lane: their limbs go to SGPRs
             const uint4 cv = sm.v[cp + c];
-            v[c] = Limbs4{cv.x, cv.y, cv.z, cv.w};
+            v[c] = Limbs4{uniform(cv.x), uniform(cv.y), uniform(cv.z), uniform(cv.w)};
         }
         if constexpr (F32) {
             // K2_ZF32_GROUP zeros per lane per compensated add, as in ctile_f32
@@ -364,7 +370,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
 #pragma unroll
                         for (int c = 0; c < ZG; ++c) {
                             uint32_t hi, lo;
-                            phase_window<TB>(g, v[c], hi, lo);
+                            phase_window<TB, true>(g, v[c], hi, lo);
                             float S, C;
                             sincos_window<TB>(hi, lo, tab32, S, C);
                             add_terms<FL>(tp[c], S, C, a, b, e);
@@ -384,7 +390,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
 #pragma unroll
                 for (int c = 0; c < ZG; ++c) {
                     uint32_t off;
-                    const uint64_t W = phase_word(g, v[c], off);
+                    const uint64_t W = phase_word<true>(g, v[c], off);
                     double S, C;
                     sincos_tab64_at((uint32_t)W, (uint32_t)(W >> 32), off, tab64, S, C, kc);
                     accumulate<FL>(acc[c], S, C, a, b, e);
