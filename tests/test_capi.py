@@ -61,3 +61,17 @@ def test_error_path_without_device_work():
     # a null output handle is rejected before any HIP call
     assert lib.k2_table_create(None, 10, None, None, None) == 1
     assert b"null" in lib.k2_last_error()
+
+
+def test_header_is_plain_c_and_cxx():
+    """The boundary header compiles on its own as strict C99 and as C++17 (a C caller, cgo or a
+    ctypes user's C shim includes it without HIP or torch headers)."""
+    import shutil
+    import pytest
+    for cc, args in (("gcc", ["-std=c99", "-pedantic", "-x", "c"]),
+                     ("g++", ["-std=c++17", "-x", "c++"])):
+        if shutil.which(cc) is None:
+            pytest.skip(f"{cc} not available")
+        r = subprocess.run([cc, *args, "-Wall", "-Wextra", "-Werror", "-fsyntax-only", HEADER],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
