@@ -8,8 +8,10 @@
 // truncated vectors).  The exact phase and the sin/cos are computed once per candidate and zero
 // and shared by all vectors: each further vector costs one FMA per requested sum.
 //
-// C tiles of 256 lanes, one candidate per lane; zero records of 128 zeros staged in LDS (double
-// buffered): G, w, g (derivatives only) and 2 wt_m; split partial rows [split][q*M + m][cand].
+// C tiles of 256 lanes, NC candidates per lane (2 while the NC * sums * M accumulators fit, so
+// that each staged zero record and weight feeds two candidates); zero records of 128 zeros staged
+// in LDS (double buffered): G, w, g (derivatives only) and 2 wt_m; split partial rows
+// [split][q*M + m][cand].
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -33,7 +35,7 @@ struct MwSmem {
     double wt[2][M][K2_MW_TZ];   // 2 wt_m[j]
 };
 
-template <int M, int FL>
+template <int M, int FL, int NC>
 __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
     const uint4* __restrict__ zg, const double2* __restrict__ zw, const double* __restrict__ zgam,
     const double* __restrict__ wt, int ld, int nz, int zps, const uint4* __restrict__ cand,
@@ -50,15 +52,19 @@ __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
     const int z0 = split * zps, z1 = min(nz, z0 + zps);
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB; i += 256) s_tab[i] = tab64[i];
-    const int c = ct * 256 + tid;
-    uint4 cv = make_uint4(0, 0, 0, 0);
-    if (c < n_cand) cv = cand[c];
-    const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
-    double acc[NQ][M];
+    Limbs4 v[NC];
+    double acc[NC][NQ][M];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * 256 * NC + i * 256 + tid;
+        uint4 cv = make_uint4(0, 0, 0, 0);
+        if (c < n_cand) cv = cand[c];
+        v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
 #pragma unroll
-        for (int m = 0; m < M; ++m) acc[q][m] = 0.0;
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) acc[i][q][m] = 0.0;
+    }
     // staging: lanes 0..127 carry G, w, g of one zero; lanes 128..255 its M weights
     const int zl = tid & (K2_MW_TZ - 1);
     const bool rec = tid < K2_MW_TZ;
@@ -95,28 +101,33 @@ __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
         if (t + 1 < ntiles) fetch(t + 1);
-        for (int z = 0; z < K2_MW_TZ; ++z) {
+        const int zn = min(K2_MW_TZ, z1 - z0 - t * K2_MW_TZ);   // zeros of this tile
+        for (int z = 0; z < zn; ++z) {
             const uint4 gg = sm.g[buf][z];
             const double2 ww = sm.w[buf][z];
-            uint32_t off;
-            const uint64_t W = phase_word(Limbs4{gg.x, gg.y, gg.z, gg.w}, v, off);
-            double S, C;
-            sincos_tab64_at((uint32_t)W, (uint32_t)(W >> 32), off, s_tab, S, C, kc);
-            const double A = fma(ww.x, C, -ww.y * S);          // Re(w e^{i th})
+            double wz[M];
 #pragma unroll
-            for (int m = 0; m < M; ++m) acc[0][m] = fma(sm.wt[buf][m][z], A, acc[0][m]);
-            if constexpr (DER) {
-                const double g = sm.gam[buf][z];
+            for (int m = 0; m < M; ++m) wz[m] = sm.wt[buf][m][z];
+            const double g = DER ? sm.gam[buf][z] : 0.0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                uint32_t off;
+                const uint64_t W = phase_word(Limbs4{gg.x, gg.y, gg.z, gg.w}, v[i], off);
+                double S, C;
+                sincos_tab64_at((uint32_t)W, (uint32_t)(W >> 32), off, s_tab, S, C, kc);
+                const double A = fma(ww.x, C, -ww.y * S);          // Re(w e^{i th})
+#pragma unroll
+                for (int m = 0; m < M; ++m) acc[i][0][m] = fma(wz[m], A, acc[i][0][m]);
                 if constexpr (D1) {
                     const double gB = g * fma(ww.x, S, ww.y * C);   // g Im(w e^{i th})
 #pragma unroll
-                    for (int m = 0; m < M; ++m) acc[1][m] = fma(-sm.wt[buf][m][z], gB, acc[1][m]);
+                    for (int m = 0; m < M; ++m) acc[i][1][m] = fma(-wz[m], gB, acc[i][1][m]);
                 }
                 if constexpr (D2) {
                     const double ggA = g * g * A;
 #pragma unroll
                     for (int m = 0; m < M; ++m)
-                        acc[NQ - 1][m] = fma(-sm.wt[buf][m][z], ggA, acc[NQ - 1][m]);
+                        acc[i][NQ - 1][m] = fma(-wz[m], ggA, acc[i][NQ - 1][m]);
                 }
             }
         }
@@ -125,9 +136,13 @@ __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
     }
     double* prow = partial + (size_t)split * NQ * M * n_cand_pad;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
+    for (int i = 0; i < NC; ++i) {
+        const int c = ct * 256 * NC + i * 256 + tid;
 #pragma unroll
-        for (int m = 0; m < M; ++m) prow[(size_t)(q * M + m) * n_cand_pad + c] = acc[q][m];
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int m = 0; m < M; ++m) prow[(size_t)(q * M + m) * n_cand_pad + c] = acc[i][q][m];
+    }
 }
 
 // row q*M + m of the split partials, summed over splits in order, into K (q = 0), then K' and
@@ -148,11 +163,19 @@ __global__ __launch_bounds__(256) void k2_multiw_reduce_kernel(
 
 namespace k2impl {
 
+// candidates per lane for M vectors and these sums
+static int mw_nc(int M, unsigned fl) {
+    const int nq = 1 + ((fl & K2_D1) ? 1 : 0) + ((fl & K2_D2) ? 1 : 0);
+    return nq * M <= 10 ? 2 : 1;
+}
+
 template <int M, int FL>
 static hipError_t launch_mw_t(dim3 grid, const k2_table* t, const double* wt, int ld, int nz,
                               int zps, const uint4* cand, int n_cand, int n_ctiles,
                               double* partial, int n_cand_pad, hipStream_t st) {
-    hipLaunchKernelGGL((k2_multiw_kernel<M, FL>), grid, dim3(256), 0, st, t->g,
+    constexpr int NQ = 1 + ((FL & K2_D1) ? 1 : 0) + ((FL & K2_D2) ? 1 : 0);
+    constexpr int NC = NQ * M <= 10 ? 2 : 1;
+    hipLaunchKernelGGL((k2_multiw_kernel<M, FL, NC>), grid, dim3(256), 0, st, t->g,
                        (const double2*)t->w, t->gamma, wt, ld, nz, zps, cand, n_cand, n_ctiles,
                        partial, n_cand_pad, t->tab64);
     return hipGetLastError();
@@ -207,8 +230,9 @@ extern "C" int k2_eval_weights_device(k2_table* t, uint32_t n_cand, const uint64
     std::lock_guard<std::mutex> lk(t->mu);
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipSetDevice(t->device);
-    const int n_ctiles = (int)((n_cand + 255) / 256);
-    const int n_cand_pad = n_ctiles * 256;
+    const int cw = 256 * mw_nc((int)n_w, flags);   // candidates per C tile
+    const int n_ctiles = (int)((n_cand + cw - 1) / cw);
+    const int n_cand_pad = n_ctiles * cw;
     int n_splits = 0, zps = 0;
     split_range((int)n_zeros, n_ctiles, t->tiling.target_blocks, &n_splits, &zps);
     const int nq = 1 + ((flags & K2_D1) ? 1 : 0) + ((flags & K2_D2) ? 1 : 0);
