@@ -70,9 +70,10 @@ __global__ __launch_bounds__(256) void k2_abs_blocks_kernel(const double2* __res
 // sums); the mixed form launches an FP64 head and an FP32 tail side by side.
 // NC <= 2 keeps to 128 VGPRs so that four workgroups share a CU; NT = 512 puts twice the waves
 // on one staged tile and one sin/cos table (LDS is what limits the workgroups per CU), walks the
-// zeros without unrolling and stays within 80 VGPRs (six waves per SIMD)
+// zeros without unrolling and stays within 80 VGPRs (six waves per SIMD; 96 and five waves with
+// K2'', whose third sum would spill at 80)
 template <int NC, int FL, bool F32, int NT>
-__global__ __launch_bounds__(NT, (NT == 512 ? 6 : (NC <= 2 ? 4 : 2))) void k2_eval_kernel(
+__global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : 6) : (NC <= 2 ? 4 : 2))) void k2_eval_kernel(
     ZeroView zv, int z_begin, int z_end, int zps, const uint4* __restrict__ cand, int n_cand,
     int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
     const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
