@@ -251,6 +251,33 @@ def run_eval(args, form):
         res["vs_plain"] = {"plain_ms_per_step": ms64, "speedup": ms64 / ms,
                            "max_abs_dK2_vs_fp64_kernel_all_candidates": float(d.max().item()),
                            "mean_abs_dK2_vs_fp64_kernel": float(d.mean().item())}
+    if form == "plain" and args.workload == "metric" and not args.no_check:
+        # the precision form on the same resident inputs, timed the same way: speed-up over the
+        # FP64 line above and its accuracy (oracle sample and every candidate vs FP64)
+        tm = ev.K2Table(tb)
+        autotune.apply_tuned(tm, n_cand, tb.n, "mixed")
+        outm = tuple(torch.empty(n_cand, dtype=torch.float64, device="cuda") for _ in range(3))
+        flm = ev.VALUE | ev.D1 | ev.MIXED
+
+        def stepm():
+            tm.eval_device(Vd, c["k"], flags=flm, out=outm, stream=st)
+
+        msm, _ = timed(stepm, args.steps, args.warmup, torch)
+        step()
+        stepm()
+        torch.cuda.synchronize()
+        d = (outm[0] - out[0]).abs()
+        accm = None
+        if os.path.exists(gpath):
+            g = np.load(gpath, allow_pickle=False)
+            m = min(len(g["K"]), n_cand)
+            if np.array_equal(g["V"][:m], V[:m]):
+                accm = float(np.max(np.abs(outm[0][:m].cpu().numpy() - g["K"][:m])))
+        res["mixed_form"] = {"value": evals / (msm * 1e-3), "ms_per_step": msm,
+                             "speedup_vs_fp64": ms / msm, "dtype": "f32+f64",
+                             "fp64_head_zeros": tm.mixed_head(c["k"]),
+                             "max_abs_dK2_vs_oracle": accm,
+                             "max_abs_dK2_vs_fp64_all_candidates": float(d.max().item())}
     if not args.no_cpu and args.workload == "metric":
         try:
             cb = cpu_baseline(tb, V, c["k"])
