@@ -137,7 +137,9 @@ int k2_batch_destroy(k2_batch *b);
 /* Newton polish: n_steps Newton iterations on K2' = 0 from each V (device buffers, V updated in
  * place).  Step rule of path1_eval_fast.cpp:120-128 (|K''| > 1e-12 guard, step = K'/K''
  * clamped to +-clamp, stop once |step| < 1e-13); the FP64 step is applied exactly to the
- * 64.64 word.  K/dK/d2K receive the values at the final point; steps_dev the iterations taken. */
+ * 64.64 word.  With K2_MIXED the sums run in the precision form and a candidate stops once
+ * |step| < 1e-11, the form's noise floor in v (FP32 rounding in K2' moves converged points by
+ * ~5e-12).  K/dK/d2K receive the values at the final point; steps_dev the iterations taken. */
 int k2_polish_device(k2_table *t, uint32_t n_cand, uint64_t *V_dev, double k, uint32_t n_zeros,
                      int n_steps, double clamp, uint32_t flags, double *K_dev, double *dK_dev,
                      double *d2K_dev, int32_t *steps_dev, void *stream);

@@ -125,7 +125,9 @@ def newton_polish(table: ev.K2Table, V0: torch.Tensor, k: float, n_steps: int = 
 def polish(table: ev.K2Table, V0: torch.Tensor, k: float, n_steps: int = 8, clamp: float = 1e-4,
            n_zeros=None, mixed: bool = False):
     """Device Newton polish (k2_polish_device: one fused K/K'/K'' pass + an exact 64.64 update
-    per step, no host round trips).  Same rule and return values as :func:`newton_polish`."""
+    per step, no host round trips).  Same rule and return values as :func:`newton_polish`; with
+    mixed=True the sums run in the precision form and a candidate stops once |step| < 1e-11, the
+    noise floor that form's FP32 rounding leaves in v."""
     K0, _, _ = table.eval_device(V0, k, n_zeros=n_zeros, flags=ev.VALUE)
     K0 = K0.clone()
     V = V0.clone()

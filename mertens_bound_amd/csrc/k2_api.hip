@@ -555,7 +555,8 @@ extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, d
         int n_act = (int)n_cand;
         if (it > 0) {
             // only candidates still moving are evaluated again (the reference stops each peak
-            // once its step falls below 1e-13 or its K'' vanishes)
+            // once its step falls below 1e-13 or its K'' vanishes; the precision form stops at
+            // 1e-11, below which its FP32 rounding in K2' moves the point by as much)
             HIPCHK(hipMemcpyAsync(act.data(), t->polish_active, (size_t)n_cand * 4,
                                   hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -580,7 +581,8 @@ extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, d
         }
         evals += n_act;
         std::lock_guard<std::mutex> lk(t->mu);
-        HIPCHK(launch_newton((uint4*)V_dev, dK_dev, d2K_dev, (int)n_cand, clamp, t->polish_active,
+        HIPCHK(launch_newton((uint4*)V_dev, dK_dev, d2K_dev, (int)n_cand, clamp,
+                             (flags & K2_MIXED) ? K2_MIXED_STEP_TOL : 1e-13, t->polish_active,
                              steps_dev, st));
     }
     evals += n_cand;

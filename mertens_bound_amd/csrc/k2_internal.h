@@ -27,6 +27,7 @@ struct WeightArrays {
 
 #define K2_ABS_BLOCK 4096   // zeros per weight-magnitude block (a multiple of the table padding)
 #define K2_MIXED_TAIL 0.16  // default FP32-tail magnitude of the mixed form
+#define K2_MIXED_STEP_TOL 1e-11  // Newton stop rule of the precision form's polish (its noise floor)
 #define K2_Z_CZ 8           // candidates per Z-tile workgroup of the one-launch path
 #define K2_Z_MAX_CAND 384   // default: candidate counts up to this use Z tiles (measured crossover)
 #define K2_Z_CHUNK 8192     // fewest zeros per Z-tile workgroup on a full grid
@@ -157,7 +158,7 @@ hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand
 hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const k2dev::Tab64* tab64,
                                const k2dev::Tab32* tab32, double* out, hipStream_t st);
 hipError_t launch_newton(uint4* cand, const double* dK, const double* d2K, int n, double clamp,
-                         int* active, int* steps, hipStream_t st);
+                         double tol, int* active, int* steps, hipStream_t st);
 hipError_t launch_ref_peak_ab(int n_peaks, const uint64_t* const* p, const double* dy0,
                               int n_zeros, const double* alpha, const double* gamma, double* A,
                               double* B, hipStream_t st);

@@ -302,10 +302,11 @@ __global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __
 
 // ---- Newton update on 64.64 points (polish) ---------------------------------------------------
 // Rule of search/path1_eval_fast.cpp:120-128: require |K''| > 1e-12, step = K'/K'' clamped to
-// +-clamp, v -= step, stop once |step| < 1e-13.  A candidate whose flag is 0 has stopped.
+// +-clamp, v -= step, stop once |step| < tol (the reference's 1e-13; the precision form stops at
+// its own noise floor, see k2_polish_device).  A candidate whose flag is 0 has stopped.
 __global__ __launch_bounds__(256) void k2_newton_kernel(
     uint4* __restrict__ cand, const double* __restrict__ dK, const double* __restrict__ d2K,
-    int n, double clamp, int* __restrict__ active, int* __restrict__ steps) {
+    int n, double clamp, double tol, int* __restrict__ active, int* __restrict__ steps) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n || !active[c]) return;
     const double fpp = d2K[c];
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(256) void k2_newton_kernel(
     const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
     cand[c] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
     steps[c] += 1;
-    if (fabs(step) < 1e-13) active[c] = 0;
+    if (fabs(step) < tol) active[c] = 0;
 }
 
 // ---- polish compaction: the candidates still moving, gathered and their sums scattered back -----
@@ -545,9 +546,9 @@ hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const Tab64* 
 }
 
 hipError_t launch_newton(uint4* cand, const double* dK, const double* d2K, int n, double clamp,
-                         int* active, int* steps, hipStream_t st) {
+                         double tol, int* active, int* steps, hipStream_t st) {
     hipLaunchKernelGGL(k2_newton_kernel, dim3((n + 255) / 256), dim3(256), 0, st, cand, dK, d2K, n,
-                       clamp, active, steps);
+                       clamp, tol, active, steps);
     return hipGetLastError();
 }
 
