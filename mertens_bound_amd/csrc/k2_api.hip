@@ -24,11 +24,12 @@ static int fail(int code, const std::string& msg);
 
 void k2impl::sincos_tables(k2dev::Tab64* t64, k2dev::Tab32* t32) {
     const long double two_pi = 6.283185307179586476925286766559005768L;
-    const long double c64 = two_pi / 18446744073709551616.0L;   // 2 pi / 2^64
+    // interval midpoints, shifted by the rounding of the remainder's offset constant:
+    // (2^52 + 2^51) K - M is exact in long double (both are doubles of about the same size)
+    const long double shift = 6755399441055744.0L * (long double)K2_REM_K - (long double)K2_REM_M;
     for (int k = 0; k < K2_TAB; ++k) {
-        const long double a = two_pi * (long double)k / (long double)K2_TAB;
-        const long double c = cosl(a), s = sinl(a);
-        t64[k] = k2dev::Tab64{(double)c, (double)s, (double)(c * c64), (double)(s * c64)};
+        const long double a = two_pi * ((long double)k + 0.5L) / (long double)K2_TAB - shift;
+        t64[k] = k2dev::Tab64{(double)cosl(a), (double)sinl(a)};
     }
     for (int k = 0; k < K2_TAB32; ++k) {
         const long double a = two_pi * (long double)k / (long double)K2_TAB32;

@@ -115,15 +115,23 @@ __device__ __noinline__ uint64_t phase_word_exact(Limbs4 g, Limbs4 v) {
     return lo | hi;
 }
 
-// byte offset of the 32-byte FP64 table entry nearest to the phase word W (K2_TAB points)
-__device__ __forceinline__ uint32_t tab64_offset(uint64_t W) {
-    return ((((uint32_t)(W >> 32)) + (1u << 22)) >> 18) & 0x3FE0u;
+// Remainder words of W for sincos_tab64_at: r = W bits [2,54) as rhi:rlo (rhi bits [0,20) are
+// W bits [34,54); its higher bits are not used) and off = 16 (W >> 54), the byte offset of the
+// 16-byte table entry of the interval holding the phase
+struct PhaseRem {
+    uint32_t rlo, rhi, off;
+};
+
+__device__ __forceinline__ PhaseRem phase_rem_of(uint64_t W) {
+    return PhaseRem{(uint32_t)(W >> 2), (uint32_t)(W >> 34), (uint32_t)(W >> 50) & 0x3FF0u};
 }
 
-// W with columns 0-1 skipped and the carry case fixed up exactly; off = tab64_offset(W), taken
-// from the column-5 word (W bits [55,64) are its bits [5,14), and rounding adds 2^54 = 16 there)
+// The product columns 2..5 of G V (columns 0-1 skipped): s3lo, s4lo, s5 are bits [96,128),
+// [128,160) and [160,174) (s5 bits above 14 are not formed exactly).  In the carry case
+// (phase_carry_case: bits [98,110) all ones) W must come from phase_word_exact instead.
 template <bool SB = false>
-__device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v, uint32_t& off) {
+__device__ __forceinline__ void phase_columns(Limbs4 g, Limbs4 v, uint32_t& s3lo, uint32_t& s4lo,
+                                              uint32_t& s5) {
     // column 2 (weight 2^64): g0 v2 + g1 v1 + g2 v0; a carry out of a 64-bit add shows as the
     // new sum being below the old one
     uint32_t k2;
@@ -144,24 +152,39 @@ __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v, uint32_t& off
     s4 = mad64<SB>(g.x2, v.x2, s4);
     s4 = mad64<SB>(g.x3, v.x1, s4);
     // column 5 (2^160): only bits [0,14) are needed; 24x24-bit products are exact mod 2^24
-    uint32_t s5 = (uint32_t)(s4 >> 32);
+    s5 = (uint32_t)(s4 >> 32);
     s5 = mad24<SB>(g.x2, v.x3, s5);
     s5 = mad24<SB>(g.x3, v.x2, s5);
-    const uint32_t s3lo = (uint32_t)s3, s4lo = (uint32_t)s4;
-    uint32_t wlo = __builtin_amdgcn_alignbit(s4lo, s3lo, 14);
-    uint32_t whi = __builtin_amdgcn_alignbit(s5, s4lo, 14);
+    s3lo = (uint32_t)s3;
+    s4lo = (uint32_t)s4;
+}
+
+__device__ __forceinline__ bool phase_carry_case(uint32_t s3lo) {
+    return ((s3lo >> 2) & 0xFFFu) == 0xFFFu;
+}
+
+// W with columns 0-1 skipped and the carry case fixed up exactly
+template <bool SB = false>
+__device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
+    uint32_t s3lo, s4lo, s5;
+    phase_columns<SB>(g, v, s3lo, s4lo, s5);
+    const uint32_t wlo = __builtin_amdgcn_alignbit(s4lo, s3lo, 14);
+    const uint32_t whi = __builtin_amdgcn_alignbit(s5, s4lo, 14);
     uint64_t W = ((uint64_t)whi << 32) | wlo;
-    off = (s5 + 16u) & 0x3FE0u;
-    if (__builtin_expect(((s3lo >> 2) & 0xFFFu) == 0xFFFu, 0)) {
-        W = phase_word_exact(g, v);
-        off = tab64_offset(W);
-    }
+    if (__builtin_expect(phase_carry_case(s3lo), 0)) W = phase_word_exact(g, v);
     return W;
 }
 
-__device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
-    uint32_t off;
-    return phase_word(g, v, off);
+// The remainder words of W (phase_rem_of) straight from the columns: W bits [2,54) are product
+// bits [112,164), and W bits [54,64) are bits [4,14) of the column-5 word, so off = s5 & 0x3FF0
+template <bool SB = false>
+__device__ __forceinline__ PhaseRem phase_rem(Limbs4 g, Limbs4 v) {
+    uint32_t s3lo, s4lo, s5;
+    phase_columns<SB>(g, v, s3lo, s4lo, s5);
+    PhaseRem r{__builtin_amdgcn_alignbit(s4lo, s3lo, 16), __builtin_amdgcn_alignbit(s5, s4lo, 16),
+               s5 & 0x3FF0u};
+    if (__builtin_expect(phase_carry_case(s3lo), 0)) r = phase_rem_of(phase_word_exact(g, v));
+    return r;
 }
 
 // ---- FP32 form: the phase window ------------------------------------------------------------
@@ -220,19 +243,25 @@ __device__ __forceinline__ void sincos_poly(double x, double& s, double& c) {
 }
 
 // ---- table-driven sin/cos used by the evaluation kernels ------------------------------------
-// The angle 2 pi W / 2^64 is split into a table point 2 pi k / 512 (k = W rounded to 9 bits) and
-// a remainder |x| <= pi/512 = 6.2e-3.  Table entries hold (cos, sin) of the table points and the
-// same scaled by c = 2 pi / 2^64 (K2_TAB entries, staged in LDS); the remainder enters unscaled as
-// u = x / c, so sin x = c u (1 - c^2 u^2/6 + c^4 u^4/120) (truncation < 1.1e-17 relative) and
-// cos x = 1 - c^2 u^2/2 + c^4 u^4/24 (< 7.5e-17) take their scale factors from the coefficients
-// and the table, and one complex rotation combines the two.  No quadrant selects, no range
-// reduction: the split comes straight from the integer word.
-#define K2_TAB 512
-#define K2_TAB_SHIFT 23   // 64-bit word: index bits 63..55 -> high word bits 31..23
+// The angle 2 pi W / 2^64 is split into the midpoint 2 pi (k + 1/2) / 1024 of its table interval
+// (k = W bits [54,64)) and a remainder |x| <= pi/1024 = 3.1e-3 given by r = W bits [2,54):
+// x = K u with u = r - 2^51 and K = 2 pi / 2^62 (W bits [0,2) are dropped: < 1.4e-18 rad).  The
+// double with high word 0x43300000 | r bits [32,52) and low word r bits [0,32) is d = 2^52 + r
+// exactly, so x = fma(d, K, -M) with M = (2^52 + 2^51) K: one v_and_or_b32 and one FMA from the
+// integer word to radians, no conversions.  K and M are doubles (K2_REM_K, K2_REM_M); the table
+// points are shifted by (2^52 + 2^51) K - M, the part of M lost to rounding (zero for these
+// constants), so the shift is exact.  Table entries hold (cos, sin) of the midpoints (16 bytes,
+// K2_TAB entries, staged in LDS: one ds_read_b128 per evaluation); sin x = x - x^3/6 + x^5/120
+// (truncation < 5e-22) and cos x = 1 - x^2/2 + x^4/24 (< 1.2e-18), and one complex rotation
+// combines the two (measured max error 1.5 u against 2^-53-relative rounding u = 1.1e-16, over
+// 2e7 random words).  No quadrant selects, no range reduction: the split comes straight from the
+// integer word.
+#define K2_TAB 1024
+#define K2_REM_K 1.3624486320346218e-18   // 2 pi / 2^62, rounded to double
+#define K2_REM_M 0.009203884727313847     // (2^52 + 2^51) K2_REM_K, rounded to double
 
 struct Tab64 {
-    double c, s;      // cos, sin of 2 pi k / K2_TAB
-    double cc, sc;    // the same times 2 pi / 2^64
+    double c, s;      // cos, sin of 2 pi (k + 1/2) / K2_TAB (shifted as above)
 };
 
 // FP32 table: cos, sin of 2 pi k / K2_TAB32 (8 bytes, so that the byte offset of entry k is
@@ -244,44 +273,39 @@ struct Tab32 {
     float c, s;
 };
 
-// the remainder polynomials' coefficients (x = u 2 pi / 2^64 folded in); a loop that calls
-// sincos_tab64 many times takes them from sincos_coef() once, so that they stay in VGPRs (an FMA
-// reads at most one scalar operand)
+// constants of sincos_tab64_at; a loop that calls it many times takes them from sincos_coef()
+// once, with -M and -1/6 held in VGPRs (an FMA reads at most one scalar operand, and x and the
+// sine polynomial each take two of these constants)
 struct SinCosCoef {
-    double s1, s2, c1, c2;
+    double k, nm, s1, s2, c2;   // K, -M, -1/6, 1/120, 1/24
 };
 
+#define K2_SINCOS_COEF {K2_REM_K, -K2_REM_M, -1.0 / 6.0, 1.0 / 120.0, 1.0 / 24.0}
+
 __device__ __forceinline__ SinCosCoef sincos_coef() {
-    SinCosCoef k{-1.9336107030552212e-38, 1.121655105290912e-76, -5.800832109165663e-38,
-                 5.608275526454561e-76};
-    asm volatile("" : "+v"(k.s1), "+v"(k.s2), "+v"(k.c1), "+v"(k.c2));
+    SinCosCoef k K2_SINCOS_COEF;
+    asm volatile("" : "+v"(k.nm), "+v"(k.s1));
     return k;
 }
 
-// off: tab64_offset of the word (phase_word hands it over; the kernels' loops save its
-// computation that way)
-__device__ __forceinline__ void sincos_tab64_at(uint32_t wlo, uint32_t whi, uint32_t off,
-                                                const Tab64* tab, double& S, double& C,
-                                                const SinCosCoef& kc) {
-    // sign-extended low K2_TAB_SHIFT bits of whi in one v_bfe_i32 (as asm: the compiler would
-    // otherwise rebuild it from the 64-bit word as a funnel shift plus an arithmetic shift)
-    int32_t rhi;
-    asm("v_bfe_i32 %0, %1, 0, %2" : "=v"(rhi) : "v"(whi), "i"(K2_TAB_SHIFT));
-    const double u = fma((double)rhi, 4294967296.0, (double)wlo);    // |u| <= 2^54
-    const double z = u * u;
-    const double sp = fma(u * z, fma(z, kc.s2, kc.s1), u);
-    const double cp = fma(z, fma(z, kc.c2, kc.c1), 1.0);
-    const Tab64 t = *reinterpret_cast<const Tab64*>(reinterpret_cast<const char*>(tab) + off);
-    C = fma(t.c, cp, -t.sc * sp);
-    S = fma(t.s, cp, t.cc * sp);
+// sin/cos of the phase from its remainder words (phase_rem / phase_rem_of)
+__device__ __forceinline__ void sincos_tab64_at(const PhaseRem& r, const Tab64* tab, double& S,
+                                                double& C, const SinCosCoef& kc) {
+    // (0x43300000 | (rhi & 0xFFFFF)) : rlo = 2^52 + r in one v_and_or_b32 (as asm: the compiler
+    // splits it into an AND and an OR, since neither constant is an inline one)
+    uint32_t hi;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(hi) : "v"(r.rhi), "s"(0xFFFFFu), "v"(0x43300000u));
+    const double x = fma(__hiloint2double((int)hi, (int)r.rlo), kc.k, kc.nm);
+    const double z = x * x;
+    const double sp = fma(x * z, fma(z, kc.s2, kc.s1), x);
+    const double cp = fma(z, fma(z, kc.c2, -0.5), 1.0);
+    const Tab64 t = *reinterpret_cast<const Tab64*>(reinterpret_cast<const char*>(tab) + r.off);
+    C = fma(t.c, cp, -t.s * sp);
+    S = fma(t.s, cp, t.c * sp);
 }
 
-__device__ __forceinline__ void sincos_tab64(uint32_t wlo, uint32_t whi, const Tab64* tab,
-                                             double& S, double& C,
-                                             const SinCosCoef& kc = SinCosCoef{
-                                                 -1.9336107030552212e-38, 1.121655105290912e-76,
-                                                 -5.800832109165663e-38, 5.608275526454561e-76}) {
-    sincos_tab64_at(wlo, whi, tab64_offset(((uint64_t)whi << 32) | wlo), tab, S, C, kc);
+__device__ __forceinline__ void sincos_tab64(uint64_t W, const Tab64* tab, double& S, double& C) {
+    sincos_tab64_at(phase_rem_of(W), tab, S, C, SinCosCoef K2_SINCOS_COEF);
 }
 
 // FP32 sin/cos of the window's phase with a 2^TB-point table (every 2^(11-TB)-th entry of the

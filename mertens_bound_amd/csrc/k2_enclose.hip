@@ -10,9 +10,10 @@
 //   x_j     k g g                                                             rel <= 6 u
 //   e^{-x}  exp_neg_bounded                                                   rel <= 2.5 u
 //   2 e^{-x} w  -> a_j (one product per component)                            |a - A| <= |A| (3 + 6.1 x) u
-//   phase   W = floor(G V / 2^110) mod 2^64 exact; dropped fraction < 2 pi 2^-64 rad
-//   sin/cos sincos_tab64: table rounding 0.5 u, remainder polynomial 1.2 u, combine 0.55 u
-//           -> |S - sin|, |C - cos| <= 2.3 u (bounded here by 3 u)
+//   phase   W = floor(G V / 2^110) mod 2^64 exact; the remainder drops W bits [0,3) and the
+//           fraction below W: < 2 pi 2^-61 rad = 2.8e-18 (0.03 u)
+//   sin/cos sincos_tab64_at: table rounding 0.5 u, remainder polynomial 1.2 u, combine 0.55 u,
+//           the dropped bits 0.03 u -> |S - sin|, |C - cos| <= 2.3 u (bounded here by 3 u)
 //   term    fma(a.x, C, -a.y S): sqrt(2) |a| 3u + 0.5 |a| u + 0.5 |a| u = 5.25 |a| u
 //   =>      |term_j - T_j| <= |a_j| (9 + 6.5 x_j) u                  (K2)
 //           |term'_j - T'_j| <= g_j |a_j| (12 + 6.5 x_j) u           (K2', b = g a adds 3.5 u)

@@ -57,8 +57,9 @@ struct LaunchArgs {
     const k2dev::Tab32* tab32 = nullptr;
 };
 
-// host-side sin/cos tables, rounded from long double: K2_TAB FP64 entries of 2 pi k / K2_TAB
-// (plain and scaled) and K2_TAB32 FP32 entries of 2 pi k / K2_TAB32
+// host-side sin/cos tables, rounded from long double: K2_TAB FP64 entries at the interval
+// midpoints 2 pi (k + 1/2) / K2_TAB (see sincos_tab64_at) and K2_TAB32 FP32 entries of
+// 2 pi k / K2_TAB32
 void sincos_tables(k2dev::Tab64* t64, k2dev::Tab32* t32);
 // device copies of those tables
 int upload_tables(k2dev::Tab64** t64, k2dev::Tab32** t32, hipStream_t st);
@@ -91,7 +92,7 @@ struct k2_table {
     uint4* g = nullptr;
     double* gamma = nullptr;
     double* w = nullptr;
-    k2dev::Tab64* tab64 = nullptr;   // sin/cos tables of 2 pi k / K2_TAB
+    k2dev::Tab64* tab64 = nullptr;   // sin/cos tables (sincos_tables)
     k2dev::Tab32* tab32 = nullptr;
     std::vector<WeightArrays> wsets;
     uint64_t use_clock = 0;

@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __
     if (i >= n) return;
     if (mode == 0) {
         double s, c;
-        sincos_tab64((uint32_t)W[i], (uint32_t)(W[i] >> 32), tab64, s, c);
+        sincos_tab64(W[i], tab64, s, c);
         out[2 * i] = s; out[2 * i + 1] = c;
     } else {
         float s, c;

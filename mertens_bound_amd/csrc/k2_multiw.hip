@@ -111,10 +111,8 @@ __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
             const double g = DER ? sm.gam[buf][z] : 0.0;
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                uint32_t off;
-                const uint64_t W = phase_word(Limbs4{gg.x, gg.y, gg.z, gg.w}, v[i], off);
                 double S, C;
-                sincos_tab64_at((uint32_t)W, (uint32_t)(W >> 32), off, s_tab, S, C, kc);
+                sincos_tab64_at(phase_rem(Limbs4{gg.x, gg.y, gg.z, gg.w}, v[i]), s_tab, S, C, kc);
                 const double A = fma(ww.x, C, -ww.y * S);          // Re(w e^{i th})
 #pragma unroll
                 for (int m = 0; m < M; ++m) acc[i][0][m] = fma(wz[m], A, acc[i][0][m]);

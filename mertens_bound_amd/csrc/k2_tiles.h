@@ -204,10 +204,8 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
             const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                uint32_t off;
-                const uint64_t W = phase_word(g, v[i], off);
                 double S, C;
-                sincos_tab64_at((uint32_t)W, (uint32_t)(W >> 32), off, tab, S, C, kc);
+                sincos_tab64_at(phase_rem(g, v[i]), tab, S, C, kc);
                 accumulate<FL>(acc[i], S, C, aa, bb, ee);
             }
         }
@@ -389,10 +387,8 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                 const double2 e = D2 ? zv.e[j] : make_double2(0.0, 0.0);
 #pragma unroll
                 for (int c = 0; c < ZG; ++c) {
-                    uint32_t off;
-                    const uint64_t W = phase_word<true>(g, v[c], off);
                     double S, C;
-                    sincos_tab64_at((uint32_t)W, (uint32_t)(W >> 32), off, tab64, S, C, kc);
+                    sincos_tab64_at(phase_rem<true>(g, v[c]), tab64, S, C, kc);
                     accumulate<FL>(acc[c], S, C, a, b, e);
                 }
             }
