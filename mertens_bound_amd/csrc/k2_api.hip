@@ -91,6 +91,7 @@ extern "C" int k2_table_create(k2_table** out, uint32_t n_zeros, const uint64_t*
     t->n_pad = (n_zeros + K2_PAD - 1) / K2_PAD * K2_PAD;
     const size_t np = t->n_pad;
     hipError_t e = hipMalloc((void**)&t->g, np * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->gf, np * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc((void**)&t->gamma, np * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&t->w, np * 2 * sizeof(double));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
@@ -103,6 +104,14 @@ extern "C" int k2_table_create(k2_table** out, uint32_t n_zeros, const uint64_t*
         e = hipMemcpyAsync(t->gamma, gamma, (size_t)n_zeros * 8, hipMemcpyHostToDevice, t->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(t->w, w, (size_t)n_zeros * 16, hipMemcpyHostToDevice, t->stream);
+    std::vector<float4> gf(np, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (uint32_t j = 0; j < n_zeros; ++j) {
+        const uint64_t lo = G[2 * j], hi = G[2 * j + 1];
+        gf[j] = make_float4((float)(uint32_t)lo, (float)(uint32_t)(lo >> 32), (float)(uint32_t)hi,
+                            (float)(uint32_t)(hi >> 32));
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->gf, gf.data(), np * sizeof(float4), hipMemcpyHostToDevice, t->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
     if (e != hipSuccess) {
         k2_table_destroy(t);
@@ -124,7 +133,7 @@ extern "C" int k2_table_destroy(k2_table* t) {
     for (auto& ws : t->wsets) free_weights(ws);
     for (auto& pp : t->prof_pending) { (void)hipEventDestroy(pp.a); (void)hipEventDestroy(pp.b); }
     for (auto ev : t->ev_pool) (void)hipEventDestroy(ev);
-    (void)hipFree(t->g); (void)hipFree(t->gamma); (void)hipFree(t->w);
+    (void)hipFree(t->g); (void)hipFree(t->gf); (void)hipFree(t->gamma); (void)hipFree(t->w);
     (void)hipFree(t->tab64); (void)hipFree(t->tab32);
     (void)hipFree(t->partial); (void)hipFree(t->host_io); (void)hipFree(t->polish_active);
     (void)hipFree(t->polish_buf);
@@ -328,6 +337,7 @@ static void fill_args(k2_table* t, WeightArrays* ws, const uint4* cand, int n_ca
     a.nc = t->tiling.nc;
     a.flags = flags;
     a.zg = t->g;
+    a.zgf = t->gf;
     a.za = ws->a; a.zb = ws->b; a.ze = ws->e;
     a.zaf = ws->af; a.zbf = ws->bf; a.zef = ws->ef;
     a.cand = cand;
@@ -395,6 +405,7 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
     a.nt = t->tiling.wg_threads;
     a.flags = flags;
     a.zg = t->g;
+    a.zgf = t->gf;
     a.za = ws->a; a.zb = ws->b; a.ze = ws->e;
     a.zaf = ws->af; a.zbf = ws->bf; a.zef = ws->ef;
     a.cand = cand;
@@ -615,7 +626,7 @@ extern "C" int k2_phase_words(k2_table* t, uint32_t n_cand, const uint64_t* V, u
                               uint32_t nz, int mode, uint64_t* out) {
     if (!t || (n_cand && nz && (!V || !out))) return fail(K2_E_ARG, "k2_phase_words: null pointer");
     if ((uint64_t)z0 + nz > t->n) return fail(K2_E_ARG, "k2_phase_words: zero range out of table");
-    if (mode < 0 || mode > 2) return fail(K2_E_ARG, "k2_phase_words: mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(K2_E_ARG, "k2_phase_words: mode must be 0..3");
     if ((uint64_t)n_cand * nz > ((uint64_t)1 << 31))
         return fail(K2_E_ARG, "k2_phase_words: n_cand * nz too large");
     if (n_cand == 0 || nz == 0) return K2_OK;
@@ -627,7 +638,8 @@ extern "C" int k2_phase_words(k2_table* t, uint32_t n_cand, const uint64_t* V, u
     uint4* dV = (uint4*)t->host_io;
     uint64_t* dO = (uint64_t*)((char*)t->host_io + nb_v);
     HIPCHK(hipMemcpyAsync(dV, V, nb_v, hipMemcpyHostToDevice, t->stream));
-    HIPCHK(launch_phase_words(t->g, (int)z0, (int)nz, dV, (int)n_cand, mode, dO, t->stream));
+    HIPCHK(launch_phase_words(t->g, t->gf, (int)z0, (int)nz, dV, (int)n_cand, mode, dO,
+                              t->stream));
     HIPCHK(hipMemcpyAsync(out, dO, nb_o, hipMemcpyDeviceToHost, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
     return K2_OK;

@@ -413,7 +413,7 @@ extern "C" int k2_batch_create(k2_table* t, uint32_t n_req, const k2_request* re
     b->n_items = (int)P.items.size();
     for (size_t i = 0; i < 4; ++i) {
         const WeightArrays& w = b->w.empty() ? WeightArrays{} : b->w[std::min(i, b->w.size() - 1)];
-        b->views.zv[i] = ZeroView{t->g, w.a, w.b, w.e, w.af, w.bf, w.ef};
+        b->views.zv[i] = ZeroView{t->g, w.a, w.b, w.e, w.af, w.bf, w.ef, t->gf};
     }
     BCHK(hipMalloc((void**)&b->d_counter, 256));
     BCHK(hipMalloc((void**)&b->d_items, std::max<size_t>(1, P.items.size()) * sizeof(BatchItem)));

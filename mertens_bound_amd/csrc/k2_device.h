@@ -215,6 +215,40 @@ __device__ __forceinline__ void phase_window(Limbs4 g, Limbs4 v, uint32_t& hi, u
     lo = (uint32_t)s4;
 }
 
+// The window of phase_window with column 3 in FP32.  Column 3 adds S3 2^96 with S3 < 2^66, less
+// than 2^-12 turn: it never wraps, so it needs no exact integer sum, only precision well below the
+// FP32 kernels' 2^-35-turn remainder.  gf holds g0..g3 as floats, vf holds v0..v3 times 2^-34 as
+// floats; four FP32 FMAs give S3 / 2^34 < 2^32 (worst-case error 2^-34 turn, random rounding),
+// and its integer part enters column 4 as 4 q (units of 2^130; truncation < 2^-44 turn).  This
+// takes four FP32 FMAs, a conversion and one multiply-add where the exact column 3 takes four
+// 64-bit multiply-adds and its carry bookkeeping.
+template <int TB, bool SB = false>
+__device__ __forceinline__ void phase_window_f(Limbs4 g, float4 gf, Limbs4 v, float4 vf,
+                                               uint32_t& hi, uint32_t& lo) {
+    static_assert(TB >= 9 && TB <= 11, "table of 512..2048 points");
+    float t = gf.w * vf.x;          // g3 v0
+    t = fmaf(gf.z, vf.y, t);        // g2 v1
+    t = fmaf(gf.y, vf.z, t);        // g1 v2
+    t = fmaf(gf.x, vf.w, t);        // g0 v3
+    const uint32_t q = (uint32_t)t;
+    // column 4 (2^128) with half a table step, 2^(173-TB) = 2^(45-TB) 2^128, then 4 q
+    uint64_t s4 = mad64<SB>(g.x1, v.x3, 1ull << (45 - TB));
+    s4 = mad64<SB>(g.x2, v.x2, s4);
+    s4 = mad64<SB>(g.x3, v.x1, s4);
+    asm("v_mad_u64_u32 %0, vcc, %1, 4, %2" : "=v"(s4) : "v"(q), "v"(s4) : "vcc");
+    uint32_t s5 = (uint32_t)(s4 >> 32);
+    s5 = mad24<SB>(g.x2, v.x3, s5);
+    s5 = mad24<SB>(g.x3, v.x2, s5);
+    hi = s5;
+    lo = (uint32_t)s4;
+}
+
+// per-candidate operand of phase_window_f: v0..v3 times 2^-34 as floats
+__device__ __forceinline__ float4 window_vf(Limbs4 v) {
+    constexpr float s = 5.8207660913467407e-11f;   // 2^-34
+    return make_float4((float)v.x0 * s, (float)v.x1 * s, (float)v.x2 * s, (float)v.x3 * s);
+}
+
 // ---- FP64 sin/cos of 2 pi W / 2^64 ---------------------------------------------------------
 #define K2_TWO_PI_2M64 3.4061215800865545e-19   // 2 pi / 2^64 (correctly rounded)
 

@@ -266,7 +266,7 @@ extern "C" int k2_enclose_device(k2_table* t, uint32_t n_cand, const uint64_t* V
         e = hipGetLastError();
     }
     if (e == hipSuccess && n_cand && n_splits > 0) {
-        ZeroView zv{t->g, t->enc_a, t->enc_b, nullptr, nullptr, nullptr, nullptr};
+        ZeroView zv{t->g, t->enc_a, t->enc_b, nullptr, nullptr, nullptr, nullptr, nullptr};
         const dim3 grid(n_ctiles * n_splits);
         const uint4* cand = (const uint4*)V_dev;
         if (nc == 1)

@@ -48,6 +48,7 @@ struct LaunchArgs {
     const float2* zaf = nullptr;
     const float2* zbf = nullptr;
     const float2* zef = nullptr;
+    const float4* zgf = nullptr;
     int z_begin = 0, z_end = 0, zps = 0;   // zero range and zeros per split of this launch
     const uint4* cand = nullptr;
     int n_cand = 0, n_ctiles = 0;
@@ -90,6 +91,7 @@ struct k2_table {
     int device = 0;
     uint32_t n = 0, n_pad = 0;
     uint4* g = nullptr;
+    float4* gf = nullptr;   // the limbs of g as floats (column 3 of the FP32 window)
     double* gamma = nullptr;
     double* w = nullptr;
     k2dev::Tab64* tab64 = nullptr;   // sin/cos tables (sincos_tables)
@@ -154,8 +156,9 @@ hipError_t launch_reduce_z(const double* partial, int n_rows, int n_groups, int 
                            unsigned flags, double* K, double* dK, double* d2K, hipStream_t st);
 hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_cand_pad,
                          unsigned flags, double* K, double* dK, double* d2K, hipStream_t st);
-hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand, int n_cand,
-                              int mode, uint64_t* out, hipStream_t st);
+hipError_t launch_phase_words(const uint4* zg, const float4* zgf, int z0, int nz,
+                              const uint4* cand, int n_cand, int mode, uint64_t* out,
+                              hipStream_t st);
 hipError_t launch_sincos_words(const uint64_t* W, int n, int mode, const k2dev::Tab64* tab64,
                                const k2dev::Tab32* tab32, double* out, hipStream_t st);
 hipError_t launch_newton(uint4* cand, const double* dK, const double* d2K, int n, double clamp,

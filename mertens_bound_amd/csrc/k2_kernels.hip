@@ -73,7 +73,8 @@ __global__ __launch_bounds__(256) void k2_abs_blocks_kernel(const double2* __res
 // zeros without unrolling and stays within 80 VGPRs (six waves per SIMD; 96 and five waves with
 // K2'', whose third sum would spill at 80)
 template <int NC, int FL, bool F32, int NT>
-__global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : 6) : (NC <= 2 ? 4 : 2))) void k2_eval_kernel(
+__global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : (F32 && NC == 2 ? 8 : 6))
+                                            : (NC <= 2 ? 4 : 2))) void k2_eval_kernel(
     ZeroView zv, int z_begin, int z_end, int zps, const uint4* __restrict__ cand, int n_cand,
     int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
     const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
@@ -257,11 +258,12 @@ __global__ __launch_bounds__(256) void k2_reduce_kernel(
 }
 
 // ---- debug: phase words --------------------------------------------------------------------------
-// mode 0: fast W (skip + fix-up), 1: schoolbook W, 2: the FP32 form's phase window, bits
-// [128,174) of the partial product (the phase floored to 2^-46 turn, low by at most 4 units)
+// mode 0: fast W (skip + fix-up), 1: schoolbook W, 2: the exact-column phase window, bits
+// [128,174) of the partial product (the phase floored to 2^-46 turn, low by at most 4 units),
+// 3: the FP32 kernels' window (column 3 in FP32, phase_window_f), in the same units
 __global__ __launch_bounds__(256) void k2_phase_words_kernel(
-    const uint4* __restrict__ zg, int z0, int nz, const uint4* __restrict__ cand, int n_cand,
-    int mode, uint64_t* __restrict__ out) {
+    const uint4* __restrict__ zg, const float4* __restrict__ zgf, int z0, int nz,
+    const uint4* __restrict__ cand, int n_cand, int mode, uint64_t* __restrict__ out) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)n_cand * nz) return;
     const int c = (int)(idx / nz), j = (int)(idx - (long long)c * nz);
@@ -272,7 +274,8 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     else if (mode == 1) W = phase_word_exact(g, v);
     else {
         uint32_t hi, lo;
-        phase_window<K2_TAB32_BITS>(g, v, hi, lo);
+        if (mode == 2) phase_window<K2_TAB32_BITS>(g, v, hi, lo);
+        else phase_window_f<K2_TAB32_BITS>(g, zgf[z0 + j], v, window_vf(v), hi, lo);
         W = ((((uint64_t)hi << 32) | lo) - (1ull << (45 - K2_TAB32_BITS))) & ((1ull << 46) - 1);
     }
     out[idx] = W;
@@ -376,7 +379,7 @@ namespace k2impl {
 
 template <int NC, int FL, bool F32>
 static hipError_t launch_eval_t(const LaunchArgs& a, hipStream_t st) {
-    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef};
+    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef, a.zgf};
     if constexpr (NC <= 2) {
         if (a.nt == 512) {
             hipLaunchKernelGGL((k2_eval_kernel<NC, FL, F32, 512>), dim3(a.n_blocks), dim3(512), 0,
@@ -428,7 +431,7 @@ hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad
 template <int NC, int FL>
 static hipError_t launch_mixed_t(const LaunchArgs& a, int head_end, int zps_head, int sp_head,
                                  int zps_tail, hipStream_t st) {
-    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef};
+    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef, a.zgf};
     hipLaunchKernelGGL((k2_eval_mixed_kernel<NC, FL>), dim3(a.n_blocks), dim3(K2_ZT), 0, st, zv,
                        head_end, zps_head, sp_head, a.z_end, zps_tail, a.cand, a.n_cand,
                        a.n_ctiles, a.partial, a.n_cand_pad, a.tab64, a.tab32);
@@ -460,7 +463,7 @@ hipError_t launch_eval_mixed(const LaunchArgs& a, int head_end, int zps_head, in
 template <int FL, bool F32>
 static hipError_t launch_eval_z_t(const LaunchArgs& a, int n_groups, int n_chunks, int zpc,
                                   hipStream_t st) {
-    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef};
+    const ZeroView zv{a.zg, a.za, a.zb, a.ze, a.zaf, a.zbf, a.zef, a.zgf};
     hipLaunchKernelGGL((k2_eval_z_kernel<FL, F32>), dim3(n_groups * n_chunks), dim3(K2_ZT), 0, st,
                        zv, a.z_begin, a.z_end, zpc, a.cand, a.n_cand, n_groups, a.partial,
                        a.row_base, a.tab64, a.tab32);
@@ -528,13 +531,14 @@ hipError_t launch_reduce(const double* partial, int n_splits, int n_cand, int n_
     return launch_reduce_t<1>(partial, n_splits, n_cand, n_cand_pad, flags, K, dK, d2K, st);
 }
 
-hipError_t launch_phase_words(const uint4* zg, int z0, int nz, const uint4* cand, int n_cand,
-                              int mode, uint64_t* out, hipStream_t st) {
+hipError_t launch_phase_words(const uint4* zg, const float4* zgf, int z0, int nz,
+                              const uint4* cand, int n_cand, int mode, uint64_t* out,
+                              hipStream_t st) {
     const long long n = (long long)n_cand * nz;
     const long long nb = (n + 255) / 256;
     if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k2_phase_words_kernel, dim3((unsigned)nb), dim3(256), 0, st, zg, z0, nz,
-                       cand, n_cand, mode, out);
+    hipLaunchKernelGGL(k2_phase_words_kernel, dim3((unsigned)nb), dim3(256), 0, st, zg, zgf, z0,
+                       nz, cand, n_cand, mode, out);
     return hipGetLastError();
 }
 

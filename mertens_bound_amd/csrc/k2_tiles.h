@@ -29,6 +29,7 @@ struct ZeroView {
     const float2* af;
     const float2* bf;
     const float2* ef;
+    const float4* gf;   // g0..g3 as floats (the FP32 window's column 3)
 };
 
 // ---- accumulators -------------------------------------------------------------------------
@@ -137,8 +138,11 @@ template <int FL>
 struct CTileSmem32 {
     static constexpr int NE = (FL & K2_D2) ? 1 : 0;
     uint4 g[2][K2_WG];
+    float4 gf[2][K2_WG];
     float4 ab[2][K2_WG];
-    float2 ew[2][NE ? K2_WG : 2];   // 2 entries (16 B) keep the struct 16-byte sized
+    // no entries without K2'' (a zero-length array): the records then take exactly 24 KiB, so
+    // that with the 16 KiB table four workgroups share a CU's 160 KiB
+    float2 ew[2][NE * K2_WG];
     __device__ __forceinline__ float2& e(int buf, int z) { return ew[buf][z]; }
 };
 
@@ -227,33 +231,54 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
     constexpr bool D2 = (FL & K2_D2) != 0;
     const int tid = threadIdx.x;
     const int zl = tid & (K2_WG - 1);
-    const bool lo_half = NT == K2_WG || tid < K2_WG;   // stages g
+    const bool lo_half = NT == K2_WG || tid < K2_WG;   // stages g, gf
     const bool hi_half = NT == K2_WG || tid >= K2_WG;  // stages a|b, e
     const int ntiles = z1 > z0 ? (z1 - z0 + K2_WG - 1) / K2_WG : 0;
-    uint4 rg = make_uint4(0, 0, 0, 0);
-    float4 rab = make_float4(0.f, 0.f, 0.f, 0.f);
-    float2 re = make_float2(0.f, 0.f);
+    float4 vf[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) vf[i] = window_vf(v[i]);
+    // prefetch registers: r0, r1 = g, gf for the first half of the lanes and a|b, e for the
+    // second (with 512 lanes each half stages one kind, so the two share registers)
+    uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
     auto fetch = [&](int t) {
         const int j = z0 + t * K2_WG + zl;
+        uint4 x0 = make_uint4(0, 0, 0, 0), x1 = x0, x2 = x0, x3 = x0;
         if (j < z1) {
-            if (lo_half) rg = zv.g[j];
+            if (lo_half) {
+                x0 = zv.g[j];
+                const float4 f = zv.gf[j];
+                x1 = make_uint4(__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z),
+                                __float_as_uint(f.w));
+            }
             if (hi_half) {
                 const float2 a = zv.af[j];
-                float2 b = make_float2(0.f, 0.f);
+                float2 b = make_float2(0.f, 0.f), e = b;
                 if (D1) b = zv.bf[j];
-                rab = make_float4(a.x, a.y, b.x, b.y);
-                if (D2) re = zv.ef[j];
+                if (D2) e = zv.ef[j];
+                x2 = make_uint4(__float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(b.x),
+                                __float_as_uint(b.y));
+                x3 = make_uint4(__float_as_uint(e.x), __float_as_uint(e.y), 0u, 0u);
             }
+        }
+        if constexpr (NT == 2 * K2_WG) {
+            r0 = lo_half ? x0 : x2;
+            r1 = lo_half ? x1 : x3;
         } else {
-            rg = make_uint4(0, 0, 0, 0); rab = make_float4(0.f, 0.f, 0.f, 0.f);
-            re = make_float2(0.f, 0.f);
+            r0 = x0; r1 = x1; r2 = x2; r3 = x3;
         }
     };
     auto stage = [&](int buf) {
-        if (lo_half) sm.g[buf][zl] = rg;
+        const uint4 g = r0, f = r1;
+        const uint4 ab = NT == 2 * K2_WG ? r0 : r2, e = NT == 2 * K2_WG ? r1 : r3;
+        if (lo_half) {
+            sm.g[buf][zl] = g;
+            sm.gf[buf][zl] = make_float4(__uint_as_float(f.x), __uint_as_float(f.y),
+                                         __uint_as_float(f.z), __uint_as_float(f.w));
+        }
         if (hi_half) {
-            sm.ab[buf][zl] = rab;
-            if (D2) sm.e(buf, zl) = re;
+            sm.ab[buf][zl] = make_float4(__uint_as_float(ab.x), __uint_as_float(ab.y),
+                                         __uint_as_float(ab.z), __uint_as_float(ab.w));
+            if (D2) sm.e(buf, zl) = make_float2(__uint_as_float(e.x), __uint_as_float(e.y));
         }
     };
     if (ntiles > 0) { fetch(0); stage(0); }
@@ -275,6 +300,7 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 #pragma unroll
             for (int h = 0; h < K2_F32_GROUP; ++h) {
                 const uint4 gg = sm.g[buf][z + h];
+                const float4 gf = sm.gf[buf][z + h];
                 const float4 ab = sm.ab[buf][z + h];
                 float2 ee = make_float2(0.f, 0.f);
                 if (D2) ee = sm.e(buf, z + h);
@@ -282,7 +308,7 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
                     uint32_t hi, lo;
-                    phase_window<TB>(g, v[i], hi, lo);
+                    phase_window_f<TB>(g, gf, v[i], vf[i], hi, lo);
                     float S, C;
                     sincos_window<TB>(hi, lo, tab, S, C);
                     add_terms<FL>(tp[i], S, C, make_float2(ab.x, ab.y), make_float2(ab.z, ab.w),
@@ -304,6 +330,10 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 // a value the same in every lane, as an SGPR
 __device__ __forceinline__ uint32_t uniform(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+}
+
+__device__ __forceinline__ float uniform_f(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
 
 template <int CZ>
@@ -343,12 +373,17 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
     for (int cp = 0; cp < cp_end; cp += ZG) {
         Acc<F32> acc[ZG];
         Limbs4 v[ZG];
+        float4 vf[F32 ? ZG : 1];
 #pragma unroll
         for (int c = 0; c < ZG; ++c) {
             acc[c].zero();
             // the same candidates in every lane: their limbs go to SGPRs
             const uint4 cv = sm.v[cp + c];
             v[c] = Limbs4{uniform(cv.x), uniform(cv.y), uniform(cv.z), uniform(cv.w)};
+            if constexpr (F32) {
+                const float4 f = window_vf(v[c]);
+                vf[c] = make_float4(uniform_f(f.x), uniform_f(f.y), uniform_f(f.z), uniform_f(f.w));
+            }
         }
         if constexpr (F32) {
             // K2_ZF32_GROUP zeros per lane per compensated add, as in ctile_f32
@@ -362,13 +397,14 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                     if (j < z1) {
                         const uint4 gg = zv.g[j];
                         const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+                        const float4 gf = zv.gf[j];
                         const float2 a = zv.af[j];
                         const float2 b = D1 ? zv.bf[j] : make_float2(0.f, 0.f);
                         const float2 e = D2 ? zv.ef[j] : make_float2(0.f, 0.f);
 #pragma unroll
                         for (int c = 0; c < ZG; ++c) {
                             uint32_t hi, lo;
-                            phase_window<TB, true>(g, v[c], hi, lo);
+                            phase_window_f<TB, true>(g, gf, v[c], vf[c], hi, lo);
                             float S, C;
                             sincos_window<TB>(hi, lo, tab32, S, C);
                             add_terms<FL>(tp[c], S, C, a, b, e);

@@ -52,6 +52,7 @@ def test_phase_words_edges(oracle_lib):
     for mode in (0, 1):
         assert np.array_equal(t.phase_words(V, mode=mode), oracle_lib.phase_words(G, V))
     _check_window(t.phase_words(V, mode=2), oracle_lib.phase_words(G, V))
+    _check_window_f(t.phase_words(V, mode=3), oracle_lib.phase_words(G, V))
 
 
 def _check_window(win, Wo):
@@ -93,6 +94,29 @@ def test_sincos_of_words():
     assert np.max(np.abs(s32 - np.sin(th))) < 3e-7 and np.max(np.abs(c32 - np.cos(th))) < 3e-7
 
 
+def _check_window_f(win, Wo):
+    """The FP32 kernels' window sums column 3 in FP32: four products of floats rounded to 24
+    bits, three FMAs, then truncation to units of 2^130.  Worst case |S3 - t 2^34| <= 6 2^-24 S3
+    with S3 < 2^66 (units of 2^96), i.e. 6 2^8 units of 2^130 = 6144 units of bit 128, plus 4
+    for the truncation and 4 for columns 0..2: |window - floor(W / 2^18)| <= 6152 (mod 2^46).
+    The rounding is to nearest, so the mean error stays near the truncation's -2.5 units."""
+    m46 = (1 << 46) - 1
+    assert np.all(win <= np.uint64(m46))
+    d = ((win.astype(object) - (Wo >> np.uint64(18)).astype(object)) & m46)
+    d = np.array([x - (1 << 46) if x >= (1 << 45) else x for x in d.ravel()], dtype=np.int64)
+    assert np.abs(d).max() <= 6152
+    assert abs(d.mean()) <= 64
+    return d
+
+
+def test_phase_window_f_within_bound(small, oracle_lib):
+    tb, t = small
+    V = synth.make_candidates(512, 3.4e18, 3.6e18, 3)
+    d = _check_window_f(t.phase_words(V, mode=3), oracle_lib.phase_words(tb.G, V))
+    # FP32 rounding moves the window by tens to thousands of units, so the bound is exercised
+    assert np.abs(d).max() > 64
+
+
 def test_phase_words_full_table(oracle_lib):
     """All 2,325,572 zeros of the full-size table against candidates spread over
     [3.4e18, 3.6e18]: every 64-bit word bit-exact, every FP32 window within 4 units."""
@@ -105,3 +129,4 @@ def test_phase_words_full_table(oracle_lib):
     Wo = oracle_lib.phase_words(tb.G, V)
     assert np.array_equal(t.phase_words(V, mode=0), Wo)
     _check_window(t.phase_words(V, mode=2), Wo)
+    _check_window_f(t.phase_words(V, mode=3), Wo)
