@@ -218,7 +218,7 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
     }
 }
 
-#define K2_F32_GROUP 8   // zeros per compensated add in the FP32 C tile (divides K2_WG)
+#define K2_F32_GROUP 16   // zeros per compensated add in the FP32 C tile (divides K2_WG)
 
 // FP32 form of ctile_f64: the phase window, a 2^TB-point FP32 table (tab), FP32 weights and
 // compensated FP32 sums

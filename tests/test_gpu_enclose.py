@@ -106,13 +106,14 @@ def test_config2_subsample_enclosed():
 
 def test_sincos_meets_enclosure_bound():
     """Per-evaluation sin/cos error of the FP64 table path against long double, over random
-    words and the worst remainders (just inside each table point's half-interval)."""
+    words and the worst remainders: the two edges of each of the 1,024 table intervals, where
+    the remainder from the interval's midpoint is largest."""
     rng = np.random.default_rng(3)
     W = rng.integers(0, 2**64, size=1 << 20, dtype=np.uint64)
-    k = rng.integers(0, 512, size=4096).astype(np.uint64)
-    half = np.uint64(1 << 54)
-    W[:4096] = (k << np.uint64(55)) + half - np.uint64(1)       # largest positive remainder
-    W[4096:8192] = (k << np.uint64(55)) + half                   # largest negative remainder
+    k = rng.integers(0, 1024, size=4096).astype(np.uint64)
+    step = np.uint64(1 << 54)
+    W[:4096] = k << np.uint64(54)                                # largest negative remainder
+    W[4096:8192] = (k << np.uint64(54)) + step - np.uint64(1)    # largest positive remainder
     W[8192:8200] = np.array([0, 1, 1 << 62, 1 << 63, 3 << 62, (1 << 64) - 1, 1 << 55,
                              (1 << 55) - 1], dtype=np.uint64)
     s, c = ev.sincos_words(W, mode=0)
