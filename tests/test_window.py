@@ -74,3 +74,45 @@ def test_window_sincos_float32_within_rounding():
     # table rounding (<= 2^-25 each), two FP32 roundings of the rotation, |x|^3/6 <= 6e-10:
     # well inside 2.5e-7; a wrong sign, index or half step misses by ~1e-3 or more
     assert errs.max() < 2.5e-7, errs.max()
+
+
+def _fma32(a, b, c):
+    """float32 fused multiply-add: the product of two float32 is exact in float64, the sum
+    rounds once in float64 and once more to float32 (a double rounding that can move the last
+    bit, which the bound below does not rely on)."""
+    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+def _window_f(G, V, tb=11):
+    """The FP32 kernels' window (phase_window_f): column 3 summed in float32 from the limbs as
+    floats (v scaled by 2^-34), its integer part q entering column 4 as 4 q; columns 4..5 exact
+    (column 5 mod 2^32, of which the kernels use the low 14 bits)."""
+    g, v = _limbs(G), _limbs(V)
+    gf = [np.float32(x) for x in g]
+    vf = [np.float32(np.float32(x) * np.float32(2.0 ** -34)) for x in v]
+    t = np.float32(gf[3] * vf[0])
+    t = _fma32(gf[2], vf[1], t)
+    t = _fma32(gf[1], vf[2], t)
+    t = _fma32(gf[0], vf[3], t)
+    q = min(int(t), 0xFFFFFFFF)
+    s4 = (g[1] * v[3] + g[2] * v[2] + g[3] * v[1] + (1 << (45 - tb)) + 4 * q) & M64
+    s5 = ((s4 >> 32) + g[2] * v[3] + g[3] * v[2]) & 0xFFFFFFFF
+    return (s5 << 32) | (s4 & 0xFFFFFFFF)
+
+
+def test_window_f_column3_in_float32_within_bound():
+    """Column 3 in FP32: 6 2^-24 relative error on S3 < 2^66 (units of 2^96) is at most 6144
+    units of bit 128, plus 4 for q's truncation and 4 for columns 0..2; rounding to nearest keeps
+    the mean near the truncation's few units.  A dropped or mis-scaled column-3 term misses by
+    2^32 units or more, a wrong half step by 2^34."""
+    m46 = (1 << 46) - 1
+    d = []
+    for G, V in _cases(3000, 4):
+        exact = ((G * V) >> 128) & m46
+        win = (_window_f(G, V) - (1 << 34)) & m46
+        x = (win - exact) & m46
+        d.append(x - (1 << 46) if x >= (1 << 45) else x)
+    d = np.array(d)
+    assert np.abs(d).max() <= 6152, np.abs(d).max()
+    assert abs(d.mean()) <= 64, d.mean()
+    assert np.abs(d).max() > 16          # FP32 rounding does move the window
