@@ -303,9 +303,10 @@ static Plan make_plan(const k2_table* t, const WeightArrays* ws, int n_cand, int
     const int tb = t->tiling.target_blocks;
     if (mixed && p.head_end < nz) {
         // the FP64 head runs beside the FP32 tail on a second stream; give it workgroups in
-        // proportion to its work (an FP64 zero costs about 1.45 FP32 zeros), at least one split
+        // proportion to its work (an FP64 zero costs about 2.1 FP32 zeros; the time hardly
+        // depends on this share, since the two launches share the CUs), at least one split
         const int nt = nz - p.head_end;
-        const double wh = 1.45 * p.head_end, wt = (double)nt;
+        const double wh = 2.1 * p.head_end, wt = (double)nt;
         const int tb_head = std::max(p.n_ctiles, (int)std::lround(tb * wh / (wh + wt)));
         split_range(p.head_end, p.n_ctiles, tb_head, &p.sp_head, &p.zps_head);
         split_range(nt, p.n_ctiles, std::max(1, tb - tb_head), &p.sp_tail, &p.zps_tail);

@@ -238,10 +238,10 @@ std::string plan_batch(uint32_t n_req, const k2_request* req, uint64_t n_cand_to
             P.useful_evals += (double)tl.c_count * nz;
         }
     }
-    // longest first (FP32 items run faster than FP64 ones)
+    // longest first (an FP32 item takes about half the time of an FP64 one of the same size)
     auto cost = [&](const BatchItem& it) {
         const double w = (it.flags & 1) ? (double)CZ : (double)CW;
-        return w * (it.z1 - it.z0) * ((it.flags & 2) ? 0.7 : 1.0);
+        return w * (it.z1 - it.z0) * ((it.flags & 2) ? 0.5 : 1.0);
     };
     std::stable_sort(P.items.begin(), P.items.end(),
                      [&](const BatchItem& x, const BatchItem& y) { return cost(x) > cost(y); });
