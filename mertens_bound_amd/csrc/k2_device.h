@@ -126,6 +126,22 @@ __device__ __forceinline__ PhaseRem phase_rem_of(uint64_t W) {
     return PhaseRem{(uint32_t)(W >> 2), (uint32_t)(W >> 34), (uint32_t)(W >> 50) & 0x3FF0u};
 }
 
+// Column 5 (weight 2^160) is needed mod 2^14 only: g2 v3 + g3 v2 mod 2^16 is the dot product of
+// the limbs' low halves packed in pairs, one v_dot2_u32_u16 (each 16x16-bit product exact mod
+// 2^16) in place of two 24-bit multiply-adds.  pack_lo16(x, y) = (x mod 2^16) | (y << 16); a zero
+// packs (g2, g3), a candidate (v3, v2).  The FP32 window uses it (3% off the FP32 tail's time,
+// the packed zero word staged in the tile); in the FP64 loops, which would pack each zero in the
+// loop, it measured no faster than the two multiply-adds.
+__device__ __forceinline__ uint32_t pack_lo16(uint32_t x, uint32_t y) {
+    return __builtin_amdgcn_perm(y, x, 0x05040100u);
+}
+
+__device__ __forceinline__ uint32_t dot2_lo16(uint32_t gp, uint32_t vp, uint32_t c) {
+    typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, gp), __builtin_bit_cast(u16x2, vp), c,
+                                  false);
+}
+
 // The product columns 2..5 of G V (columns 0-1 skipped): s3lo, s4lo, s5 are bits [96,128),
 // [128,160) and [160,174) (s5 bits above 14 are not formed exactly).  In the carry case
 // (phase_carry_case: bits [98,110) all ones) W must come from phase_word_exact instead.
@@ -221,10 +237,12 @@ __device__ __forceinline__ void phase_window(Limbs4 g, Limbs4 v, uint32_t& hi, u
 // floats; four FP32 FMAs give S3 / 2^34 < 2^32 (worst-case error 2^-34 turn, random rounding),
 // and its integer part enters column 4 as 4 q (units of 2^130; truncation < 2^-44 turn).  This
 // takes four FP32 FMAs, a conversion and one multiply-add where the exact column 3 takes four
-// 64-bit multiply-adds and its carry bookkeeping.
+// 64-bit multiply-adds and its carry bookkeeping.  Column 5 is one packed dot product
+// (dot2_lo16 of gp, vp); g.x0 is not read (the FP32 C tile stages gp in its place).
 template <int TB, bool SB = false>
-__device__ __forceinline__ void phase_window_f(Limbs4 g, float4 gf, Limbs4 v, float4 vf,
-                                               uint32_t& hi, uint32_t& lo) {
+__device__ __forceinline__ void phase_window_f(Limbs4 g, uint32_t gp, float4 gf, Limbs4 v,
+                                               uint32_t vp, float4 vf, uint32_t& hi,
+                                               uint32_t& lo) {
     static_assert(TB >= 9 && TB <= 11, "table of 512..2048 points");
     float t = gf.w * vf.x;          // g3 v0
     t = fmaf(gf.z, vf.y, t);        // g2 v1
@@ -236,10 +254,7 @@ __device__ __forceinline__ void phase_window_f(Limbs4 g, float4 gf, Limbs4 v, fl
     s4 = mad64<SB>(g.x2, v.x2, s4);
     s4 = mad64<SB>(g.x3, v.x1, s4);
     asm("v_mad_u64_u32 %0, vcc, %1, 4, %2" : "=v"(s4) : "v"(q), "v"(s4) : "vcc");
-    uint32_t s5 = (uint32_t)(s4 >> 32);
-    s5 = mad24<SB>(g.x2, v.x3, s5);
-    s5 = mad24<SB>(g.x3, v.x2, s5);
-    hi = s5;
+    hi = dot2_lo16(gp, vp, (uint32_t)(s4 >> 32));   // column 5, bits [0,16)
     lo = (uint32_t)s4;
 }
 

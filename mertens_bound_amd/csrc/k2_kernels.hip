@@ -275,7 +275,8 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     else {
         uint32_t hi, lo;
         if (mode == 2) phase_window<K2_TAB32_BITS>(g, v, hi, lo);
-        else phase_window_f<K2_TAB32_BITS>(g, zgf[z0 + j], v, window_vf(v), hi, lo);
+        else phase_window_f<K2_TAB32_BITS>(g, pack_lo16(g.x2, g.x3), zgf[z0 + j], v,
+                                           pack_lo16(v.x3, v.x2), window_vf(v), hi, lo);
         W = ((((uint64_t)hi << 32) | lo) - (1ull << (45 - K2_TAB32_BITS))) & ((1ull << 46) - 1);
     }
     out[idx] = W;

@@ -235,8 +235,12 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
     const bool hi_half = NT == K2_WG || tid >= K2_WG;  // stages a|b, e
     const int ntiles = z1 > z0 ? (z1 - z0 + K2_WG - 1) / K2_WG : 0;
     float4 vf[NC];
+    uint32_t vp[NC];
 #pragma unroll
-    for (int i = 0; i < NC; ++i) vf[i] = window_vf(v[i]);
+    for (int i = 0; i < NC; ++i) {
+        vf[i] = window_vf(v[i]);
+        vp[i] = pack_lo16(v[i].x3, v[i].x2);
+    }
     // prefetch registers: r0, r1 = g, gf for the first half of the lanes and a|b, e for the
     // second (with 512 lanes each half stages one kind, so the two share registers)
     uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
@@ -246,6 +250,7 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
         if (j < z1) {
             if (lo_half) {
                 x0 = zv.g[j];
+                x0.x = pack_lo16(x0.z, x0.w);   // column 5's packed limbs in place of g0
                 const float4 f = zv.gf[j];
                 x1 = make_uint4(__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z),
                                 __float_as_uint(f.w));
@@ -308,7 +313,7 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
                     uint32_t hi, lo;
-                    phase_window_f<TB>(g, gf, v[i], vf[i], hi, lo);
+                    phase_window_f<TB>(g, gg.x, gf, v[i], vp[i], vf[i], hi, lo);
                     float S, C;
                     sincos_window<TB>(hi, lo, tab, S, C);
                     add_terms<FL>(tp[i], S, C, make_float2(ab.x, ab.y), make_float2(ab.z, ab.w),
@@ -373,6 +378,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
     for (int cp = 0; cp < cp_end; cp += ZG) {
         Acc<F32> acc[ZG];
         Limbs4 v[ZG];
+        uint32_t vp[F32 ? ZG : 1];
         float4 vf[F32 ? ZG : 1];
 #pragma unroll
         for (int c = 0; c < ZG; ++c) {
@@ -381,6 +387,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
             const uint4 cv = sm.v[cp + c];
             v[c] = Limbs4{uniform(cv.x), uniform(cv.y), uniform(cv.z), uniform(cv.w)};
             if constexpr (F32) {
+                vp[c] = uniform(pack_lo16(cv.w, cv.z));
                 const float4 f = window_vf(v[c]);
                 vf[c] = make_float4(uniform_f(f.x), uniform_f(f.y), uniform_f(f.z), uniform_f(f.w));
             }
@@ -397,6 +404,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                     if (j < z1) {
                         const uint4 gg = zv.g[j];
                         const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
+                        const uint32_t gp = pack_lo16(gg.z, gg.w);
                         const float4 gf = zv.gf[j];
                         const float2 a = zv.af[j];
                         const float2 b = D1 ? zv.bf[j] : make_float2(0.f, 0.f);
@@ -404,7 +412,7 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
 #pragma unroll
                         for (int c = 0; c < ZG; ++c) {
                             uint32_t hi, lo;
-                            phase_window_f<TB, true>(g, gf, v[c], vf[c], hi, lo);
+                            phase_window_f<TB, true>(g, gp, gf, v[c], vp[c], vf[c], hi, lo);
                             float S, C;
                             sincos_window<TB>(hi, lo, tab32, S, C);
                             add_terms<FL>(tp[c], S, C, a, b, e);
