@@ -190,8 +190,8 @@ int k2_screen_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double
 /* Debug: phase words floor(G_j V_c / 2^110) mod 2^64 for zeros [z0, z0+nz) (host buffers,
  * out[c*nz + j]).  mode 0 = evaluator path, 1 = schoolbook product, 2 = the exact-column phase
  * window: bits [128,174) of the product's columns 3..5, i.e. floor(W / 2^18) or up to 4 below
- * it (mod 2^46); 3 = the window the FP32 kernels use (column 3 summed in FP32), in the same
- * units, within 2^12.6 of floor(W / 2^18). */
+ * it (mod 2^46); 3 = the phase the FP32 kernels use (columns 4..5 exact, column 3 summed in
+ * FP32 and rounded to a unit here), in the same units, within 2^12.6 of floor(W / 2^18). */
 int k2_phase_words(k2_table *t, uint32_t n_cand, const uint64_t *V, uint32_t z0, uint32_t nz,
                    int mode, uint64_t *out);
 

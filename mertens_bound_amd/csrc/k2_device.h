@@ -231,29 +231,30 @@ __device__ __forceinline__ void phase_window(Limbs4 g, Limbs4 v, uint32_t& hi, u
     lo = (uint32_t)s4;
 }
 
-// The window of phase_window with column 3 in FP32.  Column 3 adds S3 2^96 with S3 < 2^66, less
-// than 2^-12 turn: it never wraps, so it needs no exact integer sum, only precision well below the
-// FP32 kernels' 2^-35-turn remainder.  gf holds g0..g3 as floats, vf holds v0..v3 times 2^-34 as
-// floats; four FP32 FMAs give S3 / 2^34 < 2^32 (worst-case error 2^-34 turn, random rounding),
-// and its integer part enters column 4 as 4 q (units of 2^130; truncation < 2^-44 turn).  This
-// takes four FP32 FMAs, a conversion and one multiply-add where the exact column 3 takes four
-// 64-bit multiply-adds and its carry bookkeeping.  Column 5 is one packed dot product
-// (dot2_lo16 of gp, vp); g.x0 is not read (the FP32 C tile stages gp in its place).
+// The FP32 kernels' window: columns 4..5 exact, column 3 in FP32 beside them.  Column 3 adds
+// S3 2^96 with S3 < 2^66, less than 2^-12 turn: it never wraps, so it needs no exact integer sum,
+// only precision well below the FP32 kernels' 2^-35-turn remainder.  gf holds g0..g3 as floats,
+// vf holds v0..v3 times 2^-34 as floats; four FP32 FMAs give t3 = S3 / 2^34 < 2^32 (units of
+// 2^130; worst-case error 2^-34 turn, rounding to nearest), which sincos_window_f adds to the
+// remainder in FP32.  hi:lo = columns 4..5 plus three quarters of a table step, 3 2^(172-TB): the
+// table index rounds there because t3 adds up to half a step, so that the remainder with t3
+// stays within three quarters of a step either side.  Column 5 is one packed dot product
+// (dot2_lo16 of gp, vp); g.x0 is not read (the FP32 C tile stages gp in its place).  Against the
+// exact column 3 this takes four FP32 FMAs (half-cost instructions) and one more in the
+// remainder, where that takes four 64-bit multiply-adds and five carry and register moves.
 template <int TB, bool SB = false>
 __device__ __forceinline__ void phase_window_f(Limbs4 g, uint32_t gp, float4 gf, Limbs4 v,
                                                uint32_t vp, float4 vf, uint32_t& hi,
-                                               uint32_t& lo) {
+                                               uint32_t& lo, float& t3) {
     static_assert(TB >= 9 && TB <= 11, "table of 512..2048 points");
     float t = gf.w * vf.x;          // g3 v0
     t = fmaf(gf.z, vf.y, t);        // g2 v1
     t = fmaf(gf.y, vf.z, t);        // g1 v2
     t = fmaf(gf.x, vf.w, t);        // g0 v3
-    const uint32_t q = (uint32_t)t;
-    // column 4 (2^128) with half a table step, 2^(173-TB) = 2^(45-TB) 2^128, then 4 q
-    uint64_t s4 = mad64<SB>(g.x1, v.x3, 1ull << (45 - TB));
+    t3 = t;
+    uint64_t s4 = mad64<SB>(g.x1, v.x3, 3ull << (44 - TB));
     s4 = mad64<SB>(g.x2, v.x2, s4);
     s4 = mad64<SB>(g.x3, v.x1, s4);
-    asm("v_mad_u64_u32 %0, vcc, %1, 4, %2" : "=v"(s4) : "v"(q), "v"(s4) : "vcc");
     hi = dot2_lo16(gp, vp, (uint32_t)(s4 >> 32));   // column 5, bits [0,16)
     lo = (uint32_t)s4;
 }
@@ -377,6 +378,27 @@ __device__ __forceinline__ void sincos_window(uint32_t hi, uint32_t lo, const Ta
     const float u = (float)R;
     const float x = fmaf(u, step, -half);
     const float h = fmaf(u, 0.5f * step, -0.5f * half);   // x / 2
+    C = fmaf(-x, fmaf(h, t.c, t.s), t.c);
+    S = fmaf(x, fmaf(-h, t.s, t.c), t.s);
+}
+
+// sin/cos of the FP32 kernels' window (phase_window_f): as sincos_window, with the index rounded
+// at three quarters of a step and column 3 (t3, units of 2^130) added to the remainder:
+// x = 2 pi ((R + t3 2^(TB-12)) 2^-(32+TB) - 3/4 2^-TB), |x| <= 3/4 2 pi 2^-TB (2.3e-3 at TB = 11:
+// |x|^3/6 <= 2.0e-9 and x^4/24 <= 1.2e-12, against FP32 rounding of 6e-8; the x^3 error is odd
+// in x, and x is centred, so it averages out over the zeros).  x/2 is one multiply.
+template <int TB>
+__device__ __forceinline__ void sincos_window_f(uint32_t hi, uint32_t lo, float t3,
+                                                const Tab32* tab, float& S, float& C) {
+    constexpr uint32_t mask = ((1u << TB) - 1u) << 3;
+    const uint32_t off = (TB == 11 ? hi : hi >> (11 - TB)) & mask;   // 8 * index
+    const Tab32 t = *reinterpret_cast<const Tab32*>(reinterpret_cast<const char*>(tab) + off);
+    const uint32_t R = __builtin_amdgcn_alignbit(hi, lo, 14 - TB);
+    constexpr float step = 6.28318530717958647692f / (float)(1ull << (32 + TB));
+    constexpr float tstep = step * (float)(1u << TB) / 4096.0f;   // step 2^(TB-12), exact
+    constexpr float tq = step * 3221225472.0f;                     // 3/4 of a table step
+    const float x = fmaf((float)R, step, fmaf(t3, tstep, -tq));
+    const float h = 0.5f * x;
     C = fmaf(-x, fmaf(h, t.c, t.s), t.c);
     S = fmaf(x, fmaf(-h, t.s, t.c), t.s);
 }

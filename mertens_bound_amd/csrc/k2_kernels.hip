@@ -260,7 +260,8 @@ __global__ __launch_bounds__(256) void k2_reduce_kernel(
 // ---- debug: phase words --------------------------------------------------------------------------
 // mode 0: fast W (skip + fix-up), 1: schoolbook W, 2: the exact-column phase window, bits
 // [128,174) of the partial product (the phase floored to 2^-46 turn, low by at most 4 units),
-// 3: the FP32 kernels' window (column 3 in FP32, phase_window_f), in the same units
+// 3: the phase the FP32 kernels use, in the same units: the window of phase_window_f without
+// its rounding offset, plus its FP32 column 3 rounded to a unit
 __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     const uint4* __restrict__ zg, const float4* __restrict__ zgf, int z0, int nz,
     const uint4* __restrict__ cand, int n_cand, int mode, uint64_t* __restrict__ out) {
@@ -272,12 +273,18 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     uint64_t W;
     if (mode == 0) W = phase_word(g, v);
     else if (mode == 1) W = phase_word_exact(g, v);
-    else {
+    else if (mode == 2) {
         uint32_t hi, lo;
-        if (mode == 2) phase_window<K2_TAB32_BITS>(g, v, hi, lo);
-        else phase_window_f<K2_TAB32_BITS>(g, pack_lo16(g.x2, g.x3), zgf[z0 + j], v,
-                                           pack_lo16(v.x3, v.x2), window_vf(v), hi, lo);
+        phase_window<K2_TAB32_BITS>(g, v, hi, lo);
         W = ((((uint64_t)hi << 32) | lo) - (1ull << (45 - K2_TAB32_BITS))) & ((1ull << 46) - 1);
+    } else {
+        uint32_t hi, lo;
+        float t3;
+        phase_window_f<K2_TAB32_BITS>(g, pack_lo16(g.x2, g.x3), zgf[z0 + j], v,
+                                      pack_lo16(v.x3, v.x2), window_vf(v), hi, lo, t3);
+        const uint64_t c3 = (uint64_t)llrint(4.0 * (double)t3);   // units of 2^128
+        W = ((((uint64_t)hi << 32) | lo) - (3ull << (44 - K2_TAB32_BITS)) + c3) &
+            ((1ull << 46) - 1);
     }
     out[idx] = W;
 }

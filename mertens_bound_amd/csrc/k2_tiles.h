@@ -313,9 +313,10 @@ __device__ __forceinline__ void ctile_f32(const ZeroView& zv, int z0, int z1, co
 #pragma unroll
                 for (int i = 0; i < NC; ++i) {
                     uint32_t hi, lo;
-                    phase_window_f<TB>(g, gg.x, gf, v[i], vp[i], vf[i], hi, lo);
+                    float t3;
+                    phase_window_f<TB>(g, gg.x, gf, v[i], vp[i], vf[i], hi, lo, t3);
                     float S, C;
-                    sincos_window<TB>(hi, lo, tab, S, C);
+                    sincos_window_f<TB>(hi, lo, t3, tab, S, C);
                     add_terms<FL>(tp[i], S, C, make_float2(ab.x, ab.y), make_float2(ab.z, ab.w),
                                   ee);
                 }
@@ -412,9 +413,10 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
 #pragma unroll
                         for (int c = 0; c < ZG; ++c) {
                             uint32_t hi, lo;
-                            phase_window_f<TB, true>(g, gp, gf, v[c], vp[c], vf[c], hi, lo);
+                            float t3;
+                            phase_window_f<TB, true>(g, gp, gf, v[c], vp[c], vf[c], hi, lo, t3);
                             float S, C;
-                            sincos_window<TB>(hi, lo, tab32, S, C);
+                            sincos_window_f<TB>(hi, lo, t3, tab32, S, C);
                             add_terms<FL>(tp[c], S, C, a, b, e);
                         }
                     }

@@ -84,9 +84,9 @@ def _fma32(a, b, c):
 
 
 def _window_f(G, V, tb=11):
-    """The FP32 kernels' window (phase_window_f): column 3 summed in float32 from the limbs as
-    floats (v scaled by 2^-34), its integer part q entering column 4 as 4 q; columns 4..5 exact
-    (column 5 mod 2^32, of which the kernels use the low 14 bits)."""
+    """The FP32 kernels' window (phase_window_f): columns 4..5 exact plus three quarters of a
+    table step (column 5 mod 2^32, of which the kernels use the low 14 bits), and beside it
+    column 3 summed in float32 from the limbs as floats (v scaled by 2^-34): t3 = S3 / 2^34."""
     g, v = _limbs(G), _limbs(V)
     gf = [np.float32(x) for x in g]
     vf = [np.float32(np.float32(x) * np.float32(2.0 ** -34)) for x in v]
@@ -94,25 +94,58 @@ def _window_f(G, V, tb=11):
     t = _fma32(gf[2], vf[1], t)
     t = _fma32(gf[1], vf[2], t)
     t = _fma32(gf[0], vf[3], t)
-    q = min(int(t), 0xFFFFFFFF)
-    s4 = (g[1] * v[3] + g[2] * v[2] + g[3] * v[1] + (1 << (45 - tb)) + 4 * q) & M64
+    s4 = (g[1] * v[3] + g[2] * v[2] + g[3] * v[1] + (3 << (44 - tb))) & M64
     s5 = ((s4 >> 32) + g[2] * v[3] + g[3] * v[2]) & 0xFFFFFFFF
-    return (s5 << 32) | (s4 & 0xFFFFFFFF)
+    return (s5 << 32) | (s4 & 0xFFFFFFFF), t
 
 
 def test_window_f_column3_in_float32_within_bound():
-    """Column 3 in FP32: 6 2^-24 relative error on S3 < 2^66 (units of 2^96) is at most 6144
-    units of bit 128, plus 4 for q's truncation and 4 for columns 0..2; rounding to nearest keeps
-    the mean near the truncation's few units.  A dropped or mis-scaled column-3 term misses by
-    2^32 units or more, a wrong half step by 2^34."""
-    m46 = (1 << 46) - 1
+    """The phase the FP32 kernels use, window - offset + 4 t3 (units of bit 128): column 3 in
+    FP32 has 6 2^-24 relative error on S3 < 2^66 (units of 2^96), at most 6144 units of bit 128,
+    plus 4 for columns 0..2; rounding to nearest keeps the mean within a few units.  A dropped or
+    mis-scaled column-3 term misses by 2^32 units or more, a wrong offset by 2^33."""
+    from fractions import Fraction
+    m46 = 1 << 46
     d = []
     for G, V in _cases(3000, 4):
-        exact = ((G * V) >> 128) & m46
-        win = (_window_f(G, V) - (1 << 34)) & m46
-        x = (win - exact) & m46
-        d.append(x - (1 << 46) if x >= (1 << 45) else x)
+        exact = ((G * V) >> 128) % m46
+        w, t3 = _window_f(G, V)
+        eff = Fraction(w - (3 << 33)) + 4 * Fraction(float(t3))
+        x = (eff - exact) % m46
+        d.append(float(x - m46 if x >= m46 // 2 else x))
     d = np.array(d)
     assert np.abs(d).max() <= 6152, np.abs(d).max()
     assert abs(d.mean()) <= 64, d.mean()
-    assert np.abs(d).max() > 16          # FP32 rounding does move the window
+    assert np.abs(d).max() > 16          # FP32 rounding does move the phase
+
+
+def test_window_f_sincos_float32_within_rounding():
+    """sincos_window_f: index = top 11 bits of the window (rounded at three quarters of a step),
+    x = (R + t3/2) 2 pi 2^-43 - 3/4 2 pi 2^-11 in FP32 (two FMAs), h = x/2, and the rotation;
+    compared with sin/cos of the exact phase.  |x| stays within 3/4 of a step, so sin x = x
+    errs by at most 2e-9: the total stays inside 2.5e-7, where a wrong offset, scale or index
+    misses by 1e-3 or more."""
+    f32 = np.float32
+    two_pi = 2 * np.pi
+    k = np.arange(2048)
+    tab_c = np.cos(two_pi * k / 2048).astype(f32)
+    tab_s = np.sin(two_pi * k / 2048).astype(f32)
+    step = f32(two_pi) / f32(2.0 ** 43)
+    tstep = f32(step * f32(0.5))
+    tq = f32(step * f32(3 * 2 ** 30))
+    errs = []
+    for G, V in _cases(3000, 5):
+        w, t3 = _window_f(G, V)
+        hi, lo = w >> 32, w & 0xFFFFFFFF
+        idx = (hi >> 3) & 0x7FF
+        R = ((hi << 32 | lo) >> 3) & 0xFFFFFFFF
+        x = _fma32(f32(R), step, _fma32(t3, tstep, -tq))
+        h = f32(f32(0.5) * x)
+        c, s = tab_c[idx], tab_s[idx]
+        C = _fma32(-x, _fma32(h, c, s), c)
+        S = _fma32(x, _fma32(-h, s, c), s)
+        phase = ((G * V) >> 110) & M64
+        th = two_pi * (phase / 2.0 ** 64)
+        errs.append(max(abs(float(C) - np.cos(th)), abs(float(S) - np.sin(th))))
+    errs = np.array(errs)
+    assert errs.max() < 2.5e-7, errs.max()
