@@ -36,7 +36,11 @@ sys.path.insert(0, ROOT)
 # so an FP32 FMA is half a slot.  Mixed: the sin and cos of the reduced angle count as the three
 # FMAs of a table rotation each, plus four FMAs for the weights: 10 FP32 FMAs = 5 slots.
 WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
-        "mixed": {"int32_mad": 15, "fp32_fma": 10, "fp32_fma_slot": 0.5, "slots": 20},
+        # the precision form's FP32 tail as built: 3 multiply-adds (column 4), the packed dot
+        # product (column 5), the funnel shift and the conversion of the remainder; column 3 in
+        # 4 FP32 FMAs, x and x/2 in 3, the rotation in 4, the sums in 4 (half a slot each at the
+        # published FP32 peak); its FP64 head is charged at the plain form's 36 slots
+        "mixed": {"int32_ops": 6, "fp32_ops": 15, "fp32_slot": 0.5, "slots": 13.5},
         "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48}}
 # Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
 # published 78.6 TFLOP/s FP64 peak counted per FMA (2 flops).  The roofline reports the work in
@@ -139,15 +143,19 @@ def traffic_summary(workload):
         return None
 
 
-def roofline(form, evals, kern_ms, alg_bytes, workload):
-    w = WORK[form]
+def roofline(form, evals, kern_ms, alg_bytes, workload, head_frac=0.0):
+    """head_frac: for the mixed form, the share of the evaluations in its FP64 head"""
+    w = dict(WORK[form])
+    if form == "mixed":
+        w["fp64_head_fraction"] = head_frac
+        w["slots"] = head_frac * WORK["plain"]["slots"] + (1.0 - head_frac) * w["slots"]
     rate = evals / (kern_ms * 1e-3)
     achieved = rate * 2 * w["slots"] / 1e12
     r = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
          "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_summary(workload),
          "achieved_valu_slots_tops": rate * w["slots"] / 1e12, "valu_slot_peak_tops": VALU_PEAK_TOPS,
          "kernel_ms": kern_ms, "kernel_evals_per_s": rate, "work_per_eval": w,
-         "int32_mad_tops": rate * w["int32_mad"] / 1e12,
+         "int32_tops": rate * w.get("int32_mad", w.get("int32_ops", 0)) / 1e12,
          "algorithmic_bytes": alg_bytes,
          "hbm_GBps_algorithmic": alg_bytes / (kern_ms * 1e-3) / 1e9}
     if form in ("plain", "enclose"):
@@ -231,8 +239,9 @@ def run_eval(args, form):
          "fp64_head_zeros": t.mixed_head(c["k"]) if form == "mixed" else tb.n})
     res["accuracy"] = {"max_abs_dK2_vs_oracle": acc, "oracle_sample": gname, "bar": 1e-9,
                        "ok": (acc is not None and acc <= 1e-9)}
+    head = res["config"]["fp64_head_zeros"] / tb.n if form == "mixed" else 0.0
     res["roofline"] = roofline(form, evals, k_ms / max(1, k_n), alg_bytes,
-                               f"{args.workload}_{form}")
+                               f"{args.workload}_{form}", head)
     res["wall_s"] = wall
     if form == "mixed" and not args.no_check:
         # the FP64 kernel on the same inputs and GPU: speed-up and |dK2| for every candidate
@@ -325,8 +334,11 @@ def run_batched(args):
                                    "table", "form": form, "batch": info})
     res["per_request_loop"] = {"ms": ms_loop, "evals_per_s": useful / (ms_loop * 1e-3),
                                "max_abs_diff_vs_batch": d}
+    head = 0.0
+    if form == "mixed":   # share of the useful evaluations in the requests' FP64 heads
+        head = sum(float(n) * t.mixed_head(k, a) for a, n, _, k in reqs) / useful
     res["roofline"] = roofline(form, useful, ms, tb.n * 48 * 3 + V.shape[0] * 32,
-                               f"batched_{form}")
+                               f"batched_{form}", head)
     res["wall_s"] = wall
     return res
 
