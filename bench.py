@@ -32,7 +32,7 @@ sys.path.insert(0, ROOT)
 #   enclose: the plain work + 2 x 6 FP64 additions (TwoSum error-free accumulation of K2, K2')
 # Work per candidate-zero evaluation in full-rate VALU slots (one wave64 instruction per 4 cycles
 # per SIMD).  FP32 instructions issue in half that (2 cycles per wave64 on gfx950; the FP32 tail
-# issues 130% of the full-rate instruction rate, profiles/k2_eval_mixed_config2_trace_counters.txt),
+# issues 130% of the full-rate instruction rate, profiles/k2_eval_mixed_65536x2325572_trace_counters.txt),
 # so an FP32 FMA is half a slot.  Mixed: the sin and cos of the reduced angle count as the three
 # FMAs of a table rotation each, plus four FMAs for the weights: 10 FP32 FMAs = 5 slots.
 WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
@@ -193,7 +193,7 @@ def run_eval(args, form):
     from mertens_bound_amd import autotune
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import synth
-    c = synth.CONFIG1 if args.workload == "small" else synth.CONFIG2
+    c = synth.W1024x10000 if args.workload == "small" else synth.W65536x2325572
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     n_cand = args.n_cand or c["n_cand"]
     V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
@@ -209,7 +209,7 @@ def run_eval(args, form):
     st = torch.cuda.current_stream().cuda_stream
 
     acc = None
-    gname = "config1_oracle.npz" if args.workload == "small" else "config2_sub256.npz"
+    gname = "w1024x10000_oracle.npz" if args.workload == "small" else "w65536x2325572_sub256.npz"
     gpath = os.path.join(ROOT, "tests", "golden", gname)
     if os.path.exists(gpath) and not args.no_check:
         g = np.load(gpath, allow_pickle=False)
@@ -303,7 +303,7 @@ def run_batched(args):
     from mertens_bound_amd import autotune
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import synth
-    c = synth.CONFIG2
+    c = synth.W65536x2325572
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     torch.cuda.set_device(0)
     t = ev.K2Table(tb)
@@ -349,7 +349,7 @@ def run_polish(args):
     from mertens_bound_amd import autograd as ag
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import synth, workloads
-    c = synth.CONFIG2
+    c = synth.W65536x2325572
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     torch.cuda.set_device(0)
     t = ev.K2Table(tb)
@@ -409,7 +409,7 @@ def run_enclose(args):
     from mertens_bound_amd import autotune
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import synth
-    c = synth.CONFIG2
+    c = synth.W65536x2325572
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     n_cand = args.n_cand or c["n_cand"]
     V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
@@ -418,14 +418,14 @@ def run_enclose(args):
     autotune.apply_tuned(t, n_cand, tb.n, "plain")
     Vd = ev.v_tensor(V)
     acc = None
-    gpath = os.path.join(ROOT, "tests", "golden", "config2_sub256.npz")
+    gpath = os.path.join(ROOT, "tests", "golden", "w65536x2325572_sub256.npz")
     if os.path.exists(gpath) and not args.no_check:
         g = np.load(gpath, allow_pickle=False)
         K, Kr, dK, dKr, _ = t.enclose(g["V"], c["k"])
         inside = np.abs(K - g["K"]) <= Kr + 2.0 ** -53 * np.abs(g["K"])
         acc = {"max_abs_dK2_vs_oracle": float(np.max(np.abs(K - g["K"]))),
                "max_radius": float(Kr.max()), "max_dK_radius": float(dKr.max()),
-               "oracle_inside_all": bool(inside.all()), "oracle_sample": "config2_sub256.npz"}
+               "oracle_inside_all": bool(inside.all()), "oracle_sample": "w65536x2325572_sub256.npz"}
     state = {}
 
     def step():
@@ -455,7 +455,7 @@ def run_weights(args):
     import torch
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import pipeline, synth
-    c = synth.CONFIG2
+    c = synth.W65536x2325572
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     n_cand = args.n_cand or c["n_cand"]
     V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])

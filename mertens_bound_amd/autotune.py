@@ -160,7 +160,7 @@ def main():
         bg = {"nc": [2], "cz": [8], "chunk": [8192, 16384], "wg_per_cu": [0],
               "c_min_fill": [128, 256]}
     torch.cuda.set_device(0)
-    c2 = synth.CONFIG2
+    c2 = synth.W65536x2325572
     big = synth.make_zero_table(c2["n_zeros"], c2["seed"])
     t_big = ev.K2Table(big)
     small = synth.make_zero_table(10_000, 0)

@@ -167,8 +167,8 @@ def candidates_from_x_dy(x, dy0) -> np.ndarray:
 
 # the five workloads of this evaluator -----------------------------------------------------
 
-CONFIG1 = dict(n_zeros=10_000, n_cand=1_024, v_lo=1e18, v_hi=4e18, k=1.5e-10, seed=0)
-CONFIG2 = dict(n_zeros=2_325_572, n_cand=65_536, v_lo=3.4e18, v_hi=3.6e18, k=1.5e-10, seed=0)
+W1024x10000 = dict(n_zeros=10_000, n_cand=1_024, v_lo=1e18, v_hi=4e18, k=1.5e-10, seed=0)
+W65536x2325572 = dict(n_zeros=2_325_572, n_cand=65_536, v_lo=3.4e18, v_hi=3.6e18, k=1.5e-10, seed=0)
 BATCH_KS = (3e-9, 1e-9, 1.5e-10)
 
 

@@ -3,9 +3,9 @@
 Inputs are regenerated from the seeded generator (mertens_bound_amd.synth); the fixtures hold
 the input sha256 values and the oracle outputs only.
 
-  config1_oracle.npz   all 1,024 candidates of the 10,000-zero workload (k = 1.5e-10, seed 0):
+  w1024x10000_oracle.npz   all 1,024 candidates of the 10,000-zero workload (k = 1.5e-10, seed 0):
                        binary128 K2, K2', K2'', sum |terms|
-  config2_sub256.npz   the first 256 candidates of the 65,536-candidate workload over all
+  w65536x2325572_sub256.npz   the first 256 candidates of the 65,536-candidate workload over all
                        2,325,572 zeros: binary128 K2, K2', K2''  (about 5 min on 8 cores)
   polish_small.npz     FP64 Newton polish (8 steps) of 16 candidates on a 3,000-zero table
   ref_compat.npz       the reference evaluator (oracle/_ref, compiled from
@@ -13,7 +13,7 @@ the input sha256 values and the oracle outputs only.
                        row sums and leading entries of compute_peak_AB's A/B, and the
                        polish_peaks_with_weights outputs
 
-Usage: python tests/golden/make_golden.py [config1|config2|polish|ref|all]
+Usage: python tests/golden/make_golden.py [w1024x10000|w65536x2325572|polish|ref|all]
 """
 import os
 import sys
@@ -30,25 +30,25 @@ from mertens_bound_amd import formats, synth  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 
 
-def config1():
-    c = synth.CONFIG1
+def w1024x10000():
+    c = synth.W1024x10000
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     V = synth.make_candidates(c["n_cand"], c["v_lo"], c["v_hi"], c["seed"])
     t = time.time()
     K, dK, d2K, aS = oracle.eval_quad(tb, V, c["k"])
-    print(f"config1 oracle {time.time() - t:.1f}s")
-    np.savez_compressed(os.path.join(OUT, "config1_oracle.npz"), K=K, dK=dK, d2K=d2K, absS=aS,
+    print(f"w1024x10000 oracle {time.time() - t:.1f}s")
+    np.savez_compressed(os.path.join(OUT, "w1024x10000_oracle.npz"), K=K, dK=dK, d2K=d2K, absS=aS,
                         table_sha256=tb.sha256(), V=V)
 
 
-def config2():
-    c = synth.CONFIG2
+def w65536x2325572():
+    c = synth.W65536x2325572
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     V = synth.make_candidates(c["n_cand"], c["v_lo"], c["v_hi"], c["seed"])[:256]
     t = time.time()
     K, dK, d2K, aS = oracle.eval_quad(tb, V, c["k"])
-    print(f"config2 subsample oracle {time.time() - t:.1f}s")
-    np.savez_compressed(os.path.join(OUT, "config2_sub256.npz"), K=K, dK=dK, d2K=d2K, absS=aS,
+    print(f"w65536x2325572 subsample oracle {time.time() - t:.1f}s")
+    np.savez_compressed(os.path.join(OUT, "w65536x2325572_sub256.npz"), K=K, dK=dK, d2K=d2K, absS=aS,
                         table_sha256=tb.sha256(), V=V)
 
 
@@ -94,7 +94,7 @@ def ref():
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     oracle.build()
-    for name, fn in (("config1", config1), ("polish", polish), ("ref", ref),
-                     ("config2", config2)):
+    for name, fn in (("w1024x10000", w1024x10000), ("polish", polish), ("ref", ref),
+                     ("w65536x2325572", w65536x2325572)):
         if what in (name, "all"):
             fn()

@@ -17,20 +17,20 @@ U = 2.0 ** -53
 
 
 @pytest.fixture(scope="module")
-def cfg1():
-    c = synth.CONFIG1
+def w10k():
+    c = synth.W1024x10000
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     V = synth.make_candidates(c["n_cand"], c["v_lo"], c["v_hi"], c["seed"])
     return tb, ev.K2Table(tb), V, c["k"]
 
 
-def test_config1_encloses_binary128(cfg1, oracle_lib):
-    tb, t, V, k = cfg1
+def test_w1024x10000_encloses_binary128(w10k, oracle_lib):
+    tb, t, V, k = w10k
     K, Kr, dK, dKr, tail = t.enclose(V, k)
     hi, lo = oracle_lib.eval_quad_pair(tb, V, k)          # K2 in binary128, as hi + lo
     _, dKq, _, _ = oracle_lib.eval_quad(tb, V, k, flags=oracle_lib.VALUE | oracle_lib.D1)
     dev = np.abs((K - hi) - lo)
-    print(f"config1 enclosure: max|K - K_q| {dev.max():.3e}, max rad {Kr.max():.3e}, "
+    print(f"w1024x10000 enclosure: max|K - K_q| {dev.max():.3e}, max rad {Kr.max():.3e}, "
           f"max|dK - dK_q| {np.abs(dK - dKq).max():.3e}, max dK rad {dKr.max():.3e}")
     assert np.all(dev <= Kr)
     assert np.all(np.abs(dK - dKq) <= dKr + U * np.abs(dKq))
@@ -41,17 +41,17 @@ def test_config1_encloses_binary128(cfg1, oracle_lib):
     assert np.all(np.abs(Kp - K) <= Kr + 1e-14)
 
 
-def test_config1_d1_off_and_repeatable(cfg1):
-    tb, t, V, k = cfg1
+def test_w1024x10000_d1_off_and_repeatable(w10k):
+    tb, t, V, k = w10k
     a = t.enclose(V, k, d1=False)
     b = t.enclose(V, k, d1=True)
     assert a[2] is None and a[3] is None
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])   # bit-identical reruns
 
 
-def test_tail_bound_covers_rest_of_table(cfg1):
-    tb, t, V, k = cfg1
-    g = golden("config1_oracle.npz")
+def test_tail_bound_covers_rest_of_table(w10k):
+    tb, t, V, k = w10k
+    g = golden("w1024x10000_oracle.npz")
     assert np.array_equal(g["V"], V)
     N = 5_000
     K, Kr, dK, dKr, tail = t.enclose(V, k, n_zeros=N)
@@ -87,17 +87,17 @@ def test_closed_form_quarter_turn():
 
 @pytest.mark.skipif(not __import__("os").path.exists(
     __import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
-                               "config2_sub256.npz")), reason="config2 fixture not generated")
-def test_config2_subsample_enclosed():
+                               "w65536x2325572_sub256.npz")), reason="65,536 x 2,325,572 fixture not generated")
+def test_65536x2325572_subsample_enclosed():
     """256 candidates of the full-size workload against all 2,325,572 zeros."""
-    c = synth.CONFIG2
-    g = golden("config2_sub256.npz")
+    c = synth.W65536x2325572
+    g = golden("w65536x2325572_sub256.npz")
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     assert tb.sha256() == str(g["table_sha256"])
     t = ev.K2Table(tb)
     K, Kr, dK, dKr, tail = t.enclose(g["V"], c["k"])
     dev = np.abs(K - g["K"])
-    print(f"config2 sub256 enclosure: max|K - K_q| {dev.max():.3e}, max rad {Kr.max():.3e}, "
+    print(f"w65536x2325572 sub256 enclosure: max|K - K_q| {dev.max():.3e}, max rad {Kr.max():.3e}, "
           f"dK rad {dKr.max():.3e}")
     assert np.all(dev <= Kr + U * np.abs(g["K"]))
     assert np.all(np.abs(dK - g["dK"]) <= dKr + U * np.abs(g["dK"]))
@@ -125,9 +125,9 @@ def test_sincos_meets_enclosure_bound():
     assert float(es.max()) <= 3.0 and float(ec.max()) <= 3.0
 
 
-def test_enclose_argument_errors(cfg1):
+def test_enclose_argument_errors(w10k):
     import torch
-    tb, t, V, k = cfg1
+    tb, t, V, k = w10k
     Vt = ev.v_tensor(V)
     out = [torch.empty(V.shape[0], dtype=torch.float64, device="cuda") for _ in range(4)]
     L = ev.load_library()

@@ -19,8 +19,8 @@ ALL = ev.VALUE | ev.D1 | ev.D2
 
 
 @pytest.fixture(scope="module")
-def cfg1():
-    c = synth.CONFIG1
+def w10k():
+    c = synth.W1024x10000
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     V = synth.make_candidates(c["n_cand"], c["v_lo"], c["v_hi"], c["seed"])
     return tb, ev.K2Table(tb), V, c["k"]
@@ -32,31 +32,31 @@ def _scales(tb, k, n=None):
     return np.sum(a), np.sum(tb.gamma[:n] * a), np.sum(tb.gamma[:n] ** 2 * a)
 
 
-def test_config1_plain_vs_golden(cfg1):
-    tb, t, V, k = cfg1
-    g = golden("config1_oracle.npz")
+def test_w1024x10000_plain_vs_golden(w10k):
+    tb, t, V, k = w10k
+    g = golden("w1024x10000_oracle.npz")
     K, dK, d2K = t.eval(V, k, flags=ALL)
     s0, s1, s2 = _scales(tb, k)
     errK = np.max(np.abs(K - g["K"]))
-    print(f"config1 plain max|dK2|={errK:.3e}")
+    print(f"w1024x10000 plain max|dK2|={errK:.3e}")
     assert errK <= 1e-12
     assert np.max(np.abs(dK - g["dK"])) / s1 <= 1e-12
     assert np.max(np.abs(d2K - g["d2K"])) / s2 <= 1e-12
 
 
-def test_config1_mixed_vs_golden(cfg1):
-    tb, t, V, k = cfg1
-    g = golden("config1_oracle.npz")
+def test_w1024x10000_mixed_vs_golden(w10k):
+    tb, t, V, k = w10k
+    g = golden("w1024x10000_oracle.npz")
     K, dK, d2K = t.eval(V, k, flags=ALL | ev.MIXED)
     errK = np.max(np.abs(K - g["K"]))
     s0, s1, s2 = _scales(tb, k)
-    print(f"config1 mixed max|dK2|={errK:.3e} max|dK2'|/scale={np.max(np.abs(dK - g['dK'])) / s1:.3e}")
+    print(f"w1024x10000 mixed max|dK2|={errK:.3e} max|dK2'|/scale={np.max(np.abs(dK - g['dK'])) / s1:.3e}")
     assert errK <= 1e-9
     assert np.max(np.abs(dK - g["dK"])) / s1 <= 1e-6
 
 
-def test_deterministic(cfg1):
-    tb, t, V, k = cfg1
+def test_deterministic(w10k):
+    tb, t, V, k = w10k
     a = t.eval(V, k, flags=ALL)
     b = t.eval(V, k, flags=ALL)
     for x, y in zip(a, b):
@@ -65,9 +65,9 @@ def test_deterministic(cfg1):
 
 @pytest.mark.parametrize("nc,wg", [(1, 256), (2, 256), (4, 256), (8, 256), (1, 512), (2, 512)])
 @pytest.mark.parametrize("target", [37, 4096])
-def test_tilings_agree(cfg1, nc, wg, target):
-    tb, t, V, k = cfg1
-    g = golden("config1_oracle.npz")
+def test_tilings_agree(w10k, nc, wg, target):
+    tb, t, V, k = w10k
+    g = golden("w1024x10000_oracle.npz")
     t.set_tiling(nc=nc, target_blocks=target, wg_threads=wg)
     try:
         K, dK, _ = t.eval(V, k, flags=ev.VALUE | ev.D1)
@@ -80,8 +80,8 @@ def test_tilings_agree(cfg1, nc, wg, target):
 
 @pytest.mark.parametrize("n_cand,n_zeros", [(1, 1), (1, 10_000), (255, 257), (257, 3333),
                                             (1000, 4096), (300, 9999)])
-def test_ragged_sizes_vs_port(cfg1, oracle_lib, n_cand, n_zeros):
-    tb, t, V, k = cfg1
+def test_ragged_sizes_vs_port(w10k, oracle_lib, n_cand, n_zeros):
+    tb, t, V, k = w10k
     Vs = V[:n_cand]
     K, dK, d2K = t.eval(Vs, k, n_zeros=n_zeros, flags=ALL)
     Ko, dKo, d2Ko = oracle_lib.eval_f64(tb, Vs, k, n_zeros=n_zeros)
@@ -91,9 +91,9 @@ def test_ragged_sizes_vs_port(cfg1, oracle_lib, n_cand, n_zeros):
     assert np.max(np.abs(d2K - d2Ko)) / s2 <= 1e-12
 
 
-def test_value_only_and_device_path(cfg1):
+def test_value_only_and_device_path(w10k):
     import torch
-    tb, t, V, k = cfg1
+    tb, t, V, k = w10k
     K, _, _ = t.eval(V, k, flags=ev.VALUE)
     Kd, dKd, _ = t.eval_device(ev.v_tensor(V), k, flags=ev.VALUE | ev.D1)
     torch.cuda.synchronize()
@@ -102,8 +102,8 @@ def test_value_only_and_device_path(cfg1):
     assert np.array_equal(K, Kh)
 
 
-def test_argument_errors(cfg1):
-    tb, t, V, k = cfg1
+def test_argument_errors(w10k):
+    tb, t, V, k = w10k
     with pytest.raises(ev.K2Error):
         t.eval(V[:4], k, n_zeros=tb.n + 1)
     with pytest.raises(ev.K2Error):
@@ -118,24 +118,24 @@ def test_argument_errors(cfg1):
     assert K.shape == (0,)
 
 
-@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "config2_sub256.npz")),
-                    reason="config2 fixture not generated")
-def test_config2_subsample_vs_golden():
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "w65536x2325572_sub256.npz")),
+                    reason="65,536 x 2,325,572 fixture not generated")
+def test_65536x2325572_subsample_vs_golden():
     """256 candidates of the full-size workload against all 2,325,572 zeros."""
-    c = synth.CONFIG2
-    g = golden("config2_sub256.npz")
+    c = synth.W65536x2325572
+    g = golden("w65536x2325572_sub256.npz")
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     assert tb.sha256() == str(g["table_sha256"])
     t = ev.K2Table(tb)
     K, dK, d2K = t.eval(g["V"], c["k"], flags=ALL)
     s0, s1, s2 = _scales(tb, c["k"])
     e = np.max(np.abs(K - g["K"]))
-    print(f"config2 sub256 plain max|dK2|={e:.3e}")
+    print(f"w65536x2325572 sub256 plain max|dK2|={e:.3e}")
     assert e <= 1e-9
     assert np.max(np.abs(dK - g["dK"])) / s1 <= 1e-11
     Km, dKm, _ = t.eval(g["V"], c["k"], flags=ev.VALUE | ev.D1 | ev.MIXED)
     em = np.max(np.abs(Km - g["K"]))
-    print(f"config2 sub256 mixed max|dK2|={em:.3e}")
+    print(f"w65536x2325572 sub256 mixed max|dK2|={em:.3e}")
     assert em <= 1e-9
 
 
@@ -184,11 +184,11 @@ def test_adaptive_mixed_head(mid_table, k):
 
 
 @pytest.mark.parametrize("n_cand", [1, 5, 8, 17, 64, 65, 200])
-def test_z_tile_path_matches_c_tiles(cfg1, n_cand):
+def test_z_tile_path_matches_c_tiles(w10k, n_cand):
     """Few candidates run as Z tiles (lanes over zeros); forced C tiles must agree, and both
     must match the oracle's golden values."""
-    tb, t, V, k = cfg1
-    g = golden("config1_oracle.npz")
+    tb, t, V, k = w10k
+    g = golden("w1024x10000_oracle.npz")
     Vs = V[:n_cand]
     t.set_tiling(z_max_cand=1000)
     try:
@@ -232,13 +232,13 @@ def test_far_candidates_rotated_table():
     assert np.max(np.abs(got - np.array(want))) < 1e-13
 
 
-def test_handles_from_threads_and_caller_stream(cfg1):
+def test_handles_from_threads_and_caller_stream(w10k):
     """Distinct table handles used from two threads at once give the sequential results, and
     the device entry points run on the caller's stream (a side stream here)."""
     import threading
 
     import torch
-    tb, t, V, k = cfg1
+    tb, t, V, k = w10k
     want = t.eval(V, k, flags=ALL)
     tables = [ev.K2Table(tb), ev.K2Table(tb)]
     got = [None, None]

@@ -120,7 +120,7 @@ def test_phase_window_f_within_bound(small, oracle_lib):
 def test_phase_words_full_table(oracle_lib):
     """All 2,325,572 zeros of the full-size table against candidates spread over
     [3.4e18, 3.6e18]: every 64-bit word bit-exact, every FP32 window within 4 units."""
-    c = synth.CONFIG2
+    c = synth.W65536x2325572
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     t = ev.K2Table(tb)
     V = np.concatenate([synth.v_from_int([int(3.4e18) << 64, (int(3.5e18) << 64) + 12345,

@@ -137,9 +137,9 @@ def test_f64_tier_tracks_quad(oracle_lib):
     assert np.max(np.abs(dK - dKq)) < 1e-9
 
 
-def test_golden_config1_reproduces(oracle_lib):
-    g = golden("config1_oracle.npz")
-    c = synth.CONFIG1
+def test_golden_w1024x10000_reproduces(oracle_lib):
+    g = golden("w1024x10000_oracle.npz")
+    c = synth.W1024x10000
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     assert str(g["table_sha256"]) == tb.sha256()
     V = synth.make_candidates(c["n_cand"], c["v_lo"], c["v_hi"], c["seed"])

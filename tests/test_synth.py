@@ -9,9 +9,9 @@ def test_deterministic_and_pinned():
     a = synth.make_zero_table(10_000, 0)
     b = synth.make_zero_table(10_000, 0)
     assert a.sha256() == b.sha256()
-    assert a.sha256() == str(golden("config1_oracle.npz")["table_sha256"])
+    assert a.sha256() == str(golden("w1024x10000_oracle.npz")["table_sha256"])
     V1 = synth.make_candidates(1024, 1e18, 4e18, 0)
-    assert np.array_equal(V1, golden("config1_oracle.npz")["V"])
+    assert np.array_equal(V1, golden("w1024x10000_oracle.npz")["V"])
 
 
 def test_rvm_count_and_jitter():
