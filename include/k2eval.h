@@ -37,8 +37,9 @@ extern "C" {
 #define K2_VALUE 1u  /* K2 (always computed) */
 #define K2_D1 2u     /* dK2/dv */
 #define K2_D2 4u     /* d2K2/dv2 */
-#define K2_MIXED 8u  /* FP32 sin/cos + weights after the exact phase, compensated FP32 partial
-                        sums, FP64 head zeros and FP64 cross-block sum */
+#define K2_MIXED 8u  /* FP32 sin/cos + weights after the exact removal of the integer turns,
+                        compensated FP32 partial sums, FP64 head zeros and FP64 cross-block sum;
+                        calls with at most 8 candidates run in FP64 throughout (faster there) */
 
 /* error codes */
 #define K2_OK 0

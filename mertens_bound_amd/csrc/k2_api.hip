@@ -352,7 +352,9 @@ static void fill_args(k2_table* t, WeightArrays* ws, const uint4* cand, int n_ca
 // then FP32 tail chunks in the mixed form, one wavefront per candidate in the reduction
 static int eval_core_z(k2_table* t, int n_cand, const uint4* cand, WeightArrays* ws, int nz,
                        unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
-    const bool mixed = (flags & K2_MIXED) != 0;
+    // one candidate group runs in FP64 throughout: there the FP32 tail's second launch costs
+    // more than it saves (1 candidate x 2,325,572 zeros: 0.033 ms in FP64, 0.055 ms mixed)
+    const bool mixed = (flags & K2_MIXED) != 0 && n_cand > K2_Z_CZ;
     const int head = mixed ? (int)mixed_head_zeros(ws->abs_blocks, (uint32_t)nz,
                                                    t->tiling.mixed_head, t->tiling.mixed_tail)
                            : nz;
