@@ -206,6 +206,19 @@ def test_z_tile_path_matches_c_tiles(w10k, n_cand):
     assert np.max(np.abs(Kzm - g["K"][:n_cand])) <= 1e-9
 
 
+def test_mixed_one_z_group_runs_fp64(w10k):
+    """With at most 8 candidates (one Z group) the mixed form runs in FP64 throughout: its
+    results equal the plain form's bit for bit; 9 candidates take the FP32 tail again."""
+    tb, t, V, k = w10k
+    for n in (1, 8):
+        Kp, dKp, _ = t.eval(V[:n], k, flags=ev.VALUE | ev.D1)
+        Km, dKm, _ = t.eval(V[:n], k, flags=ev.VALUE | ev.D1 | ev.MIXED)
+        assert np.array_equal(Kp, Km) and np.array_equal(dKp, dKm)
+    Kp, _, _ = t.eval(V[:9], k, flags=ev.VALUE | ev.D1)
+    Km, _, _ = t.eval(V[:9], k, flags=ev.VALUE | ev.D1 | ev.MIXED)
+    assert not np.array_equal(Kp, Km) and np.max(np.abs(Kp - Km)) <= 1e-9
+
+
 def test_far_candidates_rotated_table():
     """Candidates above y = 2^64 evaluate exactly through formats.rotate_table: the GPU at
     v' = v - 2^64 q on the rotated table against exact big-integer phases with mpmath."""
