@@ -4,8 +4,9 @@
 // 128-bit words split into 32-bit limbs g0..g3 / v0..v3.  The product is formed with 32x32->64
 // multiply-adds on the VALU (v_mad_u64_u32).  Limb products in columns 0 and 1 (bit weights
 // 2^0 and 2^32) sum to less than 2^98 and can only move W by a carry into bit 110, which needs
-// bits [98,110) of the rest of the product to be all ones; that case (probability 2^-12) is
-// detected and recomputed exactly, so W is bit-exact for every input.
+// bits [98,110) of the rest of the product to be all ones; in that case (probability 2^-12) the
+// carry is formed exactly from the three products of columns 0-1, so W is bit-exact for every
+// input.
 //
 // sincos: the phase is in turns, so the split into a table point and a small remainder comes
 // straight from the integer word, with no floating-point range reduction.
@@ -95,7 +96,7 @@ __device__ __forceinline__ uint64_t mad_carry0_from(uint32_t a, uint32_t b, uint
     return r;
 }
 
-// exact W via a straightforward 256-bit schoolbook product (rare fix-up path and debug use)
+// exact W via a straightforward 256-bit schoolbook product (the debug entry's reference path)
 __device__ __noinline__ uint64_t phase_word_exact(Limbs4 g, Limbs4 v) {
     const uint32_t G[4] = {g.x0, g.x1, g.x2, g.x3};
     const uint32_t V[4] = {v.x0, v.x1, v.x2, v.x3};
@@ -147,13 +148,14 @@ __device__ __forceinline__ uint32_t dot2_lo16(uint32_t gp, uint32_t vp, uint32_t
 // (phase_carry_case: bits [98,110) all ones) W must come from phase_word_exact instead.
 template <bool SB = false>
 __device__ __forceinline__ void phase_columns(Limbs4 g, Limbs4 v, uint32_t& s3lo, uint32_t& s4lo,
-                                              uint32_t& s5) {
+                                              uint32_t& s5, uint32_t& tlo) {
     // column 2 (weight 2^64): g0 v2 + g1 v1 + g2 v0; a carry out of a 64-bit add shows as the
     // new sum being below the old one
     uint32_t k2;
     uint64_t t = mul32(g.x0, v.x2);
     t = mad_carry0<SB>(g.x1, v.x1, t, k2);
     t = mad_carry<SB>(g.x2, v.x0, t, k2);
+    tlo = (uint32_t)t;   // product bits [64,96), for the carry fix-up
     // column 3 (2^96): carry-in hi32(t) + k2 2^32
     uint64_t s3 = (t >> 32) | ((uint64_t)k2 << 32);
     uint32_t k3;
@@ -179,15 +181,32 @@ __device__ __forceinline__ bool phase_carry_case(uint32_t s3lo) {
     return ((s3lo >> 2) & 0xFFFu) == 0xFFFu;
 }
 
+// The carry that columns 0-1, T01 = g0 v0 + (g0 v1 + g1 v0) 2^32 < 2^98, add at bit 110 of the
+// partial product P (columns 2..5, zero below bit 64): floor((P mod 2^110 + T01) / 2^110) with
+// P mod 2^110 = A 2^64, A = P bits [64,110) (tlo and s3lo bits [0,14)).  Since A is an integer,
+// that is (A + floor(T01 / 2^64)) >> 46, and floor(T01 / 2^64) = (hi32(g0 v0) + g0 v1 + g1 v0)
+// >> 32 (the low word of g0 v0 cannot carry past a 2^32 boundary of that integer sum).
+__device__ __forceinline__ uint32_t phase_low_carry(Limbs4 g, Limbs4 v, uint32_t s3lo,
+                                                    uint32_t tlo) {
+    const uint64_t c0 = mul32(g.x0, v.x0), c1a = mul32(g.x0, v.x1), c1b = mul32(g.x1, v.x0);
+    const uint64_t s = c1a + c1b;
+    uint32_t k = s < c1a;
+    const uint64_t s2 = s + (c0 >> 32);
+    k += s2 < s;
+    const uint64_t bint = (s2 >> 32) | ((uint64_t)k << 32);   // floor(T01 / 2^64) < 2^35
+    const uint64_t A = ((uint64_t)(s3lo & 0x3FFFu) << 32) | tlo;
+    return (uint32_t)((A + bint) >> 46);
+}
+
 // W with columns 0-1 skipped and the carry case fixed up exactly
 template <bool SB = false>
 __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
-    uint32_t s3lo, s4lo, s5;
-    phase_columns<SB>(g, v, s3lo, s4lo, s5);
+    uint32_t s3lo, s4lo, s5, tlo;
+    phase_columns<SB>(g, v, s3lo, s4lo, s5, tlo);
     const uint32_t wlo = __builtin_amdgcn_alignbit(s4lo, s3lo, 14);
     const uint32_t whi = __builtin_amdgcn_alignbit(s5, s4lo, 14);
     uint64_t W = ((uint64_t)whi << 32) | wlo;
-    if (__builtin_expect(phase_carry_case(s3lo), 0)) W = phase_word_exact(g, v);
+    if (__builtin_expect(phase_carry_case(s3lo), 0)) W += phase_low_carry(g, v, s3lo, tlo);
     return W;
 }
 
@@ -195,11 +214,16 @@ __device__ __forceinline__ uint64_t phase_word(Limbs4 g, Limbs4 v) {
 // bits [112,164), and W bits [54,64) are bits [4,14) of the column-5 word, so off = s5 & 0x3FF0
 template <bool SB = false>
 __device__ __forceinline__ PhaseRem phase_rem(Limbs4 g, Limbs4 v) {
-    uint32_t s3lo, s4lo, s5;
-    phase_columns<SB>(g, v, s3lo, s4lo, s5);
+    uint32_t s3lo, s4lo, s5, tlo;
+    phase_columns<SB>(g, v, s3lo, s4lo, s5, tlo);
     PhaseRem r{__builtin_amdgcn_alignbit(s4lo, s3lo, 16), __builtin_amdgcn_alignbit(s5, s4lo, 16),
                s5 & 0x3FF0u};
-    if (__builtin_expect(phase_carry_case(s3lo), 0)) r = phase_rem_of(phase_word_exact(g, v));
+    if (__builtin_expect(phase_carry_case(s3lo), 0)) {
+        const uint64_t W = ((uint64_t)__builtin_amdgcn_alignbit(s5, s4lo, 14) << 32 |
+                            __builtin_amdgcn_alignbit(s4lo, s3lo, 14)) +
+                           phase_low_carry(g, v, s3lo, tlo);
+        r = phase_rem_of(W);
+    }
     return r;
 }
 
