@@ -224,8 +224,13 @@ def run_eval(args, form):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    t.profile(True)
     ms, wall = timed(step, args.steps, 0, torch)
+    # the kernels' own average from a second, profiled pass (per-launch events cost microseconds,
+    # which would show in the timed region of the small workloads)
+    t.profile(True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
     k_ms, k_n = t.kernel_time()
     t.profile(False)
     evals = float(n_cand) * tb.n
