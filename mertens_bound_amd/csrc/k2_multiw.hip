@@ -8,9 +8,9 @@
 // truncated vectors).  The exact phase and the sin/cos are computed once per candidate and zero
 // and shared by all vectors: each further vector costs one FMA per requested sum.
 //
-// C tiles of 256 lanes, NC candidates per lane (2 while the NC * sums * M accumulators fit, so
-// that each staged zero record and weight feeds two candidates); zero records of 128 zeros staged
-// in LDS (double buffered): G, w, g (derivatives only) and 2 wt_m; split partial rows
+// C tiles of 256 lanes, NC candidates per lane (4 for up to 5 sums x vectors, 2 for up to 10, so
+// that each staged zero record and weight feeds several candidates); zero records of 128 zeros
+// staged in LDS (double buffered): G, w, g (derivatives only) and 2 wt_m; split partial rows
 // [split][q*M + m][cand].
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -164,7 +164,7 @@ namespace k2impl {
 // candidates per lane for M vectors and these sums
 static int mw_nc(int M, unsigned fl) {
     const int nq = 1 + ((fl & K2_D1) ? 1 : 0) + ((fl & K2_D2) ? 1 : 0);
-    return nq * M <= 10 ? 2 : 1;
+    return nq * M <= 5 ? 4 : (nq * M <= 10 ? 2 : 1);
 }
 
 template <int M, int FL>
@@ -172,7 +172,7 @@ static hipError_t launch_mw_t(dim3 grid, const k2_table* t, const double* wt, in
                               int zps, const uint4* cand, int n_cand, int n_ctiles,
                               double* partial, int n_cand_pad, hipStream_t st) {
     constexpr int NQ = 1 + ((FL & K2_D1) ? 1 : 0) + ((FL & K2_D2) ? 1 : 0);
-    constexpr int NC = NQ * M <= 10 ? 2 : 1;
+    constexpr int NC = NQ * M <= 5 ? 4 : (NQ * M <= 10 ? 2 : 1);
     hipLaunchKernelGGL((k2_multiw_kernel<M, FL, NC>), grid, dim3(256), 0, st, t->g,
                        (const double2*)t->w, t->gamma, wt, ld, nz, zps, cand, n_cand, n_ctiles,
                        partial, n_cand_pad, t->tab64);
