@@ -10,7 +10,8 @@ counts only with |dK2| <= 1e-9 against the multiprecision oracle, checked on the
 Other workloads (--workload): mixed (FP32 form of the same), batched (512 uneven requests in one
 persistent launch, graph replay), polish (4,096 candidates, 8 Newton steps), small (10,000 zeros x
 1,024 candidates), enclose (the default workload with rigorous radii), weights (five weight
-vectors in one pass), copy (a 2 GiB device copy, the byte-counter calibration for rocprofv3 --pmc).
+vectors in one pass), screen (the staged 64/128/256-zero leaf screen on 2^24 candidates), copy (a
+2 GiB device copy, the byte-counter calibration for rocprofv3 --pmc).
 
 Prints one JSON line.
 """
@@ -41,7 +42,11 @@ WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
         # 4 FP32 FMAs, x and x/2 in 3, the rotation in 4, the sums in 4 (half a slot each at the
         # published FP32 peak); its FP64 head is charged at the plain form's 36 slots
         "mixed": {"int32_ops": 6, "fp32_ops": 15, "fp32_slot": 0.5, "slots": 13.5},
-        "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48}}
+        "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48},
+        # the leaf screen's loop per candidate and zero (7 shifted terms), as compiled: 14
+        # integer instructions of the exact phase window and its conversion, 37 FP32 (table
+        # sin/cos, the weight rotation, 6 shift rotations, 7 sums) at half a slot
+        "screen": {"int32_ops": 14, "fp32_ops": 37, "fp32_slot": 0.5, "slots": 32.5}}
 # Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
 # published 78.6 TFLOP/s FP64 peak counted per FMA (2 flops).  The roofline reports the work in
 # FP64-FMA-equivalent flops (2 per slot) against that published peak.
@@ -484,6 +489,59 @@ def run_weights(args):
     return res
 
 
+def run_screen(args):
+    """The staged leaf screen (k2_screen_device) of search/gpu_mertens_stream.cu:186-250 with its
+    constants: 64/128/256 zeros, thresholds 0.78/0.855/0.892, 7 shifts of 0.0014, on 2^24
+    candidates v in [3.4e18, 3.6e18] against the first zeros of the full table (k = 3e-9).  A
+    zero evaluation is one candidate against one zero with all 7 shifted terms; a candidate that
+    fails a stage stops there."""
+    import numpy as np
+    import torch
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import pipeline, synth
+    c = synth.W65536x2325572
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    n_cand = args.n_cand or (1 << 24)
+    V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
+    torch.cuda.set_device(0)
+    t = ev.K2Table(tb)
+    Vd = ev.v_tensor(V)
+    k, d, m = 3e-9, 0.0014, 3
+    stages = pipeline.REFERENCE_SCREEN
+    state = {}
+
+    def step():
+        state["r"] = t.screen_device(Vd, k, d, m, stages)
+
+    ms, wall = timed(step, args.steps, args.warmup, torch)
+    val = state["r"][0].cpu().numpy()
+    passed = int(state["r"][2].sum().item())
+    nz = np.array([a for a, _ in stages])
+    reached = (val[:, :len(stages)] >= 0).sum(axis=0)            # candidates reaching each stage
+    zero_evals = float(np.sum(reached * np.diff(np.concatenate([[0], nz]))))
+    res = base_record(args, "screened candidates/sec (64/128/256-zero staged screen, 7 shifts)",
+                      n_cand / (ms * 1e-3), "candidates/s", ms, "u32+f32",
+                      {"workload": f"leaf screen, {n_cand:,} candidates, stages "
+                                   f"{'/'.join(str(a) for a, _ in stages)} zeros, thresholds "
+                                   f"{'/'.join(str(b) for _, b in stages)}, {2 * m + 1} shifts "
+                                   f"of {d}, k={k}", "form": "screen", "n_cand": n_cand})
+    res["screen"] = {"reached_per_stage": [int(x) for x in reached], "passed": passed,
+                     "zero_evals_per_step": zero_evals,
+                     "zero_evals_per_s": zero_evals / (ms * 1e-3)}
+    rate = zero_evals / (ms * 1e-3)
+    w = WORK["screen"]
+    achieved = rate * 2 * w["slots"] / 1e12
+    res["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                       "traffic": traffic_summary("screen"),
+                       "achieved_valu_slots_tops": rate * w["slots"] / 1e12,
+                       "valu_slot_peak_tops": VALU_PEAK_TOPS, "kernel_ms": ms,
+                       "work_per_zero_eval": w, "algorithmic_bytes": n_cand * 40,
+                       "hbm_GBps_algorithmic": n_cand * 40 / (ms * 1e-3) / 1e9}
+    res["wall_s"] = wall
+    return res
+
+
 def run_copy(args):
     import torch
     n = 1 << 28                                  # 2 GiB of float64
@@ -500,7 +558,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["metric", "mixed", "batched", "polish", "small",
-                                           "enclose", "weights", "copy"], default="metric")
+                                           "enclose", "weights", "screen", "copy"],
+                    default="metric")
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
     ap.add_argument("--n-cand", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -525,6 +584,8 @@ def main():
         res = run_enclose(args)
     elif args.workload == "weights":
         res = run_weights(args)
+    elif args.workload == "screen":
+        res = run_screen(args)
     else:
         res = run_copy(args)
     print(json.dumps(res))

@@ -6,11 +6,14 @@
 // and staged cutoffs: stage s continues only if the value after n_s zeros reaches thr_s
 // (reference: 64/128/256 zeros, 0.78/0.855/0.892).  Here the phase is the integer-reduced window
 // of the FP32 form (to 2^-44 turn, not a wrapped 64-bit approximation) and sin/cos come from the
-// FP32 LDS table (not a 12-bit cos lookup); the shifted terms use one rotation per shift step,
-// e^{i g_j d}, per zero.
+// FP32 LDS table (not a 12-bit cos lookup); the shifted terms use the per-zero rotations
+// e^{i m g_j d}, formed once per call in FP64: Re(q e^{+-i m g d}) is two FMAs each.
 //
-// One lane per candidate; the zeros of all stages (<= K2_SCREEN_MAX) are staged in LDS once per
-// workgroup.  A lane that fails a stage stops accumulating; its later stage values stay -1.
+// One lane per candidate.  The zero records of all stages (<= K2_SCREEN_MAX: G, the FP32 weight
+// 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once per call by
+// k2_screen_records_kernel; every lane reads the same record at the same time, so the screen
+// kernel takes them through the scalar cache, and LDS holds only the 16 KiB sin/cos table.
+// A lane that fails a stage stops accumulating; its later stage values stay -1.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -28,32 +31,46 @@ using namespace k2dev;
 #define K2_SCREEN_STAGES 4
 #define K2_SCREEN_MAXM 8     // shifts -M..M, M <= 8
 
-struct ScreenZero {
+// one zero of the screen: r[m-1] = e^{i m g d} (r padded to an odd count: 16-byte records)
+template <int M>
+struct ScreenRec {
     uint4 g;
     float2 a;     // 2 e^{-k g^2} w (FP32)
-    float2 r;     // e^{i g d}
+    float2 r[M | 1];
 };
+#define K2_SCREEN_REC_MAX sizeof(ScreenRec<K2_SCREEN_MAXM>)
+
+// the zero records of one call, one thread per zero
+template <int M>
+__global__ __launch_bounds__(256) void k2_screen_records_kernel(
+    const uint4* __restrict__ zg, const double* __restrict__ gamma, const double2* __restrict__ w,
+    double k, double d, int n_zeros, ScreenRec<M>* __restrict__ rec) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_zeros) return;
+    const double g = gamma[j];
+    const double f = 2.0 * exp(-k * g * g);
+    const double2 ww = w[j];
+    ScreenRec<M> z;
+    z.g = zg[j];
+    z.a = make_float2((float)(f * ww.x), (float)(f * ww.y));
+#pragma unroll
+    for (int m = 0; m < (M | 1); ++m) {
+        double sr = 0.0, cr = 1.0;
+        if (m < M) sincos_general((double)(m + 1) * g * d, sr, cr);
+        z.r[m] = make_float2((float)cr, (float)sr);
+    }
+    rec[j] = z;
+}
 
 template <int M>
 __global__ __launch_bounds__(256) void k2_screen_kernel(
-    const uint4* __restrict__ zg, const double* __restrict__ gamma, const double2* __restrict__ w,
-    double k, double d, int n_zeros, const uint4* __restrict__ cand, int n_cand, int n_stages,
-    const int* __restrict__ stage_nz, const float* __restrict__ stage_thr,
+    const ScreenRec<M>* __restrict__ rec, const uint4* __restrict__ cand, int n_cand,
+    int n_stages, const int* __restrict__ stage_nz, const float* __restrict__ stage_thr,
     const Tab32* __restrict__ tab, float* __restrict__ out_val, int* __restrict__ out_shift,
     int* __restrict__ out_pass) {
-    __shared__ ScreenZero sz[K2_SCREEN_MAX];
     __shared__ Tab32 s_tab[K2_TAB32];
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB32; i += blockDim.x) s_tab[i] = tab[i];
-    for (int j = tid; j < n_zeros; j += blockDim.x) {
-        const double g = gamma[j];
-        const double f = 2.0 * exp(-k * g * g);
-        const double2 ww = w[j];
-        double sr, cr;
-        sincos_general(g * d, sr, cr);
-        sz[j] = ScreenZero{zg[j], make_float2((float)(f * ww.x), (float)(f * ww.y)),
-                           make_float2((float)cr, (float)sr)};
-    }
     __syncthreads();
     const int c = blockIdx.x * blockDim.x + tid;
     if (c >= n_cand) return;
@@ -66,7 +83,7 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
     for (int s = 0; s < n_stages; ++s) {
         const int nz = stage_nz[s];
         for (; j < nz; ++j) {
-            const ScreenZero z = sz[j];
+            const ScreenRec<M> z = rec[j];   // the same j in every lane
             uint32_t hi, lo;
             phase_window<K2_TAB32_BITS>(Limbs4{z.g.x, z.g.y, z.g.z, z.g.w}, v, hi, lo);
             float S, C;
@@ -75,17 +92,11 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
             const float qr = fmaf(z.a.x, C, -z.a.y * S);
             const float qi = fmaf(z.a.x, S, z.a.y * C);
             acc[M] += qr;
-            float pr = qr, pi = qi, nr = qr, ni = qi;
 #pragma unroll
-            for (int m = 1; m <= M; ++m) {
-                const float t1 = fmaf(pr, z.r.x, -pi * z.r.y);   // p *= r
-                pi = fmaf(pr, z.r.y, pi * z.r.x);
-                pr = t1;
-                const float t2 = fmaf(nr, z.r.x, ni * z.r.y);    // n *= conj(r)
-                ni = fmaf(ni, z.r.x, -nr * z.r.y);
-                nr = t2;
-                acc[M + m] += pr;
-                acc[M - m] += nr;
+            for (int m = 1; m <= M; ++m) {   // Re(q e^{+i m g d}), Re(q e^{-i m g d})
+                const float2 r = z.r[m - 1];
+                acc[M + m] = fmaf(-qi, r.y, fmaf(qr, r.x, acc[M + m]));
+                acc[M - m] = fmaf(qi, r.y, fmaf(qr, r.x, acc[M - m]));
             }
         }
         float mv = -1.0f;
@@ -108,23 +119,30 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
 
 namespace k2impl {
 
+// records of n_zeros zeros into rec (room for K2_SCREEN_MAX records of ScreenRec<M>), then the
+// screen
 template <int M>
-static hipError_t launch_screen_m(const uint4* zg, const double* gamma, const double2* w, double k,
-                                  double d, int n_zeros, const uint4* cand, int n_cand,
-                                  int n_stages, const int* nz, const float* thr, const Tab32* tab,
-                                  float* val, int* shift, int* pass, hipStream_t st) {
-    hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st, zg,
-                       gamma, w, k, d, n_zeros, cand, n_cand, n_stages, nz, thr, tab, val, shift,
+static hipError_t launch_screen_m(const uint4* zg, const double* gamma, const double2* w,
+                                  double k, double d, int n_zeros, void* rec, const uint4* cand,
+                                  int n_cand, int n_stages, const int* nz, const float* thr,
+                                  const Tab32* tab, float* val, int* shift, int* pass,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL((k2_screen_records_kernel<M>), dim3((n_zeros + 255) / 256), dim3(256), 0,
+                       st, zg, gamma, w, k, d, n_zeros, (ScreenRec<M>*)rec);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st,
+                       (const ScreenRec<M>*)rec, cand, n_cand, n_stages, nz, thr, tab, val, shift,
                        pass);
     return hipGetLastError();
 }
 
 hipError_t launch_screen(int M, const uint4* zg, const double* gamma, const double2* w, double k,
-                         double d, int n_zeros, const uint4* cand, int n_cand, int n_stages,
-                         const int* nz, const float* thr, const Tab32* tab, float* val,
-                         int* shift, int* pass, hipStream_t st) {
+                         double d, int n_zeros, void* rec, const uint4* cand, int n_cand,
+                         int n_stages, const int* nz, const float* thr, const Tab32* tab,
+                         float* val, int* shift, int* pass, hipStream_t st) {
     switch (M) {
-#define K2_M(X) case X: return launch_screen_m<X>(zg, gamma, w, k, d, n_zeros, cand, n_cand, \
+#define K2_M(X) case X: return launch_screen_m<X>(zg, gamma, w, k, d, n_zeros, rec, cand, n_cand, \
                                                   n_stages, nz, thr, tab, val, shift, pass, st);
         K2_M(0) K2_M(1) K2_M(2) K2_M(3) K2_M(4) K2_M(5) K2_M(6) K2_M(7) K2_M(8)
 #undef K2_M
@@ -160,10 +178,10 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
     std::lock_guard<std::mutex> lk(t->mu);
     hipError_t e = hipSetDevice(t->device);
     hipStream_t st = (hipStream_t)stream;
-    // stage parameters travel in a small device buffer owned by the table
+    // stage parameters (256 B) and the zero records travel in a device buffer owned by the table
     const size_t need = 2 * K2_SCREEN_STAGES * 4;
     if (e == hipSuccess && t->screen_params == nullptr)
-        e = hipMalloc((void**)&t->screen_params, 256);
+        e = hipMalloc((void**)&t->screen_params, 256 + K2_SCREEN_MAX * K2_SCREEN_REC_MAX);
     int32_t hostp[2 * K2_SCREEN_STAGES] = {0};
     for (int s = 0; s < n_stages; ++s) {
         hostp[s] = stage_zeros[s];
@@ -175,7 +193,8 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess)
         e = launch_screen(n_shift, t->g, t->gamma, (const double2*)t->w, k, shift_step,
-                          stage_zeros[n_stages - 1], (const uint4*)V_dev, (int)n_cand, n_stages,
+                          stage_zeros[n_stages - 1], (char*)t->screen_params + 256,
+                          (const uint4*)V_dev, (int)n_cand, n_stages,
                           (const int*)t->screen_params,
                           (const float*)(t->screen_params + K2_SCREEN_STAGES), t->tab32, val_dev,
                           best_shift_dev, pass_dev, st);
