@@ -180,9 +180,10 @@ int k2_mixed_head(k2_table *t, double k, uint32_t n_zeros, uint32_t *head);
  * sums h_m = K2 over the first n zeros at v + m*shift_step, stage value max_m |h_m|; stage s
  * (stage_zeros[s] zeros, increasing, <= 1024) continues only if its value reaches
  * stage_thresholds[s].  val_dev: [n_cand][4] stage values (-1 where not reached), best_shift_dev:
- * argmax m of the last completed stage, pass_dev: 1 if all stages passed.  FP32 sums after the
- * integer phase reduction of the FP32 form.  Replaces the leaf screen of search/gpu_mertens_stream.cu:186-250
- * (64/128/256 zeros, 7 shifts of 0.0014, thresholds 0.78/0.855/0.892). */
+ * argmax m of the last completed stage, pass_dev: 1 if all stages passed; n_cand <= 2^30.  FP32
+ * sums after the integer phase reduction of the FP32 form.  Replaces the leaf screen of
+ * search/gpu_mertens_stream.cu:186-250 (64/128/256 zeros, 7 shifts of 0.0014, thresholds
+ * 0.78/0.855/0.892). */
 int k2_screen_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double k,
                      double shift_step, int n_shift, int n_stages, const int32_t *stage_zeros,
                      const float *stage_thresholds, float *val_dev, int32_t *best_shift_dev,

@@ -108,11 +108,13 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
         out_val[(size_t)c * K2_SCREEN_STAGES + s] = mv;
         if (mv < stage_thr[s]) {
             passed = 0;
-            for (int t = s + 1; t < K2_SCREEN_STAGES; ++t) out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
+            for (int t = s + 1; t < K2_SCREEN_STAGES; ++t)
+                out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
             break;
         }
     }
-    for (int t = n_stages; t < K2_SCREEN_STAGES; ++t) out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
+    for (int t = n_stages; t < K2_SCREEN_STAGES; ++t)
+        out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
     out_shift[c] = best - M;
     out_pass[c] = passed;
 }
@@ -168,12 +170,15 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
         return set_error(K2_E_ARG, "k2_screen_device: 1..4 stages and 0..8 shifts per side");
     int prev = 0;
     for (int s = 0; s < n_stages; ++s) {
-        if (stage_zeros[s] <= prev || stage_zeros[s] > K2_SCREEN_MAX || (uint32_t)stage_zeros[s] > t->n)
-            return set_error(K2_E_ARG, "k2_screen_device: stage zero counts must increase, <= 1024");
+        if (stage_zeros[s] <= prev || stage_zeros[s] > K2_SCREEN_MAX ||
+            (uint32_t)stage_zeros[s] > t->n)
+            return set_error(K2_E_ARG,
+                             "k2_screen_device: stage zero counts must increase, <= 1024");
         prev = stage_zeros[s];
     }
     if (!(k >= 0.0) || !(shift_step == shift_step))
         return set_error(K2_E_ARG, "k2_screen_device: bad k or shift step");
+    if (n_cand > (1u << 30)) return set_error(K2_E_ARG, "k2_screen_device: too many candidates");
     if (n_cand == 0) return K2_OK;
     std::lock_guard<std::mutex> lk(t->mu);
     hipError_t e = hipSetDevice(t->device);
@@ -198,6 +203,7 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
                           (const int*)t->screen_params,
                           (const float*)(t->screen_params + K2_SCREEN_STAGES), t->tab32, val_dev,
                           best_shift_dev, pass_dev, st);
-    if (e != hipSuccess) return set_error(K2_E_HIP, std::string("k2_screen_device: ") + hipGetErrorString(e));
+    if (e != hipSuccess)
+        return set_error(K2_E_HIP, std::string("k2_screen_device: ") + hipGetErrorString(e));
     return K2_OK;
 }
