@@ -43,10 +43,10 @@ WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
         # published FP32 peak); its FP64 head is charged at the plain form's 36 slots
         "mixed": {"int32_ops": 6, "fp32_ops": 15, "fp32_slot": 0.5, "slots": 13.5},
         "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48},
-        # the leaf screen's loop per candidate and zero (7 shifted terms), as compiled: 14
-        # integer instructions of the exact phase window and its conversion, 37 FP32 (table
-        # sin/cos, the weight rotation, 6 shift rotations, 7 sums) at half a slot
-        "screen": {"int32_ops": 14, "fp32_ops": 37, "fp32_slot": 0.5, "slots": 32.5}}
+        # the leaf screen's loop per candidate and zero (7 shifted terms), as compiled: 15
+        # integer instructions of the exact phase window and its conversion, 23 FP32 (table
+        # sin/cos 6, the weight rotation 4, the 7 shifted sums 13) at half a slot
+        "screen": {"int32_ops": 15, "fp32_ops": 23, "fp32_slot": 0.5, "slots": 26.5}}
 # Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
 # published 78.6 TFLOP/s FP64 peak counted per FMA (2 flops).  The roofline reports the work in
 # FP64-FMA-equivalent flops (2 per slot) against that published peak.
