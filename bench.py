@@ -43,10 +43,11 @@ WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
         # published FP32 peak); its FP64 head is charged at the plain form's 36 slots
         "mixed": {"int32_ops": 6, "fp32_ops": 15, "fp32_slot": 0.5, "slots": 13.5},
         "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48},
-        # the leaf screen's loop per candidate and zero (7 shifted terms), as compiled: 15
-        # integer instructions of the exact phase window and its conversion, 23 FP32 (table
-        # sin/cos 6, the weight rotation 4, the 7 shifted sums 13) at half a slot
-        "screen": {"int32_ops": 15, "fp32_ops": 23, "fp32_slot": 0.5, "slots": 26.5}}
+        # the leaf screen's loop per candidate and zero (7 shifted terms), as compiled: 7
+        # integer instructions (the precision form's window: 3 multiply-adds, the packed dot
+        # product, the remainder's funnel shift, mask and conversion), 28 FP32 (column 3 4, table
+        # sin/cos 7, the weight rotation 4, the 7 shifted sums 13) at half a slot
+        "screen": {"int32_ops": 7, "fp32_ops": 28, "fp32_slot": 0.5, "slots": 21.0}}
 # Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
 # published 78.6 TFLOP/s FP64 peak counted per FMA (2 flops).  The roofline reports the work in
 # FP64-FMA-equivalent flops (2 per slot) against that published peak.

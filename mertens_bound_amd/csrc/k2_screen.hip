@@ -4,13 +4,14 @@
 // candidate v, the sums h_m = sum_{j<n} 2 e^{-k g_j^2} Re(w_j e^{i g_j (v + m d)}) over the first
 // n zeros at 2M+1 shifts m = -M..M (reference: M = 3, d = 0.0014), the stage value max_m |h_m|,
 // and staged cutoffs: stage s continues only if the value after n_s zeros reaches thr_s
-// (reference: 64/128/256 zeros, 0.78/0.855/0.892).  Here the phase is the integer-reduced window
-// of the FP32 form (to 2^-44 turn, not a wrapped 64-bit approximation) and sin/cos come from the
-// FP32 LDS table (not a 12-bit cos lookup); the shifted terms use the per-zero rotations
+// (reference: 64/128/256 zeros, 0.78/0.855/0.892).  Here the phase is the precision form's window
+// (columns 4..5 of G V exact in integers, column 3 in FP32: within 2^-34 turn, not a wrapped
+// 64-bit approximation) and sin/cos come from the FP32 LDS table (not a 12-bit cos lookup); the shifted terms use the per-zero rotations
 // e^{i m g_j d}, formed once per call in FP64: Re(q e^{+-i m g d}) is two FMAs each.
 //
-// One lane per candidate.  The zero records of all stages (<= K2_SCREEN_MAX: G, the FP32 weight
-// 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once per call by
+// One lane per candidate.  The zero records of all stages (<= K2_SCREEN_MAX: G and its limbs as
+// floats, the FP32 weight 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once
+// per call by
 // k2_screen_records_kernel; every lane reads the same record at the same time, so the screen
 // kernel takes them through the scalar cache, and LDS holds only the 16 KiB sin/cos table.
 // A lane that fails a stage stops accumulating; its later stage values stay -1.
@@ -31,10 +32,13 @@ using namespace k2dev;
 #define K2_SCREEN_STAGES 4
 #define K2_SCREEN_MAXM 8     // shifts -M..M, M <= 8
 
-// one zero of the screen: r[m-1] = e^{i m g d} (r padded to an odd count: 16-byte records)
+// one zero of the screen: g = (column 5's packed limbs pack_lo16(g2, g3), g1, g2, g3) as
+// phase_window_f reads them, gf = g0..g3 as floats, r[m-1] = e^{i m g d} (r padded to an odd
+// count: 16-byte records)
 template <int M>
 struct ScreenRec {
     uint4 g;
+    float4 gf;
     float2 a;     // 2 e^{-k g^2} w (FP32)
     float2 r[M | 1];
 };
@@ -43,15 +47,18 @@ struct ScreenRec {
 // the zero records of one call, one thread per zero
 template <int M>
 __global__ __launch_bounds__(256) void k2_screen_records_kernel(
-    const uint4* __restrict__ zg, const double* __restrict__ gamma, const double2* __restrict__ w,
-    double k, double d, int n_zeros, ScreenRec<M>* __restrict__ rec) {
+    const uint4* __restrict__ zg, const float4* __restrict__ zgf, const double* __restrict__ gamma,
+    const double2* __restrict__ w, double k, double d, int n_zeros,
+    ScreenRec<M>* __restrict__ rec) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n_zeros) return;
     const double g = gamma[j];
     const double f = 2.0 * exp(-k * g * g);
     const double2 ww = w[j];
     ScreenRec<M> z;
-    z.g = zg[j];
+    const uint4 gg = zg[j];
+    z.g = make_uint4(pack_lo16(gg.z, gg.w), gg.y, gg.z, gg.w);
+    z.gf = zgf[j];
     z.a = make_float2((float)(f * ww.x), (float)(f * ww.y));
 #pragma unroll
     for (int m = 0; m < (M | 1); ++m) {
@@ -76,6 +83,8 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
     if (c >= n_cand) return;
     const uint4 cv = cand[c];
     const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
+    const float4 vf = window_vf(v);
+    const uint32_t vp = pack_lo16(v.x3, v.x2);
     float acc[2 * M + 1];
 #pragma unroll
     for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
@@ -85,9 +94,11 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
         for (; j < nz; ++j) {
             const ScreenRec<M> z = rec[j];   // the same j in every lane
             uint32_t hi, lo;
-            phase_window<K2_TAB32_BITS>(Limbs4{z.g.x, z.g.y, z.g.z, z.g.w}, v, hi, lo);
+            float t3;
+            phase_window_f<K2_TAB32_BITS>(Limbs4{0u, z.g.y, z.g.z, z.g.w}, z.g.x, z.gf, v, vp, vf,
+                                          hi, lo, t3);
             float S, C;
-            sincos_window<K2_TAB32_BITS>(hi, lo, s_tab, S, C);
+            sincos_window_f<K2_TAB32_BITS>(hi, lo, t3, s_tab, S, C);
             // q = a e^{i th}
             const float qr = fmaf(z.a.x, C, -z.a.y * S);
             const float qi = fmaf(z.a.x, S, z.a.y * C);
@@ -124,13 +135,13 @@ namespace k2impl {
 // records of n_zeros zeros into rec (room for K2_SCREEN_MAX records of ScreenRec<M>), then the
 // screen
 template <int M>
-static hipError_t launch_screen_m(const uint4* zg, const double* gamma, const double2* w,
-                                  double k, double d, int n_zeros, void* rec, const uint4* cand,
-                                  int n_cand, int n_stages, const int* nz, const float* thr,
-                                  const Tab32* tab, float* val, int* shift, int* pass,
-                                  hipStream_t st) {
+static hipError_t launch_screen_m(const uint4* zg, const float4* zgf, const double* gamma,
+                                  const double2* w, double k, double d, int n_zeros, void* rec,
+                                  const uint4* cand, int n_cand, int n_stages, const int* nz,
+                                  const float* thr, const Tab32* tab, float* val, int* shift,
+                                  int* pass, hipStream_t st) {
     hipLaunchKernelGGL((k2_screen_records_kernel<M>), dim3((n_zeros + 255) / 256), dim3(256), 0,
-                       st, zg, gamma, w, k, d, n_zeros, (ScreenRec<M>*)rec);
+                       st, zg, zgf, gamma, w, k, d, n_zeros, (ScreenRec<M>*)rec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st,
@@ -139,13 +150,15 @@ static hipError_t launch_screen_m(const uint4* zg, const double* gamma, const do
     return hipGetLastError();
 }
 
-hipError_t launch_screen(int M, const uint4* zg, const double* gamma, const double2* w, double k,
-                         double d, int n_zeros, void* rec, const uint4* cand, int n_cand,
-                         int n_stages, const int* nz, const float* thr, const Tab32* tab,
-                         float* val, int* shift, int* pass, hipStream_t st) {
+hipError_t launch_screen(int M, const uint4* zg, const float4* zgf, const double* gamma,
+                         const double2* w, double k, double d, int n_zeros, void* rec,
+                         const uint4* cand, int n_cand, int n_stages, const int* nz,
+                         const float* thr, const Tab32* tab, float* val, int* shift, int* pass,
+                         hipStream_t st) {
     switch (M) {
-#define K2_M(X) case X: return launch_screen_m<X>(zg, gamma, w, k, d, n_zeros, rec, cand, n_cand, \
-                                                  n_stages, nz, thr, tab, val, shift, pass, st);
+#define K2_M(X) case X: return launch_screen_m<X>(zg, zgf, gamma, w, k, d, n_zeros, rec, cand, \
+                                                  n_cand, n_stages, nz, thr, tab, val, shift, \
+                                                  pass, st);
         K2_M(0) K2_M(1) K2_M(2) K2_M(3) K2_M(4) K2_M(5) K2_M(6) K2_M(7) K2_M(8)
 #undef K2_M
         default: return hipErrorInvalidValue;
@@ -197,7 +210,7 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
         e = hipMemcpyAsync(t->screen_params, hostp, need, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess)
-        e = launch_screen(n_shift, t->g, t->gamma, (const double2*)t->w, k, shift_step,
+        e = launch_screen(n_shift, t->g, t->gf, t->gamma, (const double2*)t->w, k, shift_step,
                           stage_zeros[n_stages - 1], (char*)t->screen_params + 256,
                           (const uint4*)V_dev, (int)n_cand, n_stages,
                           (const int*)t->screen_params,
