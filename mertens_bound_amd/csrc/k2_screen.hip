@@ -6,18 +6,18 @@
 // and staged cutoffs: stage s continues only if the value after n_s zeros reaches thr_s
 // (reference: 64/128/256 zeros, 0.78/0.855/0.892).  Here the phase is the precision form's window
 // (columns 4..5 of G V exact in integers, column 3 in FP32: within 2^-34 turn, not a wrapped
-// 64-bit approximation) and sin/cos come from the FP32 LDS table (not a 12-bit cos lookup); the shifted terms use the per-zero rotations
-// e^{i m g_j d}, formed once per call in FP64: Re(q e^{+-i m g d}) is two FMAs each.
+// 64-bit approximation) and sin/cos come from the FP32 LDS table (not a 12-bit cos lookup); the
+// shifted terms use the per-zero rotations e^{i m g_j d}, formed once per call in FP64:
+// Re(q e^{+-i m g d}) is two FMAs each.
 //
 // One lane per candidate.  The zero records of all stages (<= K2_SCREEN_MAX: G and its limbs as
 // floats, the FP32 weight 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once
-// per call by
-// k2_screen_records_kernel; every lane reads the same record at the same time, so the screen
-// kernel takes them through the scalar cache, and LDS holds only the 16 KiB sin/cos table.
-// A lane that fails a stage stops accumulating; its later stage values stay -1.
+// per call by k2_screen_records_kernel; every lane reads the same record at the same time, so the
+// screen kernel takes them through the scalar cache, and LDS holds only the 16 KiB sin/cos table.
+// A lane that fails a stage stops accumulating; its later stage values stay -1.  The stage zero
+// counts and thresholds are kernel arguments: the call does not synchronise.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include <string.h>
 
 #include <string>
 #include <vector>
@@ -31,6 +31,12 @@ using namespace k2dev;
 #define K2_SCREEN_MAX 1024   // zeros the screen can stage
 #define K2_SCREEN_STAGES 4
 #define K2_SCREEN_MAXM 8     // shifts -M..M, M <= 8
+
+// stage zero counts and thresholds, passed by value (kernel arguments)
+struct ScreenStages {
+    int nz[K2_SCREEN_STAGES];
+    float thr[K2_SCREEN_STAGES];
+};
 
 // one zero of the screen: g = (column 5's packed limbs pack_lo16(g2, g3), g1, g2, g3) as
 // phase_window_f reads them, gf = g0..g3 as floats, r[m-1] = e^{i m g d} (r padded to an odd
@@ -72,7 +78,7 @@ __global__ __launch_bounds__(256) void k2_screen_records_kernel(
 template <int M>
 __global__ __launch_bounds__(256) void k2_screen_kernel(
     const ScreenRec<M>* __restrict__ rec, const uint4* __restrict__ cand, int n_cand,
-    int n_stages, const int* __restrict__ stage_nz, const float* __restrict__ stage_thr,
+    int n_stages, ScreenStages sp,
     const Tab32* __restrict__ tab, float* __restrict__ out_val, int* __restrict__ out_shift,
     int* __restrict__ out_pass) {
     __shared__ Tab32 s_tab[K2_TAB32];
@@ -90,7 +96,7 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
     for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
     int j = 0, best = M, passed = 1;
     for (int s = 0; s < n_stages; ++s) {
-        const int nz = stage_nz[s];
+        const int nz = sp.nz[s];
         for (; j < nz; ++j) {
             const ScreenRec<M> z = rec[j];   // the same j in every lane
             uint32_t hi, lo;
@@ -117,7 +123,7 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
             if (a > mv) { mv = a; best = m; }
         }
         out_val[(size_t)c * K2_SCREEN_STAGES + s] = mv;
-        if (mv < stage_thr[s]) {
+        if (mv < sp.thr[s]) {
             passed = 0;
             for (int t = s + 1; t < K2_SCREEN_STAGES; ++t)
                 out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
@@ -137,28 +143,27 @@ namespace k2impl {
 template <int M>
 static hipError_t launch_screen_m(const uint4* zg, const float4* zgf, const double* gamma,
                                   const double2* w, double k, double d, int n_zeros, void* rec,
-                                  const uint4* cand, int n_cand, int n_stages, const int* nz,
-                                  const float* thr, const Tab32* tab, float* val, int* shift,
-                                  int* pass, hipStream_t st) {
+                                  const uint4* cand, int n_cand, int n_stages, ScreenStages sp,
+                                  const Tab32* tab, float* val, int* shift, int* pass,
+                                  hipStream_t st) {
     hipLaunchKernelGGL((k2_screen_records_kernel<M>), dim3((n_zeros + 255) / 256), dim3(256), 0,
                        st, zg, zgf, gamma, w, k, d, n_zeros, (ScreenRec<M>*)rec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st,
-                       (const ScreenRec<M>*)rec, cand, n_cand, n_stages, nz, thr, tab, val, shift,
+                       (const ScreenRec<M>*)rec, cand, n_cand, n_stages, sp, tab, val, shift,
                        pass);
     return hipGetLastError();
 }
 
 hipError_t launch_screen(int M, const uint4* zg, const float4* zgf, const double* gamma,
                          const double2* w, double k, double d, int n_zeros, void* rec,
-                         const uint4* cand, int n_cand, int n_stages, const int* nz,
-                         const float* thr, const Tab32* tab, float* val, int* shift, int* pass,
-                         hipStream_t st) {
+                         const uint4* cand, int n_cand, int n_stages, ScreenStages sp,
+                         const Tab32* tab, float* val, int* shift, int* pass, hipStream_t st) {
     switch (M) {
 #define K2_M(X) case X: return launch_screen_m<X>(zg, zgf, gamma, w, k, d, n_zeros, rec, cand, \
-                                                  n_cand, n_stages, nz, thr, tab, val, shift, \
-                                                  pass, st);
+                                                  n_cand, n_stages, sp, tab, val, shift, pass, \
+                                                  st);
         K2_M(0) K2_M(1) K2_M(2) K2_M(3) K2_M(4) K2_M(5) K2_M(6) K2_M(7) K2_M(8)
 #undef K2_M
         default: return hipErrorInvalidValue;
@@ -196,26 +201,20 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
     std::lock_guard<std::mutex> lk(t->mu);
     hipError_t e = hipSetDevice(t->device);
     hipStream_t st = (hipStream_t)stream;
-    // stage parameters (256 B) and the zero records travel in a device buffer owned by the table
-    const size_t need = 2 * K2_SCREEN_STAGES * 4;
+    // the zero records live in a device buffer owned by the table; the stage parameters travel
+    // as kernel arguments, so the call stays asynchronous
     if (e == hipSuccess && t->screen_params == nullptr)
-        e = hipMalloc((void**)&t->screen_params, 256 + K2_SCREEN_MAX * K2_SCREEN_REC_MAX);
-    int32_t hostp[2 * K2_SCREEN_STAGES] = {0};
+        e = hipMalloc((void**)&t->screen_params, K2_SCREEN_MAX * K2_SCREEN_REC_MAX);
+    ScreenStages sp{};
     for (int s = 0; s < n_stages; ++s) {
-        hostp[s] = stage_zeros[s];
-        float f = stage_thresholds[s];
-        memcpy(&hostp[K2_SCREEN_STAGES + s], &f, 4);
+        sp.nz[s] = stage_zeros[s];
+        sp.thr[s] = stage_thresholds[s];
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(t->screen_params, hostp, need, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess)
         e = launch_screen(n_shift, t->g, t->gf, t->gamma, (const double2*)t->w, k, shift_step,
-                          stage_zeros[n_stages - 1], (char*)t->screen_params + 256,
-                          (const uint4*)V_dev, (int)n_cand, n_stages,
-                          (const int*)t->screen_params,
-                          (const float*)(t->screen_params + K2_SCREEN_STAGES), t->tab32, val_dev,
-                          best_shift_dev, pass_dev, st);
+                          stage_zeros[n_stages - 1], t->screen_params, (const uint4*)V_dev,
+                          (int)n_cand, n_stages, sp, t->tab32, val_dev, best_shift_dev, pass_dev,
+                          st);
     if (e != hipSuccess)
         return set_error(K2_E_HIP, std::string("k2_screen_device: ") + hipGetErrorString(e));
     return K2_OK;

@@ -60,6 +60,28 @@ def test_screen_staging_and_survivors(setup):
     np.testing.assert_allclose(np.abs(K.cpu().numpy()), val[idx.cpu().numpy(), 2], atol=2e-5)
 
 
+def test_screen_calls_queue_without_host_sync(setup):
+    """Two screens with different k, shifts and stages queued on one stream (the call does not
+    synchronise; the second rewrites the table's zero records after the first has read them)
+    give what each gives alone."""
+    tb, t, torch = setup
+    Vt = ev.v_tensor(synth.make_candidates(2048, 3.4e18, 3.6e18, 23))
+    a = (3e-9, 0.0014, 3, ((64, 0.2), (128, 0.3), (256, 0.4)))
+    b = (1.5e-10, 0.002, 5, ((100, 0.25), (400, 0.35)))
+    alone = []
+    for p in (a, b):
+        r = t.screen_device(Vt, *p)
+        torch.cuda.synchronize()
+        alone.append([x.clone() for x in r])
+    ra = t.screen_device(Vt, *a)
+    rb = t.screen_device(Vt, *b)
+    torch.cuda.synchronize()
+    for got, want in zip((ra, rb), alone):
+        for x, y in zip(got, want):
+            assert torch.equal(x, y)
+    assert not torch.equal(ra[0], rb[0])
+
+
 def test_grid_then_newton_matches_reference_polish(setup, oracle_lib):
     """Canonical grid + Newton vs the reference's polish_peaks_with_weights on A/B rows of the
     same table (reference-format words, dy0 = 0)."""
