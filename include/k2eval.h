@@ -17,7 +17,9 @@
  * of the call.  Calls on one table handle are serialised by the caller; distinct handles may be
  * used from distinct threads.  *_device entry points take device pointers and a hipStream_t
  * (as void*; NULL = the default stream) and are asynchronous with respect to the host, except
- * the first use of a new k, which computes that k's weight arrays and waits for them.
+ * the first use of a new k, which computes that k's weight arrays and waits for them,
+ * k2_polish_device, which reads the still-moving candidates back between Newton steps, and
+ * k2_eval_batched_device, which waits for its one-call plan before freeing it.
  *
  * Reference interfaces replaced (reference repository bzyzz/mertens_bound):
  *   search/path1_eval_fast.cpp:15-31   compute_peak_AB            (same signature, below)
