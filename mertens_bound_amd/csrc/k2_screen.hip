@@ -11,7 +11,7 @@
 // Re(q e^{+-i m g d}) is two FMAs each.
 //
 // One lane per candidate.  The zero records of all stages (<= K2_SCREEN_MAX: G and its limbs as
-// floats, the FP32 weight 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once
+// scaled floats, the FP32 weight 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once
 // per call by k2_screen_records_kernel; every lane reads the same record at the same time, so the
 // screen kernel takes them through the scalar cache, and LDS holds only the 16 KiB sin/cos table.
 // A lane that fails a stage stops accumulating; its later stage values stay -1.  The stage zero
@@ -38,13 +38,15 @@ struct ScreenStages {
     float thr[K2_SCREEN_STAGES];
 };
 
-// one zero of the screen: g = (column 5's packed limbs pack_lo16(g2, g3), g1, g2, g3) as
-// phase_window_f reads them, gf = g0..g3 as floats, r[m-1] = e^{i m g d} (r padded to an odd
-// count: 16-byte records)
+// one zero of the screen: g = (column 5's packed limbs pack_lo16(g3, g2), g1, g2, g3) and
+// gvf = g0..g3 times 2^-34 as floats, the operands phase_window_f takes on its candidate side
+// (the window's columns are symmetric in G and V, so the uniform G takes the scalar operand
+// slots and the lane's V the other side); r[m-1] = e^{i m g d} (r padded to an odd count:
+// 16-byte records)
 template <int M>
 struct ScreenRec {
     uint4 g;
-    float4 gf;
+    float4 gvf;
     float2 a;     // 2 e^{-k g^2} w (FP32)
     float2 r[M | 1];
 };
@@ -53,9 +55,8 @@ struct ScreenRec {
 // the zero records of one call, one thread per zero
 template <int M>
 __global__ __launch_bounds__(256) void k2_screen_records_kernel(
-    const uint4* __restrict__ zg, const float4* __restrict__ zgf, const double* __restrict__ gamma,
-    const double2* __restrict__ w, double k, double d, int n_zeros,
-    ScreenRec<M>* __restrict__ rec) {
+    const uint4* __restrict__ zg, const double* __restrict__ gamma, const double2* __restrict__ w,
+    double k, double d, int n_zeros, ScreenRec<M>* __restrict__ rec) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n_zeros) return;
     const double g = gamma[j];
@@ -63,8 +64,8 @@ __global__ __launch_bounds__(256) void k2_screen_records_kernel(
     const double2 ww = w[j];
     ScreenRec<M> z;
     const uint4 gg = zg[j];
-    z.g = make_uint4(pack_lo16(gg.z, gg.w), gg.y, gg.z, gg.w);
-    z.gf = zgf[j];
+    z.g = make_uint4(pack_lo16(gg.w, gg.z), gg.y, gg.z, gg.w);
+    z.gvf = window_vf(Limbs4{gg.x, gg.y, gg.z, gg.w});
     z.a = make_float2((float)(f * ww.x), (float)(f * ww.y));
 #pragma unroll
     for (int m = 0; m < (M | 1); ++m) {
@@ -89,8 +90,9 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
     if (c >= n_cand) return;
     const uint4 cv = cand[c];
     const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
-    const float4 vf = window_vf(v);
-    const uint32_t vp = pack_lo16(v.x3, v.x2);
+    // the lane's side of the window: V's limbs as floats (unscaled) and column 5's packed limbs
+    const float4 vf = make_float4((float)v.x0, (float)v.x1, (float)v.x2, (float)v.x3);
+    const uint32_t vp = pack_lo16(v.x2, v.x3);
     float acc[2 * M + 1];
 #pragma unroll
     for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
@@ -101,8 +103,8 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
             const ScreenRec<M> z = rec[j];   // the same j in every lane
             uint32_t hi, lo;
             float t3;
-            phase_window_f<K2_TAB32_BITS>(Limbs4{0u, z.g.y, z.g.z, z.g.w}, z.g.x, z.gf, v, vp, vf,
-                                          hi, lo, t3);
+            phase_window_f<K2_TAB32_BITS, true>(v, vp, vf, Limbs4{0u, z.g.y, z.g.z, z.g.w},
+                                                z.g.x, z.gvf, hi, lo, t3);
             float S, C;
             sincos_window_f<K2_TAB32_BITS>(hi, lo, t3, s_tab, S, C);
             // q = a e^{i th}
@@ -141,13 +143,13 @@ namespace k2impl {
 // records of n_zeros zeros into rec (room for K2_SCREEN_MAX records of ScreenRec<M>), then the
 // screen
 template <int M>
-static hipError_t launch_screen_m(const uint4* zg, const float4* zgf, const double* gamma,
+static hipError_t launch_screen_m(const uint4* zg, const double* gamma,
                                   const double2* w, double k, double d, int n_zeros, void* rec,
                                   const uint4* cand, int n_cand, int n_stages, ScreenStages sp,
                                   const Tab32* tab, float* val, int* shift, int* pass,
                                   hipStream_t st) {
     hipLaunchKernelGGL((k2_screen_records_kernel<M>), dim3((n_zeros + 255) / 256), dim3(256), 0,
-                       st, zg, zgf, gamma, w, k, d, n_zeros, (ScreenRec<M>*)rec);
+                       st, zg, gamma, w, k, d, n_zeros, (ScreenRec<M>*)rec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st,
@@ -156,12 +158,12 @@ static hipError_t launch_screen_m(const uint4* zg, const float4* zgf, const doub
     return hipGetLastError();
 }
 
-hipError_t launch_screen(int M, const uint4* zg, const float4* zgf, const double* gamma,
+hipError_t launch_screen(int M, const uint4* zg, const double* gamma,
                          const double2* w, double k, double d, int n_zeros, void* rec,
                          const uint4* cand, int n_cand, int n_stages, ScreenStages sp,
                          const Tab32* tab, float* val, int* shift, int* pass, hipStream_t st) {
     switch (M) {
-#define K2_M(X) case X: return launch_screen_m<X>(zg, zgf, gamma, w, k, d, n_zeros, rec, cand, \
+#define K2_M(X) case X: return launch_screen_m<X>(zg, gamma, w, k, d, n_zeros, rec, cand, \
                                                   n_cand, n_stages, sp, tab, val, shift, pass, \
                                                   st);
         K2_M(0) K2_M(1) K2_M(2) K2_M(3) K2_M(4) K2_M(5) K2_M(6) K2_M(7) K2_M(8)
@@ -211,7 +213,7 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
         sp.thr[s] = stage_thresholds[s];
     }
     if (e == hipSuccess)
-        e = launch_screen(n_shift, t->g, t->gf, t->gamma, (const double2*)t->w, k, shift_step,
+        e = launch_screen(n_shift, t->g, t->gamma, (const double2*)t->w, k, shift_step,
                           stage_zeros[n_stages - 1], t->screen_params, (const uint4*)V_dev,
                           (int)n_cand, n_stages, sp, t->tab32, val_dev, best_shift_dev, pass_dev,
                           st);
