@@ -9,9 +9,11 @@
   scanned in order of m), Newton on all zeros from the best grid point, keep the larger of |h0| and
   |h_final|.  Here every point is an exact 64.64 word and each Newton step is a fresh exact
   evaluation instead of a rotation of precomputed A/B rows.
-* :func:`cascade` -- the hierarchical grid polish of search/gpu_mertens_stream.cu:381-505: stages
-  of (zeros, half-steps, step, threshold, centred on the previous best?); a candidate stops at the
-  first stage whose best |h| is below its threshold.
+* :func:`cascade` -- the hierarchical grid polish ``ZeroData::polish`` of
+  search/gpu_mertens_stream.cu:381-505: five grid stages (256/512/2,048 zeros with thresholds
+  0.895/0.918/0.948, then 641 points of 2.5e-6 on the first 15,000 zeros and 241 points of 5e-7
+  on the whole table); a candidate stops at the first thresholded stage whose best |h| is below
+  its threshold.
 * :func:`polish_single_peak` -- search/path1_kernel_opt.py:94-147: a grid on all zeros, Newton
   from 0 and from the best grid point, the largest |h| kept.
 * :func:`fast_polish_batch` -- search/path1_kernel_opt.py:150-221: a 121-point grid (evaluated
@@ -30,9 +32,15 @@ from . import autograd as ag
 from . import evaluator as ev
 
 REFERENCE_SCREEN = ((64, 0.78), (128, 0.855), (256, 0.892))
-# (zeros, half-steps M, step, threshold, centred on the previous stage's best)
-REFERENCE_CASCADE = ((256, 38, 1.5e-4, 0.895, False), (512, 228, 2.5e-5, 0.918, False),
-                     (2048, 60, 1.0e-5, 0.948, True))
+# Stages of ZeroData::polish (search/gpu_mertens_stream.cu:21-29,381-505):
+# (zeros (0: the whole table), half-steps M, step, threshold (None: none), centred on the previous
+#  stage's best offset, sets the returned h).  The 641-point stage on the first 15,000 zeros moves
+# the point only; the 241-point full-table stage sets both the point and h.
+REFERENCE_CASCADE = ((256, 38, 1.5e-4, 0.895, False, True),
+                     (512, 228, 2.5e-5, 0.918, False, True),
+                     (2048, 60, 1.0e-5, 0.948, True, True),
+                     (15000, 320, 2.5e-6, None, True, False),
+                     (0, 120, 5e-7, None, True, True))
 
 
 def shifted(V: torch.Tensor, offsets: torch.Tensor) -> torch.Tensor:
@@ -95,32 +103,43 @@ def _delta(V0: torch.Tensor, V1: torch.Tensor) -> torch.Tensor:
 
 
 def cascade(table: ev.K2Table, V: torch.Tensor, k: float, stages=REFERENCE_CASCADE):
-    """Hierarchical grid scan.  Returns (best |h| of the last stage reached, signed best h,
-    offset of the best point, number of stages passed) per candidate."""
+    """Hierarchical grid polish (search/gpu_mertens_stream.cu:381-505) for a batch of candidates.
+    Stage s evaluates K2 over its first zeros at the points d = c + m step, |m| <= M (c = the
+    previous stage's best offset when centred, else 0; d in FP64 as the reference forms dy),
+    each the exact word V + rint(d 2^64); the first point of largest |h| in scan order becomes
+    the best (when |h| > 0, as the reference's strict comparison from 0).  A thresholded stage
+    whose best |h| is below its threshold ends the candidate there.  Stages that do not set h
+    move the point only.  Returns (|h|, h, offset d of the returned point, thresholded stages
+    passed) per candidate."""
     n = V.shape[0]
     dev = V.device
-    center = torch.zeros(n, dtype=torch.float64, device=dev)
     best_h = torch.zeros(n, dtype=torch.float64, device=dev)
     best_d = torch.zeros(n, dtype=torch.float64, device=dev)
     passed = torch.zeros(n, dtype=torch.int32, device=dev)
     alive = torch.arange(n, device=dev)
-    for s, (nz, M, step, thr, centred) in enumerate(stages):
+    for nz, M, step, thr, centred, sets_h in stages:
         if alive.numel() == 0:
             break
+        na = alive.numel()
         m = torch.arange(-M, M + 1, device=dev, dtype=torch.float64) * step
-        base = ag.offset_points(V[alive], center[alive]) if centred else V[alive]
-        Vg = shifted(base, m)
-        h, _, _ = table.eval_device(Vg, k, n_zeros=min(nz, table.n), flags=ev.VALUE)
-        h = h.view(alive.numel(), 2 * M + 1)
+        d = m[None, :].expand(na, -1)
+        if centred:
+            d = best_d[alive][:, None] + m[None, :]
+        Vg = ag.offset_points(V[alive].repeat_interleave(2 * M + 1, dim=0), d.reshape(-1))
+        nzs = table.n if nz == 0 else min(nz, table.n)
+        h, _, _ = table.eval_device(Vg, k, n_zeros=nzs, flags=ev.VALUE)
+        h = h.view(na, 2 * M + 1)
         j = torch.argmax(h.abs(), dim=1)          # first maximum in scan order
         hb = h.gather(1, j[:, None]).flatten()
-        db = m[j] + (center[alive] if centred else 0.0)
-        best_h[alive] = hb
-        best_d[alive] = db
-        center[alive] = db
-        ok = hb.abs() >= thr
-        passed[alive[ok]] = s + 1
-        alive = alive[ok]
+        db = d.gather(1, j[:, None]).flatten()
+        upd = hb.abs() > 0
+        best_d[alive] = torch.where(upd, db, best_d[alive])
+        if sets_h:
+            best_h[alive] = torch.where(upd, hb, best_h[alive])
+        if thr is not None:
+            ok = hb.abs() >= thr
+            passed[alive[ok]] += 1
+            alive = alive[ok]
     return best_h.abs(), best_h, best_d, passed
 
 

@@ -7,6 +7,8 @@ mertens_bound_amd.pipeline; see this package's __init__ for who may import it).
                               instead of through the degree-16 Taylor moments: the moment
                               truncation itself is parity unpinned)
 * :func:`scan`                search/polish.py:51-73
+* :func:`cascade`             search/gpu_mertens_stream.cu:381-505 (ZeroData::polish, five grid
+                              stages; stage constants at :21-29 and :474-503)
 
 Every point is the exact 64.64 word V + rint(offset * 2^64); offsets accumulate in FP64 like the
 reference's ddy.  Pinned by closed forms on a one-zero table (tests/test_oracle.py), where the
@@ -114,3 +116,43 @@ def scan(table, V0, k, half_width=0.25, step=2e-5, n_zeros=None, chunk=4000):
         best_m = np.where(better, m[j], best_m)
         have[:] = True
     return best_h, best_m
+
+
+# ZeroData::polish stages (search/gpu_mertens_stream.cu:21-29,474-503): zeros (0 = all), M, step,
+# threshold (None = none), centred on the previous best, sets the returned h
+CASCADE_STAGES = ((256, 38, 1.5e-4, 0.895, False, True), (512, 228, 2.5e-5, 0.918, False, True),
+                  (2048, 60, 1.0e-5, 0.948, True, True), (15000, 320, 2.5e-6, None, True, False),
+                  (0, 120, 5e-7, None, True, True))
+
+
+def cascade(table, V, k, stages=CASCADE_STAGES):
+    """ZeroData::polish per candidate, as the reference writes it: in each stage a running best
+    |s| starting from 0 is replaced by every strictly larger grid value in scan order
+    (:398-409, :425-433, :451-459, :484-490, :494-502); best_dy = center + m step in FP64; a
+    thresholded stage with best below its threshold returns (best_h, best_dy) (:410, :435, :461).
+    Every grid value is K2 over the stage's zeros at the exact word V + rint(dy 2^64).
+    Returns (h, dy, thresholded stages passed) arrays."""
+    n = V.shape[0]
+    H = np.zeros(n); D = np.zeros(n); P = np.zeros(n, dtype=np.int32)
+    for c in range(n):
+        best_h = 0.0
+        best_dy = 0.0
+        for nz, M, step, thr, centred, sets_h in stages:
+            nzs = table.n if nz == 0 else min(nz, table.n)
+            center = best_dy if centred else 0.0
+            dys = np.array([center + m * step for m in range(-M, M + 1)])
+            vals = _value(table, offset(np.repeat(V[c:c + 1], dys.size, axis=0), dys), k, nzs)
+            best_abs = 0.0
+            for dy, s in zip(dys, vals):
+                if abs(s) > best_abs:
+                    best_abs = abs(s)
+                    best_dy = float(dy)
+                    if sets_h:
+                        best_h = float(s)
+            if thr is not None:
+                if best_abs < thr:
+                    break
+                P[c] += 1
+        H[c] = best_h
+        D[c] = best_dy
+    return H, D, P

@@ -102,12 +102,43 @@ def test_grid_then_newton_matches_reference_polish(setup, oracle_lib):
     np.testing.assert_allclose(ddy.cpu().numpy(), dd_r, atol=1e-9)
 
 
-def test_cascade_stages(setup, oracle_lib):
+def test_cascade_reference_stages_vs_restatement(oracle_lib):
+    """The five-stage ZeroData::polish with the reference's stage table (REFERENCE_CASCADE)
+    against the oracle's restatement: same stages passed, the same grid point (offset) and h to
+    1e-12.  A 20,000-zero table with weights scaled 2.5x, so that the 40 largest of 20,000
+    random candidates on the first 256 zeros spread over every stage (plus 24 random ones, which
+    stop at the first), including the 641-point grid on the first 15,000 zeros and the
+    241-point grid over the whole table."""
+    import torch
+    from oracle import pipelines as P
+    tb0 = synth.make_zero_table(20_000, 21)
+    tb = synth.ZeroTable(tb0.G, tb0.gamma, tb0.w * 2.5)
+    k = 3e-9
+    pool = synth.make_candidates(20_000, 3.4e18, 3.6e18, 24)
+    K256 = oracle_lib.eval_f64(tb, pool, k, flags=1, n_zeros=256)[0]
+    top = np.argsort(-np.abs(K256), kind="stable")[:40]
+    V = np.concatenate([pool[top], pool[-24:]])
+    assert pipeline.REFERENCE_CASCADE == P.CASCADE_STAGES
+    t = ev.K2Table(tb)
+    babs, bh, bd, passed = pipeline.cascade(t, ev.v_tensor(V), k)
+    torch.cuda.synchronize()
+    H, D, Pa = P.cascade(tb, V, k)
+    passed = passed.cpu().numpy(); bd = bd.cpu().numpy(); bh = bh.cpu().numpy()
+    print("stages passed:", np.bincount(Pa, minlength=4))
+    assert np.bincount(Pa, minlength=4).min() >= 1        # every outcome is exercised
+    np.testing.assert_array_equal(passed, Pa)
+    np.testing.assert_array_equal(bd, D)
+    np.testing.assert_allclose(bh, H, rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(babs.cpu().numpy(), np.abs(bh))
+
+
+def test_cascade_custom_stages(setup, oracle_lib):
+    """Custom stage tables: h at the returned offset over the last stage's zeros."""
     tb, t, torch = setup
     k = 3e-9
     V = synth.make_candidates(64, 3.4e18, 3.6e18, 24)
-    stages = ((256, 10, 1.5e-4, 0.2, False), (512, 20, 2.5e-5, 0.3, False),
-              (2048, 8, 1e-5, 0.4, True))
+    stages = ((256, 10, 1.5e-4, 0.2, False, True), (512, 20, 2.5e-5, 0.3, False, True),
+              (2048, 8, 1e-5, 0.4, True, True))
     babs, bh, bd, passed = pipeline.cascade(t, ev.v_tensor(V), k, stages)
     bd = bd.cpu().numpy(); passed = passed.cpu().numpy(); bh = bh.cpu().numpy()
     for i in range(0, 64, 9):

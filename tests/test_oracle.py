@@ -244,3 +244,67 @@ def test_pipeline_restatements_reach_the_one_zero_peak(oracle_lib):
     far = synth.v_from_int([int(mpmath.nint((vstar + mpmath.mpf("4e-4")) * 2 ** 64))])
     d_far, _ = P.newton(tb, far, k, 1, 8, 2e-5, 1e-14)
     assert abs(d_far[0] + 8 * 2e-5) < 1e-15
+
+
+def _one_zero_table(amp, k):
+    """One zero (gamma ~ 1000.3) with |w| chosen so that 2 e^{-k g^2} |w| = amp, arg w = 0.4."""
+    import mpmath
+    from mertens_bound_amd import synth
+    G = int(round(1000.3 / (2 * np.pi) * 2.0 ** 110))
+    with mpmath.workdps(50):
+        g = 2 * mpmath.pi * mpmath.mpf(G) / mpmath.mpf(2) ** 110
+    r = amp / (2 * np.exp(-k * float(g) ** 2))
+    tb = synth.ZeroTable(np.array([[G & (2 ** 64 - 1), G >> 64]], dtype=np.uint64),
+                         np.array([float(g)]), np.array([r * np.exp(0.4j)]))
+    return tb, g
+
+
+def _nearest_peak_cascade(g, amp, v0, stages):
+    """Closed form of ZeroData::polish on one zero: K2(v) = amp cos(g v + 0.4), so in every grid
+    stage the first point of largest |K2| is the first point whose phase g v + 0.4 lies nearest
+    a multiple of pi (computed in mpmath at the exact word of each point)."""
+    import mpmath
+    best_h, best_dy, passed = 0.0, 0.0, 0
+    with mpmath.workdps(50):
+        for nz, M, step, thr, centred, sets_h in stages:
+            center = best_dy if centred else 0.0
+            best = None
+            for m in range(-M, M + 1):
+                dy = center + m * step
+                v = mpmath.mpf(v0 + int(round(dy * 2.0 ** 64))) / mpmath.mpf(2) ** 64
+                t = (g * v + mpmath.mpf("0.4")) / mpmath.pi
+                dist = abs(t - mpmath.nint(t))
+                if best is None or dist < best[0]:
+                    best = (dist, dy, amp * mpmath.cos(mpmath.pi * t))
+            best_dy = best[1]
+            if sets_h:
+                best_h = float(best[2])
+            if thr is not None:
+                if abs(float(best[2])) < thr:
+                    break
+                passed += 1
+    return best_h, best_dy, passed
+
+
+def test_cascade_restatement_on_one_zero(oracle_lib):
+    """The oracle's five-stage ZeroData::polish (reference stage table) against the closed form:
+    amplitudes 0.99 (every stage passes: the full-table stage sets the returned point and h),
+    0.93 (stops at the 2,048-zero threshold 0.948), 0.9 (stops at 0.918) and 0.5 (stops at the
+    first stage's 0.895)."""
+    from oracle import pipelines as P
+    from mertens_bound_amd import synth
+    k = 3e-9
+    rng = np.random.default_rng(5)
+    for amp, want_passed in ((0.99, 3), (0.93, 2), (0.9, 1), (0.5, 0)):
+        tb, g = _one_zero_table(amp, k)
+        v0s = [int(x) for x in rng.integers(1 << 60, 1 << 61, size=2, dtype=np.uint64)]
+        V = synth.v_from_int(v0s)
+        H, D, Pa = P.cascade(tb, V, k)
+        for i, v0 in enumerate(v0s):
+            h, dy, passed = _nearest_peak_cascade(g, amp, v0, P.CASCADE_STAGES)
+            assert Pa[i] == passed == want_passed, (amp, Pa[i], passed)
+            assert D[i] == dy, (amp, D[i], dy)
+            assert abs(H[i] - h) <= 1e-15
+        if want_passed == 3:
+            # the returned point lies within half a final grid step (2.5e-7) of a peak
+            assert np.all(np.abs(np.abs(H) - amp) <= amp * (1 - np.cos(float(g) * 2.5e-7)) + 1e-15)
