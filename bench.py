@@ -25,47 +25,157 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic VALU work per candidate-zero evaluation (lane operations), K2 + K2':
-#   plain: 15 INT32 multiply-adds for the exact phase window + 21 FP64 FMA-equivalents
-#          (42 flops: sin/cos of the reduced phase, weighting, two accumulations)
-#   mixed: 15 INT32 multiply-adds + 2 transcendental-equivalents (counted as 2 slots each,
-#          half rate) + 4 FP32 FMA-equivalents (8 flops)
-#   enclose: the plain work + 2 x 6 FP64 additions (TwoSum error-free accumulation of K2, K2')
-# Work per candidate-zero evaluation in full-rate VALU slots (one wave64 instruction per 4 cycles
-# per SIMD).  FP32 instructions issue in half that (2 cycles per wave64 on gfx950; the FP32 tail
-# issues 130% of the full-rate instruction rate, profiles/k2_eval_mixed_65536x2325572_trace_counters.txt),
-# so an FP32 FMA is half a slot.  Mixed: the sin and cos of the reduced angle count as the three
-# FMAs of a table rotation each, plus four FMAs for the weights: 10 FP32 FMAs = 5 slots.
-WORK = {"plain": {"int32_mad": 15, "fp64_fma": 21, "slots": 36},
-        # the precision form's FP32 tail as built: 3 multiply-adds (column 4), the packed dot
-        # product (column 5), the funnel shift and the conversion of the remainder; column 3 in
-        # 4 FP32 FMAs, x and x/2 in 3, the rotation in 4, the sums in 4 (half a slot each at the
-        # published FP32 peak); its FP64 head is charged at the plain form's 36 slots
-        "mixed": {"int32_ops": 6, "fp32_ops": 15, "fp32_slot": 0.5, "slots": 13.5},
-        "enclose": {"int32_mad": 15, "fp64_fma": 21, "fp64_add": 12, "slots": 48},
-        # the leaf screen's loop per candidate and zero (7 shifted terms), as compiled: 7
-        # integer instructions (the precision form's window: 3 multiply-adds, the packed dot
-        # product, the remainder's funnel shift, mask and conversion), 28 FP32 (column 3 4, table
-        # sin/cos 7, the weight rotation 4, the 7 shifted sums 13) at half a slot
-        "screen": {"int32_ops": 7, "fp32_ops": 28, "fp32_slot": 0.5, "slots": 21.0}}
-# Full-rate VALU lane-operation peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s, which is the
-# published 78.6 TFLOP/s FP64 peak counted per FMA (2 flops).  The roofline reports the work in
-# FP64-FMA-equivalent flops (2 per slot) against that published peak.
-VALU_PEAK_TOPS = 39.3
-FP64_PEAK_TFLOPS = 78.6
+# Algorithmic work per candidate-zero evaluation, by instruction class (the frozen per-evaluation
+# constants of the path's roofline; flops count an FMA as 2):
+#   plain K2 + K2':        42 FP64 flops (sin/cos of the reduced phase, weighting, two sums)
+#                          + 15 INT32 multiply-adds (the exact 128 x 128-bit phase window)
+#   plain K2 + K2' + K2'': 46 FP64 flops + 15 INT32 multiply-adds
+#   precision form:        2 transcendentals (sin, cos) + 8 FP32 flops (weight, compensated sum)
+#                          + 15 INT32 multiply-adds per FP32-tail evaluation; its FP64 head at the
+#                          plain constants
+#   enclosure K2 + K2':    the plain work + 12 FP64 additions (TwoSum accumulation of K2, K2')
+#   leaf screen (one zero, 7 shifted sums): 15 INT32 multiply-adds + 2 transcendentals + 31 FP32
+#                          flops (weight rotation 6, unshifted sum 1, three shift pairs 4 FMAs each)
+# roofline.frac is the largest class fraction (achieved class rate / class peak).  Class peaks:
+# FP64 78.6 TFLOP/s and FP32 157.3 TFLOP/s (MI355X published); the INT32 multiply-add
+# (v_mad_u64_u32) and transcendental (v_sin_f32) rates are measured on the same GPU in the same
+# run by the library's microbenchmark (k2_valu_rates).
+ALG = {"plain": {"fp64_flops": 42, "int32_mad": 15},
+       "plain_d2": {"fp64_flops": 46, "int32_mad": 15},
+       "mixed": {"trans": 2, "fp32_flops": 8, "int32_mad": 15},
+       "enclose": {"fp64_flops": 54, "int32_mad": 15},
+       "screen": {"int32_mad": 15, "trans": 2, "fp32_flops": 31}}
+PUBLISHED_PEAKS = {"fp64_flops": 78.6e12, "fp32_flops": 157.3e12}
+# measured once on gfx950 (k2_valu_rates, lane operations/s); used only where the live
+# measurement fails
+RECORDED_RATES = {"int32_mad": 33.0e12, "trans": 9.8e12}
+# As-executed VALU issue model (not the roofline): full-rate slot-equivalents per evaluation,
+# one wave64 instruction per 4 cycles per SIMD = 39.3 T lane-slots/s on 256 CUs at 2.4 GHz;
+# FP32 instructions issue in half a slot.  plain: 15 INT32 multiply-adds + 21 FP64 FMAs; mixed
+# FP32 tail as built: 6 integer instructions + 15 FP32 at half a slot; enclosure: + 12 FP64
+# additions; screen: 7 integer + 28 FP32 instructions per zero.
+SLOTS = {"plain": 36.0, "plain_d2": 38.0, "mixed": 13.5, "enclose": 48.0, "screen": 21.0}
+VALU_SLOT_PEAK = 39.3e12
+# SQ_INSTS_VALU per evaluation from the committed counter summaries (wave instructions x 64 /
+# evaluations)
+VALU_INSTS = {"plain": 41.7, "mixed": 22.7, "screen": 35.0}
+_RATES = {}
+
+
+def class_peaks():
+    """Published FP64/FP32 peaks and the INT32 multiply-add / transcendental rates measured on
+    this GPU now (recorded values if the measurement fails)."""
+    if not _RATES:
+        peaks = dict(PUBLISHED_PEAKS)
+        src = "measured (k2_valu_rates)"
+        try:
+            from mertens_bound_amd import evaluator as ev
+            r = ev.valu_rates()
+            peaks["int32_mad"] = r["int32_mad"]
+            peaks["trans"] = r["trans"]
+            peaks["measured_fp64_fma_lane_ops"] = r["fp64_fma"]
+            peaks["measured_fp32_fma_lane_ops"] = r["fp32_fma"]
+        except Exception as exc:   # reported in the line
+            peaks.update(RECORDED_RATES)
+            src = f"recorded (measurement failed: {exc!r})"
+        peaks["source_int32_trans"] = src
+        _RATES.update(peaks)
+    return _RATES
+
+
+def roofline(form, evals, kern_ms, alg_bytes, workload, head_frac=0.0):
+    """Roofline of the dominant kernel: per-class achieved rates against the class peaks,
+    frac = the largest.  head_frac: for the mixed form, the share of the evaluations in its
+    FP64 head (charged at the plain constants)."""
+    rate = evals / (kern_ms * 1e-3)
+    peaks = class_peaks()
+    if form == "mixed":
+        work = {"fp64_flops": head_frac * ALG["plain"]["fp64_flops"],
+                "int32_mad": ALG["mixed"]["int32_mad"],
+                "trans": (1.0 - head_frac) * ALG["mixed"]["trans"],
+                "fp32_flops": (1.0 - head_frac) * ALG["mixed"]["fp32_flops"]}
+    else:
+        work = dict(ALG[form])
+    units = {"fp64_flops": "TFLOP/s", "fp32_flops": "TFLOP/s", "int32_mad": "T MAD/s",
+             "trans": "T op/s"}
+    classes = {}
+    for cl, per in work.items():
+        ach = rate * per
+        classes[cl] = {"per_eval": per, "achieved": ach / 1e12, "peak": peaks[cl] / 1e12,
+                       "unit": units[cl], "frac": ach / peaks[cl]}
+    bind = max(classes, key=lambda c: classes[c]["frac"])
+    b = classes[bind]
+    slots = SLOTS[form] if form != "mixed" else (head_frac * SLOTS["plain"]
+                                                 + (1.0 - head_frac) * SLOTS["mixed"])
+    r = {"bound": "valu", "binding_class": bind, "achieved": b["achieved"], "peak": b["peak"],
+         "unit": b["unit"], "frac": b["frac"], "traffic": traffic_summary(workload),
+         "classes": classes,
+         "peaks_source": {"fp64_flops": "published", "fp32_flops": "published",
+                          "int32_mad": peaks["source_int32_trans"],
+                          "trans": peaks["source_int32_trans"]},
+         "kernel_ms": kern_ms, "kernel_evals_per_s": rate,
+         "algorithmic_bytes": alg_bytes,
+         "hbm_GBps_algorithmic": alg_bytes / (kern_ms * 1e-3) / 1e9,
+         "valu_issue": {"slots_per_eval_as_built": slots,
+                        "valu_issue_frac": rate * slots / VALU_SLOT_PEAK,
+                        "sq_insts_valu_per_eval": VALU_INSTS.get(form),
+                        "note": "issue model, not the roofline: full-rate VALU slots per "
+                                "evaluation as compiled against 39.3 T slots/s"}}
+    for k in ("measured_fp64_fma_lane_ops", "measured_fp32_fma_lane_ops"):
+        if k in peaks:
+            r.setdefault("measured_rates", {})[k] = peaks[k]
+    return r
 
 
 def cpu_baseline(tb, V, k, budget_s=15.0, n_c=256, chunk=1 << 17):
     """The reference evaluator (oracle/_ref, compiled from search/path1_eval_fast.cpp; 16 OpenMP
     threads, hard-coded there) on a bounded sample: batches of n_c candidates against the whole
     table until about budget_s seconds have run -- compute_peak_AB over zero chunks plus the
-    weighted sums K = sum e^{-k g^2} A and K' = -sum g e^{-k g^2} B.  Falls back to the oracle's
-    fused FP64 port where the reference library was not built."""
+    weighted sums K = sum e^{-k g^2} A and K' = -sum g e^{-k g^2} B.  Beside it: the oracle's
+    fused FP64 port of the same arithmetic on n_c candidates x the whole table at all physical
+    cores the process may use (lscpu, capped at its CPU share) and at 16 threads, and both
+    evaluators on the 1,024 x 10,000 workload in full.  Falls back to the port as the headline
+    where the reference library was not built."""
     import numpy as np
     import oracle
-    from mertens_bound_amd import formats
+    from mertens_bound_amd import formats, synth
+    info = cpu_info()
+    threads = info.get("threads_usable") or 16
+    out = {"host": info}
     L = oracle.load_ref()
-    Vs = V[:n_c]
+    runs = []
+    # the fused port (libm sin/cos, exact phase, fused sums), n_c candidates x all zeros
+    for nt in sorted({threads, 16}):
+        t0 = time.perf_counter()
+        oracle.eval_f64(tb, V[:n_c], k, flags=oracle.VALUE | oracle.D1, nthreads=nt)
+        dt = time.perf_counter() - t0
+        runs.append({"kind": "port", "threads": nt, "workload": f"{n_c} x {tb.n}",
+                     "value": n_c * tb.n / dt, "s": dt})
+    # the 1,024 x 10,000 workload in full
+    c1 = synth.W1024x10000
+    tb1 = synth.make_zero_table(c1["n_zeros"], c1["seed"])
+    V1 = synth.make_candidates(c1["n_cand"], c1["v_lo"], c1["v_hi"], c1["seed"])
+    for nt in sorted({threads, 16}):
+        t0 = time.perf_counter()
+        oracle.eval_f64(tb1, V1, c1["k"], flags=oracle.VALUE | oracle.D1, nthreads=nt)
+        dt = time.perf_counter() - t0
+        runs.append({"kind": "port", "threads": nt, "workload": f"{tb1.n} zeros x {V1.shape[0]}",
+                     "value": V1.shape[0] * tb1.n / dt, "s": dt})
+    if L is not None:
+        xs, dy = formats.candidates_to_x_dy(V1)
+        xlo, xhi = formats.x_words(xs)
+        rw = ref_words_fast(tb1)
+        wk = np.exp(-c1["k"] * tb1.gamma ** 2)
+        t0 = time.perf_counter()
+        A, B = oracle._call_cpab(L.compute_peak_AB, xlo, xhi, dy, rw["alpha"], rw["gamma"],
+                                 rw["ghi_lo"], rw["ghi_hi"], rw["glo_lo"], rw["glo_hi"],
+                                 rw["psi_lo"], rw["psi_hi"])
+        _ = A @ wk, B @ (tb1.gamma * wk)
+        dt = time.perf_counter() - t0
+        runs.append({"kind": "reference", "threads": 16,
+                     "workload": f"{tb1.n} zeros x {V1.shape[0]}",
+                     "value": V1.shape[0] * tb1.n / dt, "s": dt})
+    out["runs"] = runs
     if L is not None:
         batches = []                                 # candidate words: not timed
         for b0 in range(0, min(V.shape[0], 64 * n_c), n_c):
@@ -95,17 +205,18 @@ def cpu_baseline(tb, V, k, budget_s=15.0, n_c=256, chunk=1 << 17):
             if time.perf_counter() - t0 > budget_s:
                 break
         dt = time.perf_counter() - t0
-        return {"value": evals / dt, "unit": "candidate-zero evals/s", "cores": 16,
-                "kind": "reference",
-                "sample": f"compute_peak_AB (reference, 16 threads) + weighted sums: {done} "
-                          f"candidates x all {tb.n} zeros in zero chunks of {chunk}, {dt:.1f} s"}
-    nt = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    oracle.eval_f64(tb, Vs, k, flags=oracle.VALUE | oracle.D1, nthreads=nt)
-    dt = time.perf_counter() - t0
-    return {"value": n_c * tb.n / dt, "unit": "candidate-zero evals/s", "cores": nt,
-            "kind": "port", "sample": f"oracle FP64 port on {n_c} candidates x {tb.n} zeros, "
-                                      f"{dt:.1f} s"}
+        out.update({"value": evals / dt, "unit": "candidate-zero evals/s", "cores": 16,
+                    "kind": "reference",
+                    "sample": f"compute_peak_AB (reference, 16 threads) + weighted sums: {done} "
+                              f"candidates x all {tb.n} zeros in zero chunks of {chunk}, "
+                              f"{dt:.1f} s"})
+        return out
+    best = max((r for r in runs if r["workload"] == f"{n_c} x {tb.n}"), key=lambda r: r["value"])
+    out.update({"value": best["value"], "unit": "candidate-zero evals/s",
+                "cores": best["threads"], "kind": "port",
+                "sample": f"oracle FP64 port on {n_c} candidates x {tb.n} zeros, "
+                          f"{best['s']:.1f} s"})
+    return out
 
 
 def ref_words_fast(tb):
@@ -129,13 +240,36 @@ def ref_words_fast(tb):
             "glo_lo": glo_lo, "glo_hi": glo_hi, "psi_lo": psi_lo, "psi_hi": psi_hi}
 
 
+def cpu_share():
+    """CPUs this process may use: its affinity mask, capped by a cgroup CPU quota if one is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            n = min(n, max(1, int(int(q) // int(per))))
+    except Exception:
+        pass
+    return n
+
+
 def cpu_info():
+    """CPU model and physical core count (lscpu) and the threads this process can use."""
+    d = {}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        d = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
-        return {"model": d.get("Model name", "").strip(), "cpus": d.get("CPU(s)", "").strip()}
+        d = dict((l.split(":", 1)[0].strip(), l.split(":", 1)[1].strip())
+                 for l in out.splitlines() if ":" in l)
     except Exception:
-        return {}
+        pass
+    phys = None
+    try:
+        phys = int(d.get("Core(s) per socket", "0")) * int(d.get("Socket(s)", "0")) or None
+    except ValueError:
+        pass
+    share = cpu_share()
+    return {"model": d.get("Model name", ""), "logical_cpus": d.get("CPU(s)", ""),
+            "physical_cores": phys, "cpu_share": share,
+            "threads_usable": min(phys, share) if phys else share}
 
 
 def traffic_summary(workload):
@@ -147,27 +281,6 @@ def traffic_summary(workload):
         return d.get(workload, {}).get("hbm_bytes_per_launch")
     except Exception:
         return None
-
-
-def roofline(form, evals, kern_ms, alg_bytes, workload, head_frac=0.0):
-    """head_frac: for the mixed form, the share of the evaluations in its FP64 head"""
-    w = dict(WORK[form])
-    if form == "mixed":
-        w["fp64_head_fraction"] = head_frac
-        w["slots"] = head_frac * WORK["plain"]["slots"] + (1.0 - head_frac) * w["slots"]
-    rate = evals / (kern_ms * 1e-3)
-    achieved = rate * 2 * w["slots"] / 1e12
-    r = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_summary(workload),
-         "achieved_valu_slots_tops": rate * w["slots"] / 1e12, "valu_slot_peak_tops": VALU_PEAK_TOPS,
-         "kernel_ms": kern_ms, "kernel_evals_per_s": rate, "work_per_eval": w,
-         "int32_tops": rate * w.get("int32_mad", w.get("int32_ops", 0)) / 1e12,
-         "algorithmic_bytes": alg_bytes,
-         "hbm_GBps_algorithmic": alg_bytes / (kern_ms * 1e-3) / 1e9}
-    if form in ("plain", "enclose"):
-        r["fp64_tflops"] = rate * (2 * w["fp64_fma"] + w.get("fp64_add", 0)) / 1e12
-        r["fp64_frac"] = r["fp64_tflops"] / FP64_PEAK_TFLOPS
-    return r
 
 
 def timed(fn, steps, warmup, torch):
@@ -301,7 +414,6 @@ def run_eval(args, form):
     if not args.no_cpu and args.workload == "metric":
         try:
             cb = cpu_baseline(tb, V, c["k"])
-            cb["host"] = cpu_info()
             res["cpu_baseline"] = cb
         except Exception as exc:  # reported, never silently dropped
             res["cpu_baseline"] = {"error": repr(exc)}
@@ -529,16 +641,7 @@ def run_screen(args):
     res["screen"] = {"reached_per_stage": [int(x) for x in reached], "passed": passed,
                      "zero_evals_per_step": zero_evals,
                      "zero_evals_per_s": zero_evals / (ms * 1e-3)}
-    rate = zero_evals / (ms * 1e-3)
-    w = WORK["screen"]
-    achieved = rate * 2 * w["slots"] / 1e12
-    res["roofline"] = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                       "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                       "traffic": traffic_summary("screen"),
-                       "achieved_valu_slots_tops": rate * w["slots"] / 1e12,
-                       "valu_slot_peak_tops": VALU_PEAK_TOPS, "kernel_ms": ms,
-                       "work_per_zero_eval": w, "algorithmic_bytes": n_cand * 40,
-                       "hbm_GBps_algorithmic": n_cand * 40 / (ms * 1e-3) / 1e9}
+    res["roofline"] = roofline("screen", zero_evals, ms, n_cand * 40, "screen")
     res["wall_s"] = wall
     return res
 

@@ -11,7 +11,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libk2eval.so")
 SOURCES = ["k2_kernels.hip", "k2_batch.hip", "k2_screen.hip", "k2_compat.hip", "k2_enclose.hip",
-           "k2_multiw.hip", "k2_api.hip"]
+           "k2_multiw.hip", "k2_rates.hip", "k2_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("K2_OFFLOAD_ARCH", "gfx950")
 

@@ -100,6 +100,7 @@ def _declare(L):
     L.k2_phase_words.argtypes = [_vp, ctypes.c_uint32, _u64p, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_int, _u64p]
     L.k2_sincos_words.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_int, _f64p]
+    L.k2_valu_rates.argtypes = [_f64p]
     L.k2_profile.argtypes = [_vp, ctypes.c_int]
     L.k2_kernel_time.argtypes = [_vp, _f64p, ctypes.POINTER(ctypes.c_int64)]
     L.compute_peak_AB.argtypes = [ctypes.c_int, _u64p, _u64p, _f64p, ctypes.c_int, _f64p, _f64p] \
@@ -340,6 +341,14 @@ class K2Table:
         _check(load_library().k2_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n)),
                "k2_kernel_time")
         return ms.value, n.value
+
+
+def valu_rates():
+    """Measured issue rates (lane operations/s) of v_mad_u64_u32, v_fma_f64, v_fma_f32 and
+    v_sin_f32 on the current GPU (k2_valu_rates)."""
+    out = np.zeros(4)
+    _check(load_library().k2_valu_rates(_p(out, _f64p)), "k2_valu_rates")
+    return {"int32_mad": out[0], "fp64_fma": out[1], "fp32_fma": out[2], "trans": out[3]}
 
 
 def exp_neg_host(x):
