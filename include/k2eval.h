@@ -14,8 +14,12 @@
  *
  * Conventions: every entry point returns 0 on success or a K2_E* code, and k2_last_error()
  * describes the last failure of the calling thread.  Host buffers are borrowed for the duration
- * of the call.  Calls on one table handle are serialised by the caller; distinct handles may be
- * used from distinct threads.  *_device entry points take device pointers and a hipStream_t
+ * of the call.  Calls on one table handle are serialised by the caller (host threads); distinct
+ * handles may be used from distinct threads.  The device work of calls on one handle (or one
+ * batch plan) runs in call order even when the calls name different streams: a call whose stream
+ * differs from the previous call's first makes its stream wait for that call's work, since the
+ * calls share the handle's scratch buffers and weight cache.  Work of distinct handles may
+ * overlap.  *_device entry points take device pointers and a hipStream_t
  * (as void*; NULL = the default stream) and are asynchronous with respect to the host, except
  * the first use of a new k, which computes that k's weight arrays and waits for them,
  * k2_polish_device, which reads the still-moving candidates back between Newton steps, and
@@ -202,6 +206,12 @@ int k2_phase_words(k2_table *t, uint32_t n_cand, const uint64_t *V, uint32_t z0,
 /* Debug: the evaluator's sin/cos of 2 pi W / 2^64 (mode 0, FP64) or the FP32 form's of the
  * window floor(W / 2^18) (mode 1, widened); out[2i] = sin, out[2i+1] = cos.  Host buffers. */
 int k2_sincos_words(uint32_t n, const uint64_t *W, int mode, double *out);
+
+/* Debug: measured issue rates of the VALU instruction classes of the evaluation kernels on the
+ * current GPU, in lane operations per second: out[0] v_mad_u64_u32 (the exact phase's integer
+ * multiply-add), out[1] v_fma_f64, out[2] v_fma_f32, out[3] v_sin_f32 (hardware transcendental).
+ * The class peaks of the roofline; no counterpart in the reference. */
+int k2_valu_rates(double *out);
 
 /* Timing of the evaluation kernels on the stream they run on (HIP events around each launch).
  * enable != 0 starts collecting; k2_kernel_time returns the summed milliseconds and the launch

@@ -122,6 +122,16 @@ extern "C" int k2_table_create(k2_table** out, uint32_t n_zeros, const uint64_t*
         k2_table_destroy(t);
         return fail(K2_E_HIP, msg);
     }
+    // the mixed form's second stream and its fork/join events, and the call-ordering event,
+    // exist from here on (no entry point creates them later)
+    e = hipStreamCreateWithFlags(&t->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->order.ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        k2_table_destroy(t);
+        return fail(K2_E_HIP, std::string("k2_table_create: ") + hipGetErrorString(e));
+    }
     *out = t;
     return K2_OK;
 }
@@ -129,6 +139,7 @@ extern "C" int k2_table_create(k2_table** out, uint32_t n_zeros, const uint64_t*
 extern "C" int k2_table_destroy(k2_table* t) {
     if (!t) return K2_OK;
     (void)hipSetDevice(t->device);
+    if (t->order.valid) (void)hipEventSynchronize(t->order.ev);   // the last call's work
     if (t->stream) (void)hipStreamSynchronize(t->stream);
     for (auto& ws : t->wsets) free_weights(ws);
     for (auto& pp : t->prof_pending) { (void)hipEventDestroy(pp.a); (void)hipEventDestroy(pp.b); }
@@ -143,6 +154,7 @@ extern "C" int k2_table_destroy(k2_table* t) {
     if (t->aux) (void)hipStreamDestroy(t->aux);
     if (t->fork) (void)hipEventDestroy(t->fork);
     if (t->join) (void)hipEventDestroy(t->join);
+    if (t->order.ev) (void)hipEventDestroy(t->order.ev);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return K2_OK;
@@ -348,10 +360,42 @@ static void fill_args(k2_table* t, WeightArrays* ws, const uint4* cand, int n_ca
     a.tab32 = t->tab32;
 }
 
+// Where the split partial sums go: the table's buffer, grown on demand (grow), or a buffer of
+// fixed size cap (the plan is then cut to fit it, and calls it cannot fit are rejected).
+struct PartialBuf {
+    double* p = nullptr;
+    size_t cap = 0;
+    bool grow = true;
+};
+
+static int partial_for(k2_table* t, PartialBuf& pb, size_t need, const char* who) {
+    if (pb.grow) {
+        int rc = ensure_bytes((void**)&t->partial, &t->partial_bytes, need);
+        pb.p = t->partial;
+        return rc;
+    }
+    if (need > pb.cap)
+        return fail(K2_E_ARG, std::string(who) + ": the partial sums need " + std::to_string(need) +
+                                  " bytes, more than the " + std::to_string(pb.cap) +
+                                  "-byte scratch");
+    return K2_OK;
+}
+
+// splits of at most max_rows rows over nz zeros (whole 256-zero tiles where the splits are that
+// long, else multiples of K2_MIN_SPLIT)
+static void fit_splits(int nz, int max_rows, int* n_splits, int* zps) {
+    if (nz <= 0) { *n_splits = 0; *zps = 0; return; }
+    int z = (nz + max_rows - 1) / max_rows;
+    z = z >= 256 ? (z + 255) / 256 * 256 : (z + K2_MIN_SPLIT - 1) / K2_MIN_SPLIT * K2_MIN_SPLIT;
+    *zps = z;
+    *n_splits = (nz + z - 1) / z;
+}
+
 // few candidates: Z tiles (lanes over zeros, K2_Z_CZ candidates per workgroup), FP64 head chunks
 // then FP32 tail chunks in the mixed form, one wavefront per candidate in the reduction
 static int eval_core_z(k2_table* t, int n_cand, const uint4* cand, WeightArrays* ws, int nz,
-                       unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
+                       unsigned flags, double* K, double* dK, double* d2K, hipStream_t st,
+                       PartialBuf& pb) {
     // one candidate group runs in FP64 throughout: there the FP32 tail's second launch costs
     // more than it saves (1 candidate x 2,325,572 zeros: 0.033 ms in FP64, 0.055 ms mixed)
     const bool mixed = (flags & K2_MIXED) != 0 && n_cand > K2_Z_CZ;
@@ -364,12 +408,14 @@ static int eval_core_z(k2_table* t, int n_cand, const uint4* cand, WeightArrays*
     z_chunks(nz - head, n_groups, t->tiling.target_blocks, &ch_t, &zpc_t);
     const int n_rows = std::max(1, ch_h + ch_t);
     const size_t row = (size_t)n_groups * 3 * K2_Z_CZ;
-    int rc = ensure_bytes((void**)&t->partial, &t->partial_bytes, n_rows * row * sizeof(double));
+    int rc = partial_for(t, pb, n_rows * row * sizeof(double), "Z-tile evaluation");
     if (rc) return rc;
     LaunchArgs a;
     fill_args(t, ws, cand, n_cand, flags, a);
+    a.partial = pb.p;
+    const bool prof = t->profile && pb.grow;
     ProfPair pp{nullptr, nullptr};
-    if (t->profile) {
+    if (prof) {
         pp.a = pool_event(t);
         pp.b = pool_event(t);
         HIPCHK(hipEventRecord(pp.a, st));
@@ -382,26 +428,50 @@ static int eval_core_z(k2_table* t, int n_cand, const uint4* cand, WeightArrays*
         a.z_begin = head; a.z_end = nz; a.row_base = ch_h;
         HIPCHK(launch_eval_z(a, true, n_groups, ch_t, zpc_t, st));
     }
-    if (t->profile) {
+    if (prof) {
         HIPCHK(hipEventRecord(pp.b, st));
         t->prof_pending.push_back(pp);
     }
-    if (ch_h + ch_t == 0) HIPCHK(hipMemsetAsync(t->partial, 0, row * sizeof(double), st));
-    HIPCHK(launch_reduce_z(t->partial, n_rows, n_groups, n_cand, flags, K, dK, d2K, st));
+    if (ch_h + ch_t == 0) return fail(K2_E_ARG, "Z-tile evaluation: empty zero range");
+    HIPCHK(launch_reduce_z(pb.p, n_rows, n_groups, n_cand, flags, K, dK, d2K, st));
     return K2_OK;
 }
 
-// evaluation of one candidate list over zeros [0, nz) with weights ws; outputs on device
+// evaluation of one candidate list over zeros [0, nz) (nz >= 1) with weights ws; outputs on
+// device; partial sums in pb
 static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* ws, int nz,
-                     unsigned flags, double* K, double* dK, double* d2K, hipStream_t st) {
+                     unsigned flags, double* K, double* dK, double* d2K, hipStream_t st,
+                     PartialBuf& pb) {
     const int zmax = t->tiling.z_max_cand == 0 ? K2_Z_MAX_CAND : t->tiling.z_max_cand;
     if (n_cand > 0 && n_cand <= zmax)
-        return eval_core_z(t, n_cand, cand, ws, nz, flags, K, dK, d2K, st);
+        return eval_core_z(t, n_cand, cand, ws, nz, flags, K, dK, d2K, st, pb);
+    if (nz <= 0) return fail(K2_E_ARG, "evaluation: empty zero range");
     const bool mixed = (flags & K2_MIXED) != 0;
     Plan p = make_plan(t, ws, n_cand, nz, mixed);
+    const size_t row_bytes = (size_t)3 * p.n_cand_pad * sizeof(double);
+    if (!pb.grow) {
+        // fixed scratch: fewer, longer splits where the plan's rows do not fit
+        const int64_t max_rows = (int64_t)(pb.cap / row_bytes);
+        const int need = (p.sp_head > 0) + (p.sp_tail > 0);
+        if (max_rows < need)
+            return fail(K2_E_ARG, "evaluation: " + std::to_string(n_cand) + " candidates need " +
+                                      std::to_string(need * row_bytes) + " bytes of partial sums, "
+                                      "more than the " + std::to_string(pb.cap) + "-byte scratch");
+        if (p.sp_head + p.sp_tail > max_rows) {
+            if (p.sp_tail > 0 && p.sp_head > 0) {
+                int rh = (int)std::max<int64_t>(1, max_rows * p.sp_head / (p.sp_head + p.sp_tail));
+                rh = (int)std::min<int64_t>(rh, max_rows - 1);
+                fit_splits(p.head_end, rh, &p.sp_head, &p.zps_head);
+                fit_splits(nz - p.head_end, (int)(max_rows - p.sp_head), &p.sp_tail, &p.zps_tail);
+            } else if (p.sp_tail > 0) {
+                fit_splits(nz - p.head_end, (int)max_rows, &p.sp_tail, &p.zps_tail);
+            } else {
+                fit_splits(p.head_end, (int)max_rows, &p.sp_head, &p.zps_head);
+            }
+        }
+    }
     const int n_rows = std::max(1, p.sp_head + p.sp_tail);
-    int rc = ensure_bytes((void**)&t->partial, &t->partial_bytes,
-                          (size_t)n_rows * 3 * p.n_cand_pad * sizeof(double));
+    int rc = partial_for(t, pb, (size_t)n_rows * row_bytes, "evaluation");
     if (rc) return rc;
     LaunchArgs a;
     a.nc = t->tiling.nc;
@@ -414,12 +484,13 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
     a.cand = cand;
     a.n_cand = n_cand;
     a.n_ctiles = p.n_ctiles;
-    a.partial = t->partial;
+    a.partial = pb.p;
     a.n_cand_pad = p.n_cand_pad;
     a.tab64 = t->tab64;
     a.tab32 = t->tab32;
+    const bool prof = t->profile && pb.grow;
     ProfPair pp{nullptr, nullptr};
-    if (t->profile) {
+    if (prof) {
         pp.a = pool_event(t);
         pp.b = pool_event(t);
         HIPCHK(hipEventRecord(pp.a, st));
@@ -433,11 +504,6 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
         HIPCHK(launch_eval_mixed(a, p.head_end, p.zps_head, p.sp_head, p.zps_tail, st));
     } else if (p.sp_tail > 0 && p.sp_head > 0) {
         // mixed form: FP64 head on the auxiliary stream, FP32 tail on st, joined before reducing
-        if (!t->aux) {
-            HIPCHK(hipStreamCreateWithFlags(&t->aux, hipStreamNonBlocking));
-            HIPCHK(hipEventCreateWithFlags(&t->fork, hipEventDisableTiming));
-            HIPCHK(hipEventCreateWithFlags(&t->join, hipEventDisableTiming));
-        }
         HIPCHK(hipEventRecord(t->fork, st));
         HIPCHK(hipStreamWaitEvent(t->aux, t->fork, 0));
         a.z_begin = 0; a.z_end = p.head_end; a.zps = p.zps_head; a.row_base = 0;
@@ -457,15 +523,11 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
         a.n_blocks = p.n_ctiles * p.sp_head;
         HIPCHK(launch_eval(a, false, st));
     }
-    if (t->profile) {
+    if (prof) {
         HIPCHK(hipEventRecord(pp.b, st));
         t->prof_pending.push_back(pp);
     }
-    if (p.sp_head + p.sp_tail == 0) {
-        // empty zero range: the sums are zero
-        HIPCHK(hipMemsetAsync(t->partial, 0, (size_t)3 * p.n_cand_pad * sizeof(double), st));
-    }
-    HIPCHK(launch_reduce(t->partial, n_rows, n_cand, p.n_cand_pad, flags, K, dK, d2K, st));
+    HIPCHK(launch_reduce(pb.p, n_rows, n_cand, p.n_cand_pad, flags, K, dK, d2K, st));
     return K2_OK;
 }
 
@@ -492,11 +554,14 @@ extern "C" int k2_eval_device(k2_table* t, uint32_t n_cand, const uint64_t* V_de
     std::lock_guard<std::mutex> lk(t->mu);
     HIPCHK(hipSetDevice(t->device));
     hipStream_t st = (hipStream_t)stream;   // NULL is the default stream
+    CallOrder order(t, st);
+    HIPCHK(order.err());
     WeightArrays* ws = nullptr;
     rc = get_weights(t, k, st, &ws);
     if (rc) return rc;
+    PartialBuf pb;
     return eval_core(t, (int)n_cand, (const uint4*)V_dev, ws, (int)n_zeros, flags, K_dev, dK_dev,
-                     d2K_dev, st);
+                     d2K_dev, st, pb);
 }
 
 extern "C" int k2_eval(k2_table* t, uint32_t n_cand, const uint64_t* V, double k, uint32_t n_zeros,
@@ -512,6 +577,8 @@ extern "C" int k2_eval(k2_table* t, uint32_t n_cand, const uint64_t* V, double k
     const size_t nb_v = (size_t)n_cand * 16, nb_o = (size_t)n_cand * 8;
     rc = ensure_bytes(&t->host_io, &t->host_io_bytes, nb_v + 3 * nb_o + 64);
     if (rc) return rc;
+    CallOrder order(t, t->stream);
+    HIPCHK(order.err());
     char* base = (char*)t->host_io;
     uint4* dV = (uint4*)base;
     double* dK0 = (double*)(base + nb_v);
@@ -521,7 +588,8 @@ extern "C" int k2_eval(k2_table* t, uint32_t n_cand, const uint64_t* V, double k
     WeightArrays* ws = nullptr;
     rc = get_weights(t, k, t->stream, &ws);
     if (rc) return rc;
-    rc = eval_core(t, (int)n_cand, dV, ws, (int)n_zeros, flags, dK0, dK1, dK2, t->stream);
+    PartialBuf pb;
+    rc = eval_core(t, (int)n_cand, dV, ws, (int)n_zeros, flags, dK0, dK1, dK2, t->stream, pb);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(K, dK0, nb_o, hipMemcpyDeviceToHost, t->stream));
     if (flags & K2_D1) HIPCHK(hipMemcpyAsync(dK, dK1, nb_o, hipMemcpyDeviceToHost, t->stream));
@@ -551,6 +619,9 @@ extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, d
     {
         std::lock_guard<std::mutex> lk(t->mu);
         HIPCHK(hipSetDevice(t->device));
+        // after the previous call's work on the table (the evaluations inside order themselves,
+        // and the last of them records the end of this call)
+        HIPCHK(t->order.wait(st));
         int rc = ensure_bytes((void**)&t->polish_active, &t->polish_cap, (size_t)n_cand * 4);
         if (!rc) rc = ensure_bytes(&t->polish_buf, &t->polish_buf_bytes,
                                    off_k + (size_t)n_cand * 3 * sizeof(double));
@@ -618,6 +689,8 @@ extern "C" int k2_mixed_head(k2_table* t, double k, uint32_t n_zeros, uint32_t* 
     if (n_zeros > t->n) return fail(K2_E_ARG, "k2_mixed_head: n_zeros exceeds the table");
     std::lock_guard<std::mutex> lk(t->mu);
     HIPCHK(hipSetDevice(t->device));
+    CallOrder order(t, t->stream);
+    HIPCHK(order.err());
     WeightArrays* ws = nullptr;
     int rc = get_weights(t, k, t->stream, &ws);
     if (rc) return rc;
@@ -638,6 +711,8 @@ extern "C" int k2_phase_words(k2_table* t, uint32_t n_cand, const uint64_t* V, u
     const size_t nb_v = (size_t)n_cand * 16, nb_o = (size_t)n_cand * nz * 8;
     int rc = ensure_bytes(&t->host_io, &t->host_io_bytes, nb_v + nb_o + 64);
     if (rc) return rc;
+    CallOrder order(t, t->stream);
+    HIPCHK(order.err());
     uint4* dV = (uint4*)t->host_io;
     uint64_t* dO = (uint64_t*)((char*)t->host_io + nb_v);
     HIPCHK(hipMemcpyAsync(dV, V, nb_v, hipMemcpyHostToDevice, t->stream));

@@ -314,6 +314,7 @@ struct k2_batch {
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    StreamOrder order;   // runs on different streams share the plan's partial sums and counter
     k2_batch_info info{};
 };
 
@@ -332,6 +333,8 @@ extern "C" int k2_batch_destroy(k2_batch* b) {
     if (!b) return K2_OK;
     if (b->t) (void)hipSetDevice(b->t->device);
     if (b->cap) (void)hipStreamSynchronize(b->cap);
+    if (b->order.valid) (void)hipEventSynchronize(b->order.ev);
+    if (b->order.ev) (void)hipEventDestroy(b->order.ev);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (auto& w : b->w) free_weights(w);
@@ -392,6 +395,7 @@ extern "C" int k2_batch_create(k2_table* t, uint32_t n_req, const k2_request* re
     b->d2K = d2K_dev;
     BCHK(hipSetDevice(t->device));
     BCHK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    BCHK(hipEventCreateWithFlags(&b->order.ev, hipEventDisableTiming));
     for (double k : ks) {
         WeightArrays w;
         if (alloc_weights(w, t->n_pad) != K2_OK) {
@@ -458,7 +462,9 @@ static hipError_t batch_launches(k2_batch* b, hipStream_t st) {
 extern "C" int k2_batch_run_direct(k2_batch* b, void* stream) {
     if (!b) return berr(K2_E_ARG, "k2_batch_run_direct: null batch");
     hipError_t e = hipSetDevice(b->t->device);
+    if (e == hipSuccess) e = b->order.wait((hipStream_t)stream);
     if (e == hipSuccess) e = batch_launches(b, (hipStream_t)stream);
+    if (e == hipSuccess) e = b->order.mark((hipStream_t)stream);
     if (e != hipSuccess) return berr(K2_E_HIP, std::string("k2_batch_run_direct: ") + hipGetErrorString(e));
     return K2_OK;
 }
@@ -476,7 +482,9 @@ extern "C" int k2_batch_run(k2_batch* b, void* stream) {
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
     }
+    if (e == hipSuccess) e = b->order.wait((hipStream_t)stream);
     if (e == hipSuccess) e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    if (e == hipSuccess) e = b->order.mark((hipStream_t)stream);
     if (e != hipSuccess) return berr(K2_E_HIP, std::string("k2_batch_run: ") + hipGetErrorString(e));
     return K2_OK;
 }

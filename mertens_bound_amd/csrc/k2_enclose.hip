@@ -232,6 +232,8 @@ extern "C" int k2_enclose_device(k2_table* t, uint32_t n_cand, const uint64_t* V
     std::lock_guard<std::mutex> lk(t->mu);
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipSetDevice(t->device);
+    CallOrder order(t, st);   // the enclosure buffers are shared with earlier calls on the table
+    if (e == hipSuccess) e = order.err();
     const int n_blk = (int)(t->n_pad / K2_WG);
     if (e == hipSuccess && !t->enc_a) e = hipMalloc((void**)&t->enc_a, (size_t)t->n_pad * sizeof(double2));
     if (e == hipSuccess && !t->enc_b) e = hipMalloc((void**)&t->enc_b, (size_t)t->n_pad * sizeof(double2));

@@ -83,6 +83,25 @@ struct ProfPair {
     hipEvent_t a, b;
 };
 
+// Orders the device work of successive calls on one handle (a table or a batch plan) whose
+// scratch buffers they share: every call records `ev` on its stream when it has queued its
+// work, and a call arriving on another stream first makes that stream wait for `ev`.  Calls on
+// one stream are ordered by the stream itself.
+struct StreamOrder {
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;
+    bool valid = false;
+    hipError_t wait(hipStream_t st) {
+        if (!valid || last == st) return hipSuccess;
+        return hipStreamWaitEvent(st, ev, 0);
+    }
+    hipError_t mark(hipStream_t st) {
+        hipError_t e = hipEventRecord(ev, st);
+        if (e == hipSuccess) { last = st; valid = true; }
+        return e;
+    }
+};
+
 }  // namespace k2impl
 
 struct k2_table {
@@ -119,6 +138,7 @@ struct k2_table {
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;             // second stream for the FP64 head of the mixed form
     hipEvent_t fork = nullptr, join = nullptr;
+    k2impl::StreamOrder order;             // orders calls on different streams (shared scratch)
     k2_tiling tiling{2, 3072, 0, K2_MIXED_TAIL, K2_Z_MAX_CAND, 256};
     bool profile = false;
     std::vector<ProfPair> prof_pending;
@@ -130,6 +150,18 @@ struct k2_table {
 
 
 namespace k2impl {
+
+// RAII form of StreamOrder for one entry point on table t and stream st: the constructor makes st
+// wait for the previous call's work when that ran on another stream (check err()), the
+// destructor records this call's end on st
+struct CallOrder {
+    k2_table* t;
+    hipStream_t st;
+    hipError_t e;
+    CallOrder(k2_table* t_, hipStream_t st_) : t(t_), st(st_), e(t_->order.wait(st_)) {}
+    ~CallOrder() { (void)t->order.mark(st); }
+    hipError_t err() const { return e; }
+};
 
 hipError_t launch_eval(const LaunchArgs& a, bool mixed, hipStream_t st);
 // mixed form in one launch: splits [0, sp_head) FP64 over [0, head_end) in zps_head steps, the

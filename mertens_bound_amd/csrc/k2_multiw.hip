@@ -228,6 +228,8 @@ extern "C" int k2_eval_weights_device(k2_table* t, uint32_t n_cand, const uint64
     std::lock_guard<std::mutex> lk(t->mu);
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipSetDevice(t->device);
+    CallOrder order(t, st);   // the partial sums are shared with earlier calls on the table
+    if (e == hipSuccess) e = order.err();
     const int cw = 256 * mw_nc((int)n_w, flags);   // candidates per C tile
     const int n_ctiles = (int)((n_cand + cw - 1) / cw);
     const int n_cand_pad = n_ctiles * cw;

@@ -203,6 +203,8 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
     std::lock_guard<std::mutex> lk(t->mu);
     hipError_t e = hipSetDevice(t->device);
     hipStream_t st = (hipStream_t)stream;
+    CallOrder order(t, st);   // the zero records are shared with earlier calls on the table
+    if (e == hipSuccess) e = order.err();
     // the zero records live in a device buffer owned by the table; the stage parameters travel
     // as kernel arguments, so the call stays asynchronous
     if (e == hipSuccess && t->screen_params == nullptr)
