@@ -100,6 +100,19 @@ int k2_eval_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double k
                    uint32_t n_zeros, uint32_t flags, double *K_dev, double *dK_dev,
                    double *d2K_dev, void *stream);
 
+/* k2_eval_device without any device allocation: no counterpart in the reference; for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops).  The partial sums go
+ * to a static 64 MiB scratch array of the library (one per GPU, held until the process ends)
+ * instead of the table's growable buffer; plans are cut to fewer zero splits where they would
+ * not fit, and calls whose candidates need more than 64 MiB even so (above about 2.7 million
+ * candidates, 1.3 million in the mixed form) return K2_E_ARG.  The weights of k must already be
+ * resident in the table (a k2_mixed_head or k2_eval_device call with this k computes them; at
+ * most 4 k per table), else K2_E_ARG.  All calls on one GPU share the scratch: calls on
+ * different tables or streams that overlap in time race (calls on one table are ordered). */
+int k2_eval_device_noalloc(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double k,
+                           uint32_t n_zeros, uint32_t flags, double *K_dev, double *dK_dev,
+                           double *d2K_dev, void *stream);
+
 /* Uneven requests in one call (device buffers): requests share the table, each has its own
  * zero cutoff, k and candidate slice (slices may not overlap).  Outputs are indexed like V
  * (cand_offset + i); entries outside every request are left untouched.  One persistent launch

@@ -69,6 +69,34 @@ static int fail(int code, const std::string& msg) { return set_error(code, msg);
 extern "C" const char* k2_last_error(void) { return g_err.c_str(); }
 extern "C" int k2_version(void) { return K2_VERSION; }
 
+// ---- static scratch of the allocation-free entry point (k2_eval_device_noalloc) ---------------
+// Each GPU the library runs on holds one copy of this array until the process ends.  In host code
+// `scratch` names a host-side stand-in, not the device array: the device address comes from
+// hipGetSymbolAddress, taken once per device (with that device current) when the first table
+// is created on it, and only that address is passed to kernels.
+#define K2_SCRATCH_BYTES ((size_t)64 << 20)
+#define K2_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[K2_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[K2_MAX_DEVICES];
+
+static int scratch_init(int dev) {
+    if (dev < 0 || dev >= K2_MAX_DEVICES) return fail(K2_E_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return K2_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(K2_E_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+    return K2_OK;
+}
+
+static void* scratch_ptr(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (dev >= 0 && dev < K2_MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+}
+
 static int ensure_bytes(void** p, size_t* cap, size_t need) {
     if (*cap >= need) return K2_OK;
     if (*p) (void)hipFree(*p);
@@ -131,6 +159,12 @@ extern "C" int k2_table_create(k2_table** out, uint32_t n_zeros, const uint64_t*
     if (e != hipSuccess) {
         k2_table_destroy(t);
         return fail(K2_E_HIP, std::string("k2_table_create: ") + hipGetErrorString(e));
+    }
+    int rc = scratch_init(t->device);
+    if (rc) {
+        std::string msg = g_err;
+        k2_table_destroy(t);
+        return fail(rc, msg);
     }
     *out = t;
     return K2_OK;
@@ -360,8 +394,9 @@ static void fill_args(k2_table* t, WeightArrays* ws, const uint4* cand, int n_ca
     a.tab32 = t->tab32;
 }
 
-// Where the split partial sums go: the table's buffer, grown on demand (grow), or a buffer of
-// fixed size cap (the plan is then cut to fit it, and calls it cannot fit are rejected).
+// Where the split partial sums go: the table's buffer, grown on demand (grow), or the static
+// scratch array of the allocation-free entry point, of fixed size cap (the plan is then cut to
+// fit it, and calls it cannot fit are rejected).
 struct PartialBuf {
     double* p = nullptr;
     size_t cap = 0;
@@ -560,6 +595,40 @@ extern "C" int k2_eval_device(k2_table* t, uint32_t n_cand, const uint64_t* V_de
     rc = get_weights(t, k, st, &ws);
     if (rc) return rc;
     PartialBuf pb;
+    return eval_core(t, (int)n_cand, (const uint4*)V_dev, ws, (int)n_zeros, flags, K_dev, dK_dev,
+                     d2K_dev, st, pb);
+}
+
+extern "C" int k2_eval_device_noalloc(k2_table* t, uint32_t n_cand, const uint64_t* V_dev,
+                                      double k, uint32_t n_zeros, uint32_t flags, double* K_dev,
+                                      double* dK_dev, double* d2K_dev, void* stream) {
+    int rc = check_eval_args(t, n_cand, V_dev, &n_zeros, K_dev, "k2_eval_device_noalloc");
+    if (rc) return rc;
+    if (((uintptr_t)V_dev & 15) != 0)
+        return fail(K2_E_ARG, "k2_eval_device_noalloc: V not 16-byte aligned");
+    if (((flags & K2_D1) && !dK_dev) || ((flags & K2_D2) && !d2K_dev))
+        return fail(K2_E_ARG, "k2_eval_device_noalloc: derivative requested without an output buffer");
+    if (!(k >= 0.0) || !isfinite(k))
+        return fail(K2_E_ARG, "k2_eval_device_noalloc: k must be finite and >= 0");
+    if (n_cand == 0) return K2_OK;
+    std::lock_guard<std::mutex> lk(t->mu);
+    HIPCHK(hipSetDevice(t->device));
+    hipStream_t st = (hipStream_t)stream;   // NULL is the default stream
+    WeightArrays* ws = nullptr;
+    for (auto& w : t->wsets)   // lookup only: computing a new k's weights would allocate
+        if (w.k == k) { ws = &w; break; }
+    if (!ws)
+        return fail(K2_E_ARG, "k2_eval_device_noalloc: the weights for this k are not resident "
+                              "(a k2_mixed_head or k2_eval_device call with this k computes them)");
+    t->use_clock++;
+    ws->last_use = t->use_clock;
+    PartialBuf pb;
+    pb.p = (double*)scratch_ptr(t->device);
+    pb.cap = K2_SCRATCH_BYTES;
+    pb.grow = false;
+    if (!pb.p) return fail(K2_E_HIP, "k2_eval_device_noalloc: no scratch address for this device");
+    CallOrder order(t, st);
+    HIPCHK(order.err());
     return eval_core(t, (int)n_cand, (const uint4*)V_dev, ws, (int)n_zeros, flags, K_dev, dK_dev,
                      d2K_dev, st, pb);
 }
