@@ -5,9 +5,10 @@ points ``V0``: v = V0 + delta.  A float64 tensor cannot hold v ~ 3.5e18 to sub-u
 so the exact part stays in the integer words and only the small offset flows through autograd.
 ``V0 + delta`` is formed exactly on the device (delta rounded to 2^-64).
 
-forward:  K2(v) -- the kernel computes K2 and, when a gradient is needed, K2' (and K2'' when a
-          second derivative may be taken) in the same pass (the adjoint sums share the phase
-          and sin/cos of the value sum);
+forward:  K2(v) -- the kernel computes K2 and, when a gradient is needed, K2' (and K2'' when the
+          caller announces a second derivative, ``second_order=True``) in the same pass (the
+          adjoint sums share the phase and sin/cos of the value sum); a second derivative taken
+          without that announcement costs one more pass for K2'';
 backward: grad_delta = grad_out * K2'; that product is itself differentiable, with
           d/d delta (grad_out * K2') = grad_out * K2'', so torch.autograd.grad(...,
           create_graph=True) gives the diagonal second derivative used by Newton.
@@ -43,44 +44,59 @@ def offset_points(V0: torch.Tensor, delta: torch.Tensor) -> torch.Tensor:
 
 class _Deriv(torch.autograd.Function):
     """K2'(delta) as a function of delta (values from the forward pass); its derivative is
-    K2'' (third derivatives are not provided)."""
+    K2'' -- from the forward pass when it computed it, else from one more pass at the same exact
+    points (third derivatives are not provided)."""
 
     @staticmethod
-    def forward(ctx, delta, dK, d2K):
-        ctx.save_for_backward(d2K)
+    def forward(ctx, delta, dK, d2K, ev_args):
+        ctx.have_d2 = d2K is not None
+        ctx.ev_args = ev_args
+        ctx.save_for_backward(d2K if d2K is not None else dK)
         return dK.clone()
 
     @staticmethod
     def backward(ctx, g):
         (d2K,) = ctx.saved_tensors
-        return g * d2K, None, None
+        if not ctx.have_d2:
+            table, V, k, n_zeros, mixed = ctx.ev_args
+            flags = ev.VALUE | ev.D1 | ev.D2 | (ev.MIXED if mixed else 0)
+            _, _, d2K = table.eval_device(V, k, n_zeros=n_zeros, flags=flags)
+        return g * d2K, None, None, None
 
 
 class K2Function(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, delta, table, V0, k, n_zeros, mixed):
+    def forward(ctx, delta, table, V0, k, n_zeros, mixed, second_order):
         V = offset_points(V0, delta)
-        flags = ev.VALUE | ev.D1 | ev.D2 | (ev.MIXED if mixed else 0)
+        need_d1 = ctx.needs_input_grad[0]
+        flags = ev.VALUE | (ev.D1 if need_d1 else 0) | (ev.D2 if need_d1 and second_order else 0)
+        flags |= ev.MIXED if mixed else 0
         K, dK, d2K = table.eval_device(V, k, n_zeros=n_zeros, flags=flags)
-        ctx.save_for_backward(delta, dK, d2K)
+        ctx.ev_args = (table, V, k, n_zeros, mixed)
+        ctx.have_d2 = bool(flags & ev.D2)
+        if need_d1:
+            ctx.save_for_backward(delta, dK, d2K if ctx.have_d2 else dK)
         return K
 
     @staticmethod
     def backward(ctx, grad_out):
         delta, dK, d2K = ctx.saved_tensors
-        return grad_out * _Deriv.apply(delta, dK, d2K), None, None, None, None, None
+        d2 = d2K if ctx.have_d2 else None
+        return (grad_out * _Deriv.apply(delta, dK, d2, ctx.ev_args),
+                None, None, None, None, None, None)
 
 
 def k2(delta: torch.Tensor, table: ev.K2Table, V0: torch.Tensor, k: float, n_zeros=None,
-       mixed: bool = False) -> torch.Tensor:
-    """Differentiable K2 at v = V0 + delta."""
-    return K2Function.apply(delta, table, V0, float(k), n_zeros, mixed)
+       mixed: bool = False, second_order: bool = False) -> torch.Tensor:
+    """Differentiable K2 at v = V0 + delta.  second_order=True computes K2'' in the forward pass
+    as well (for callers that differentiate twice, e.g. Newton through autograd)."""
+    return K2Function.apply(delta, table, V0, float(k), n_zeros, mixed, second_order)
 
 
 def _second_derivative_graph(delta, table, V0, k, n_zeros, mixed):
     """K2' and K2'' via autograd (first derivative with create_graph, then its derivative)."""
     d = delta.detach().requires_grad_(True)
-    K = k2(d, table, V0, k, n_zeros, mixed)
+    K = k2(d, table, V0, k, n_zeros, mixed, second_order=True)
     (g,) = torch.autograd.grad(K.sum(), d, create_graph=True)
     (h,) = torch.autograd.grad(g.sum(), d)
     return K.detach(), g.detach(), h.detach()
@@ -125,7 +141,8 @@ def newton_polish(table: ev.K2Table, V0: torch.Tensor, k: float, n_steps: int = 
 def polish(table: ev.K2Table, V0: torch.Tensor, k: float, n_steps: int = 8, clamp: float = 1e-4,
            n_zeros=None, mixed: bool = False):
     """Device Newton polish (k2_polish_device: one fused K/K'/K'' pass + an exact 64.64 update
-    per step, no host round trips).  Same rule and return values as :func:`newton_polish`; with
+    per step on the device; the still-moving candidates are read back between steps to compact
+    the next pass).  Same rule and return values as :func:`newton_polish`; with
     mixed=True the sums run in the precision form and a candidate stops once |step| < 1e-11, the
     noise floor that form's FP32 rounding leaves in v."""
     K0, _, _ = table.eval_device(V0, k, n_zeros=n_zeros, flags=ev.VALUE)

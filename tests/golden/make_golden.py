@@ -12,8 +12,20 @@ the input sha256 values and the oracle outputs only.
                        search/path1_eval_fast.cpp) on reference-format words of a small table:
                        row sums and leading entries of compute_peak_AB's A/B, and the
                        polish_peaks_with_weights outputs
+  batch512_sample.npz  the 512 uneven requests on the 2,325,572-zero table (seed 0): two
+                       candidates (first and last) of each of 32 requests (every 16th), each at
+                       its own cutoff and k: binary128 K2, K2' and sum |terms|, sum g |terms|
+                       (about 1 min on 8 cores)
+  polish4096_sample.npz the 4,096-candidate polish workload on the 2,325,572-zero table: the
+                       start words of 16 candidates (every 256th), the oracle's FP64 Newton polish
+                       of them (8 steps, clamp 1e-4, the reference's step rule) and binary128 K2,
+                       K2', K2'' at its end points.  The start points come from a GPU screen
+                       (mertens_bound_amd.workloads.polish_candidates), so this one runs where a
+                       GPU is visible: OMP_NUM_THREADS=1 make_golden.py polish4096 --out <dir>
+                       (one single-threaded oracle polish per sampled candidate, side by side)
 
-Usage: python tests/golden/make_golden.py [w1024x10000|w65536x2325572|polish|ref|all]
+Usage: python tests/golden/make_golden.py [w1024x10000|w65536x2325572|polish|ref|batch512|
+                                           polish4096|all] [--out DIR]
 """
 import os
 import sys
@@ -60,6 +72,62 @@ def polish():
                         steps=st, table_sha256=tb.sha256(), k=3e-9)
 
 
+def batch512_sample_index(nz, ks, cnt, offs, every=16):
+    """(request, candidate) pairs of the batch fixture: the first and last candidate of every
+    `every`-th request (one where a request has a single candidate)."""
+    pairs = []
+    for r in range(0, len(nz), every):
+        pairs.append((r, int(offs[r])))
+        if cnt[r] > 1:
+            pairs.append((r, int(offs[r] + cnt[r] - 1)))
+    return pairs
+
+
+def batch512():
+    c = synth.W65536x2325572
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    nz, ks, cnt, offs, V = synth.make_batched_requests(tb, n_req=512, seed=0)
+    pairs = batch512_sample_index(nz, ks, cnt, offs)
+    t = time.time()
+    K = []; dK = []; aS = []; gS = []
+    for r, i in pairs:
+        sub = synth.ZeroTable(tb.G[:nz[r]], tb.gamma[:nz[r]], tb.w[:nz[r]])
+        k0, k1, _, s = oracle.eval_quad(sub, V[i:i + 1], ks[r], flags=oracle.VALUE | oracle.D1)
+        K.append(k0[0]); dK.append(k1[0]); aS.append(s[0])
+        gS.append(float(np.sum(2 * np.exp(-ks[r] * sub.gamma ** 2) * np.abs(sub.w) * sub.gamma)))
+    print(f"batch512 sample ({len(pairs)} candidates) oracle {time.time() - t:.1f}s")
+    req = np.array([r for r, _ in pairs]); idx = np.array([i for _, i in pairs])
+    np.savez_compressed(os.path.join(OUT, "batch512_sample.npz"), req=req, idx=idx, V=V[idx],
+                        n_zeros=nz[req], k=ks[req], K=np.array(K), dK=np.array(dK),
+                        absS=np.array(aS), gabsS=np.array(gS), table_sha256=tb.sha256())
+
+
+def polish4096():
+    import concurrent.futures
+    import torch
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import workloads
+    c = synth.W65536x2325572
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    torch.cuda.set_device(0)
+    V0 = workloads.polish_candidates(ev.K2Table(tb), 4096, seed=0)
+    idx = np.arange(0, 4096, 256)
+    t = time.time()
+    # one oracle polish per candidate, in parallel (the library releases the GIL)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(idx)) as ex:
+        res = list(ex.map(lambda i: oracle.polish_f64(tb, V0[i:i + 1], c["k"], n_steps=8,
+                                                      clamp=1e-4), idx))
+    Vp = np.concatenate([r[0] for r in res]); steps = np.concatenate([r[4] for r in res])
+    K0 = np.concatenate([r[1] for r in res])
+    print(f"polish4096 oracle polish of {len(idx)} candidates {time.time() - t:.1f}s")
+    t = time.time()
+    K, dK, d2K, aS = oracle.eval_quad(tb, Vp, c["k"], nthreads=len(idx))
+    print(f"polish4096 binary128 at the end points {time.time() - t:.1f}s")
+    np.savez_compressed(os.path.join(OUT, "polish4096_sample.npz"), idx=idx, V0=V0[idx], V=Vp,
+                        steps=steps, K0=K0, K=K, dK=dK, d2K=d2K, absS=aS, k=c["k"],
+                        table_sha256=tb.sha256())
+
+
 def ref_inputs():
     """Reference-format inputs of the ref_compat fixture (also used by the tests)."""
     tb = synth.make_zero_table(2000, 5)
@@ -92,9 +160,16 @@ def ref():
 
 
 if __name__ == "__main__":
-    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    args = sys.argv[1:]
+    if "--out" in args:
+        i = args.index("--out")
+        OUT = os.path.abspath(args[i + 1])
+        os.makedirs(OUT, exist_ok=True)
+        del args[i:i + 2]
+    what = args[0] if args else "all"
     oracle.build()
     for name, fn in (("w1024x10000", w1024x10000), ("polish", polish), ("ref", ref),
-                     ("w65536x2325572", w65536x2325572)):
+                     ("w65536x2325572", w65536x2325572), ("batch512", batch512),
+                     ("polish4096", polish4096)):
         if what in (name, "all"):
             fn()

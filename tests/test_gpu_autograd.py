@@ -18,13 +18,16 @@ def setup():
     return tb, ev.K2Table(tb), torch
 
 
-def test_gradient_vs_oracle_and_fd(setup, oracle_lib):
+@pytest.mark.parametrize("second_order", [False, True])
+def test_gradient_vs_oracle_and_fd(setup, oracle_lib, second_order):
+    """K2'' through create_graph: from the forward pass (second_order=True) or from one more pass
+    at the same points (the default forward computes K2 and K2' only)."""
     tb, t, torch = setup
     k = 1.5e-10
     V0 = synth.make_candidates(64, 3.4e18, 3.6e18, 12)
     V0t = ev.v_tensor(V0)
     delta = torch.zeros(64, dtype=torch.float64, device="cuda", requires_grad=True)
-    K = ag.k2(delta, t, V0t, k)
+    K = ag.k2(delta, t, V0t, k, second_order=second_order)
     (g,) = torch.autograd.grad(K.sum(), delta, create_graph=True)
     (h,) = torch.autograd.grad(g.sum(), delta)
     Ko, dKo, d2Ko, _ = oracle_lib.eval_quad(tb, V0, k)
