@@ -411,6 +411,21 @@ def run_eval(args, form):
                              "fp64_head_zeros": tm.mixed_head(c["k"]),
                              "max_abs_dK2_vs_oracle": accm,
                              "max_abs_dK2_vs_fp64_all_candidates": float(d.max().item())}
+    if form == "plain" and args.workload == "metric" and not args.no_check:
+        # the other full-size workloads on the same table, each timed the same way with its own
+        # accuracy check and roofline: 512 batched requests and the 4,096-candidate polish in
+        # both forms, and the leaf screen
+        ctx = (tb, ev.K2Table(tb))          # the library's default tiling, as standalone
+        bp = run_batched(args, "plain", ctx)
+        bm = run_batched(args, "mixed", ctx)
+        res["batched"] = {"plain": sub_record(bp), "mixed": sub_record(bm),
+                          "mixed_speedup_vs_batched_plain": bp["ms_per_step"] / bm["ms_per_step"],
+                          "mixed_evals_per_s_vs_plain_fp64_kernel": bm["value"] / res["value"]}
+        pp = run_polish(args, "plain", ctx)
+        pm = run_polish(args, "mixed", ctx)
+        res["polish"] = {"plain": sub_record(pp), "mixed": sub_record(pm),
+                         "mixed_speedup": pp["ms_per_step"] / pm["ms_per_step"]}
+        res["screen"] = sub_record(run_screen(args, ctx))
     if not args.no_cpu and args.workload == "metric":
         try:
             cb = cpu_baseline(tb, V, c["k"])
@@ -420,20 +435,40 @@ def run_eval(args, form):
     return res
 
 
-def run_batched(args):
+def full_table(ctx=None):
+    """(table, K2Table) of the 2,325,572-zero workload, reused when the caller built them."""
+    import torch
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import synth
+    if ctx is not None:
+        return ctx
+    c = synth.W65536x2325572
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    torch.cuda.set_device(0)
+    return tb, ev.K2Table(tb)
+
+
+def golden_npz(name):
+    """A committed fixture under tests/golden (data only), or None."""
+    import numpy as np
+    p = os.path.join(ROOT, "tests", "golden", name)
+    return np.load(p, allow_pickle=False) if os.path.exists(p) else None
+
+
+def run_batched(args, form=None, ctx=None):
+    """512 uneven requests (cutoffs, k, counts) in one persistent launch replayed from its HIP
+    graph; accuracy on the committed binary128 sample (batch512_sample.npz) and against the
+    per-request loop of one-launch calls (search/eval_tech1_600k.py:252-267)."""
     import numpy as np
     import torch
     from mertens_bound_amd import autotune
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import synth
-    c = synth.W65536x2325572
-    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
-    torch.cuda.set_device(0)
-    t = ev.K2Table(tb)
+    tb, t = full_table(ctx)
     nz, ks, cnt, offs, V = synth.make_batched_requests(tb, n_req=512, seed=0)
     reqs = [(int(a), int(b), int(o), float(k)) for a, b, o, k in zip(nz, cnt, offs, ks)]
     Vt = ev.v_tensor(V)
-    form = "mixed" if args.form == "mixed" else "plain"
+    form = form or ("mixed" if args.form == "mixed" else "plain")
     flags = ev.VALUE | ev.D1 | (ev.MIXED if form == "mixed" else 0)
     b = ev.K2Batch(t, reqs, Vt, flags=flags, tiling=autotune.batch_tiling(form))
     useful = float(np.sum(cnt.astype(float) * nz))
@@ -447,6 +482,8 @@ def run_batched(args):
                           out=(out[0][o:o + n], out[1][o:o + n], out[2][o:o + n]))
 
     ms_loop, _ = timed(loop, 1, 1, torch)
+    b.run()
+    torch.cuda.synchronize()
     d = (out[0] - b.K).abs().max().item()
     info = b.info()
     res = base_record(args, "candidate-zero evals/sec, 512 uneven requests (one graph launch)",
@@ -457,30 +494,42 @@ def run_batched(args):
                                    "table", "form": form, "batch": info})
     res["per_request_loop"] = {"ms": ms_loop, "evals_per_s": useful / (ms_loop * 1e-3),
                                "max_abs_diff_vs_batch": d}
+    g = None if args.no_check else golden_npz("batch512_sample.npz")
+    if g is not None and np.array_equal(V[g["idx"]], g["V"]):
+        K = b.K.cpu().numpy()[g["idx"]]
+        dK = b.dK.cpu().numpy()[g["idx"]]
+        eK = float(np.max(np.abs(K - g["K"])))
+        res["accuracy"] = {"max_abs_dK2_vs_oracle": eK,
+                           "max_dK2prime_vs_oracle_over_scale":
+                               float(np.max(np.abs(dK - g["dK"]) / g["gabsS"])),
+                           "oracle_sample": f"batch512_sample.npz ({len(g['idx'])} candidates of "
+                                            f"32 requests, binary128)",
+                           "bar": 1e-9, "ok": eK <= 1e-9}
     head = 0.0
     if form == "mixed":   # share of the useful evaluations in the requests' FP64 heads
         head = sum(float(n) * t.mixed_head(k, a) for a, n, _, k in reqs) / useful
     res["roofline"] = roofline(form, useful, ms, tb.n * 48 * 3 + V.shape[0] * 32,
                                f"batched_{form}", head)
     res["wall_s"] = wall
+    b.close()
     return res
 
 
-def run_polish(args):
+def run_polish(args, form=None, ctx=None):
+    """4,096 candidates near synthetic peaks, 8 Newton steps on the exact 64.64 points (the step
+    rule of search/path1_eval_fast.cpp:103-141), K2/K2'/K2'' passes on the device; accuracy on the
+    committed sample of the oracle's own polish (polish4096_sample.npz, binary128 values)."""
     import numpy as np
     import torch
     from mertens_bound_amd import autograd as ag
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import synth, workloads
+    tb, t = full_table(ctx)
     c = synth.W65536x2325572
-    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
-    torch.cuda.set_device(0)
-    t = ev.K2Table(tb)
     V0 = workloads.polish_candidates(t, 4096, seed=0)
     V0t = ev.v_tensor(V0)
     k = c["k"]
-
-    mixed = args.form == "mixed"
+    mixed = (form or args.form) == "mixed"
 
     def step():
         ag.polish(t, V0t, k, n_steps=8, mixed=mixed)
@@ -500,29 +549,46 @@ def run_polish(args):
                        "form": "mixed" if mixed else "plain",
                        "candidate_evals_per_polish": cand_evals,
                        "passes_equivalent": cand_evals / V0.shape[0]})
-    res["polish"] = {"mean_steps": float(steps.float().mean().item()),
+    st = steps.cpu().numpy()
+    res["polish"] = {"steps_histogram": np.bincount(st, minlength=9).tolist(),
                      "max_abs_dK_final": float(dK.abs().max().item()),
                      "mean_abs_K": float(K.abs().mean().item())}
-    if mixed and not args.no_check:
-        # the FP64 polish on the same candidates: time and agreement of the final values
-        def step64():
-            ag.polish(t, V0t, k, n_steps=8)
-
-        ms64, _ = timed(step64, args.steps, args.warmup, torch)
-        _, K64, _, steps64 = ag.polish(t, V0t, k, n_steps=8)
-        torch.cuda.synchronize()
-        conv = steps64 < 8   # stopped on the step rule in FP64
-        d = (K - K64).abs()
-        res["vs_plain"] = {"plain_ms_per_step": ms64, "speedup": ms64 / ms,
-                           "fp64_converged": int(conv.sum().item()),
-                           "max_abs_dK2_converged_vs_fp64_polish":
-                               float(d[conv].max().item()) if bool(conv.any()) else None,
-                           "max_abs_dK2_all_vs_fp64_polish": float(d.max().item()),
-                           "note": "candidates still moving after 8 steps end at different "
-                                   "points in the two forms (their Newton steps amplify K2' "
-                                   "differences); converged ones agree"}
+    g = None if args.no_check else golden_npz("polish4096_sample.npz")
+    if g is not None and np.array_equal(V0[g["idx"]], g["V0"]):
+        idx = g["idx"]
+        Vn = ev.v_from_tensor(V)[idx]
+        Kn = K.cpu().numpy()[idx]
+        dv = np.array([abs(a - b) / 2.0 ** 64 for a, b in
+                       zip(synth.v_to_int(Vn), synth.v_to_int(g["V"]))])
+        conv = (g["steps"] < 8) & (st[idx] < 8)
+        eK = np.abs(Kn - g["K"])
+        acc = {"oracle_sample": f"polish4096_sample.npz ({len(idx)} candidates: the oracle's FP64 "
+                                f"polish, binary128 values at its end points)",
+               "converged_in_both": int(conv.sum()),
+               "max_abs_dK2_converged": float(eK[conv].max()) if conv.any() else None,
+               "max_dv_converged": float(dv[conv].max()) if conv.any() else None,
+               "max_abs_dK2_all": float(eK.max()), "max_dv_all": float(dv.max()), "bar": 1e-9}
+        if not mixed:
+            acc["same_steps_as_oracle"] = bool(np.array_equal(st[idx], g["steps"]))
+            acc["ok"] = bool(eK.max() <= 1e-9 and acc["same_steps_as_oracle"])
+        else:
+            acc["ok"] = bool(conv.any() and eK[conv].max() <= 1e-9)
+            acc["note"] = ("candidates still moving after 8 steps end at other points in the "
+                           "precision form (its steps follow its own K2'), so only candidates "
+                           "converged in both forms are compared with the oracle's end points")
+        res["accuracy"] = acc
+    res["roofline"] = roofline("plain_d2" if not mixed else "mixed", evals, ms,
+                               tb.n * 64 + V0.shape[0] * 40, "polish_" + ("mixed" if mixed else
+                                                                          "plain"),
+                               t.mixed_head(k) / tb.n if mixed else 0.0)
     res["wall_s"] = wall
     return res
+
+
+def sub_record(res, keep=("value", "unit", "ms_per_step", "dtype", "config", "accuracy",
+                          "roofline", "per_request_loop", "polish", "screen", "wall_s")):
+    """The fields of a workload's own line that go into the default line as a sub-record."""
+    return {k: res[k] for k in keep if k in res}
 
 
 def run_enclose(args):
@@ -602,7 +668,7 @@ def run_weights(args):
     return res
 
 
-def run_screen(args):
+def run_screen(args, ctx=None):
     """The staged leaf screen (k2_screen_device) of search/gpu_mertens_stream.cu:186-250 with its
     constants: 64/128/256 zeros, thresholds 0.78/0.855/0.892, 7 shifts of 0.0014, on 2^24
     candidates v in [3.4e18, 3.6e18] against the first zeros of the full table (k = 3e-9).  A
@@ -613,11 +679,9 @@ def run_screen(args):
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import pipeline, synth
     c = synth.W65536x2325572
-    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
-    n_cand = args.n_cand or (1 << 24)
+    tb, t = full_table(ctx)
+    n_cand = args.n_cand if (args.n_cand and ctx is None) else (1 << 24)
     V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
-    torch.cuda.set_device(0)
-    t = ev.K2Table(tb)
     Vd = ev.v_tensor(V)
     k, d, m = 3e-9, 0.0014, 3
     stages = pipeline.REFERENCE_SCREEN
@@ -641,6 +705,16 @@ def run_screen(args):
     res["screen"] = {"reached_per_stage": [int(x) for x in reached], "passed": passed,
                      "zero_evals_per_step": zero_evals,
                      "zero_evals_per_s": zero_evals / (ms * 1e-3)}
+    g = None if args.no_check else golden_npz("screen_sample.npz")
+    if g is not None and np.array_equal(V[:g["V"].shape[0]], g["V"]):
+        n = g["V"].shape[0]
+        got = val[:n, :3]
+        err = np.where(got >= 0, np.abs(got - g["val"]) / g["scale"][None, :], 0.0)
+        res["accuracy"] = {"max_stage_value_error_over_sum_abs_a": float(err.max()),
+                           "values_compared": int((got >= 0).sum()),
+                           "oracle_sample": "screen_sample.npz (first 512 candidates, FP64 "
+                                            "oracle, every stage reached)",
+                           "bar": 2e-6, "ok": bool(err.max() <= 2e-6)}
     res["roofline"] = roofline("screen", zero_evals, ms, n_cand * 40, "screen")
     res["wall_s"] = wall
     return res

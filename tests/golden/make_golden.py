@@ -24,8 +24,13 @@ the input sha256 values and the oracle outputs only.
                        GPU is visible: OMP_NUM_THREADS=1 make_golden.py polish4096 --out <dir>
                        (one single-threaded oracle polish per sampled candidate, side by side)
 
+  screen_sample.npz    the leaf-screen workload (2^24 candidates, k = 3e-9, 7 shifts of 0.0014):
+                       for the first 512 candidates, max over the shifts of |K2| over the first
+                       64, 128 and 256 zeros (the three stage values) in the oracle's FP64 tier,
+                       and sum |a| per stage (the values' scale)
+
 Usage: python tests/golden/make_golden.py [w1024x10000|w65536x2325572|polish|ref|batch512|
-                                           polish4096|all] [--out DIR]
+                                           polish4096|screen|all] [--out DIR]
 """
 import os
 import sys
@@ -128,6 +133,24 @@ def polish4096():
                         table_sha256=tb.sha256())
 
 
+def screen():
+    c = synth.W65536x2325572
+    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
+    V = synth.make_candidates(1 << 24, c["v_lo"], c["v_hi"], c["seed"])[:512]
+    k, d = 3e-9, 0.0014
+    vals = np.zeros((V.shape[0], 3))
+    scale = np.zeros(3)
+    for s, nz in enumerate((64, 128, 256)):
+        hs = []
+        for m in range(-3, 4):
+            Vm = synth.v_from_int([v + int(round(m * d * 2.0 ** 64)) for v in synth.v_to_int(V)])
+            hs.append(oracle.eval_f64(tb, Vm, k, flags=oracle.VALUE, n_zeros=nz)[0])
+        vals[:, s] = np.max(np.abs(np.stack(hs, axis=1)), axis=1)
+        scale[s] = np.sum(2 * np.exp(-k * tb.gamma[:nz] ** 2) * np.abs(tb.w[:nz]))
+    np.savez_compressed(os.path.join(OUT, "screen_sample.npz"), V=V, val=vals, scale=scale, k=k,
+                        shift=d, table_sha256=tb.sha256())
+
+
 def ref_inputs():
     """Reference-format inputs of the ref_compat fixture (also used by the tests)."""
     tb = synth.make_zero_table(2000, 5)
@@ -170,6 +193,6 @@ if __name__ == "__main__":
     oracle.build()
     for name, fn in (("w1024x10000", w1024x10000), ("polish", polish), ("ref", ref),
                      ("w65536x2325572", w65536x2325572), ("batch512", batch512),
-                     ("polish4096", polish4096)):
+                     ("polish4096", polish4096), ("screen", screen)):
         if what in (name, "all"):
             fn()
