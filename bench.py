@@ -175,6 +175,16 @@ def cpu_baseline(tb, V, k, budget_s=15.0, n_c=256, chunk=1 << 17):
         runs.append({"kind": "reference", "threads": 16,
                      "workload": f"{tb1.n} zeros x {V1.shape[0]}",
                      "value": V1.shape[0] * tb1.n / dt, "s": dt})
+        # compute_peak_AB alone at its published size (path1_report.txt:34: 0.737 s)
+        sub, a, _ = compat_inputs(tb)
+        t0 = time.perf_counter()
+        A, B = oracle._call_cpab(L.compute_peak_AB, *a)
+        dt = time.perf_counter() - t0
+        del A, B
+        runs.append({"kind": "reference", "threads": 16,
+                     "workload": f"compute_peak_AB {a[0].shape[0]} peaks x {sub.n} zeros "
+                                 f"(published 0.737 s)",
+                     "value": a[0].shape[0] * sub.n / dt, "s": dt})
     out["runs"] = runs
     if L is not None:
         batches = []                                 # candidate words: not timed
@@ -426,6 +436,7 @@ def run_eval(args, form):
         res["polish"] = {"plain": sub_record(pp), "mixed": sub_record(pm),
                          "mixed_speedup": pp["ms_per_step"] / pm["ms_per_step"]}
         res["screen"] = sub_record(run_screen(args, ctx))
+        res["compat"] = run_compat(args, ctx)
     if not args.no_cpu and args.workload == "metric":
         try:
             cb = cpu_baseline(tb, V, c["k"])
@@ -585,6 +596,74 @@ def run_polish(args, form=None, ctx=None):
     return res
 
 
+def compat_inputs(tb, n_peaks=1919, n_zeros=100_000, seed=0):
+    """Reference-format inputs of the published compute_peak_AB size
+    (search/results/path1_report.txt:34): the first n_zeros zeros of the synthetic table as
+    (alpha, gamma, ghi, glo, psi) words and n_peaks candidates as (x, dy0) with sub-grid offsets
+    of the size the reference's CLUSTER lines carry (|dy0| <= 4e-3).  Returns (sub-table,
+    argument tuple after n_peaks/n_zeros, V of the points x / 2^30 + dy0 as 64.64 words)."""
+    import numpy as np
+    from mertens_bound_amd import formats, synth
+    c = synth.W65536x2325572
+    sub = type(tb)(tb.G[:n_zeros], tb.gamma[:n_zeros], tb.w[:n_zeros])
+    rw = ref_words_fast(sub)
+    V = synth.make_candidates(n_peaks, c["v_lo"], c["v_hi"], seed + 1919)
+    xs, dy = formats.candidates_to_x_dy(V)
+    dy = dy + np.random.default_rng(seed).uniform(-4e-3, 4e-3, n_peaks)
+    xlo, xhi = formats.x_words(xs)
+    Vd = synth.candidates_from_x_dy(xs, dy)
+    return sub, (xlo, xhi, dy, rw["alpha"], rw["gamma"], rw["ghi_lo"], rw["ghi_hi"],
+                 rw["glo_lo"], rw["glo_hi"], rw["psi_lo"], rw["psi_hi"]), Vd
+
+
+def run_compat(args, ctx=None):
+    """The reference drivers' ctypes sequence on libk2eval.so at the published size: one
+    compute_peak_AB of 1,919 peaks x 100,000 zeros (A and B, 3.07 GB, back in host memory), then
+    polish_peaks_with_weights per weight vector for the five vectors of
+    search/eval_tech1_600k.py:243-267.  The reference took 0.737 s for that compute_peak_AB
+    (search/results/path1_report.txt:34, 16 threads)."""
+    import numpy as np
+    import torch
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import pipeline
+    tb, t = full_table(ctx)
+    sub, a, Vd = compat_inputs(tb)
+    npk, nz = a[0].shape[0], sub.n
+    A = np.empty((npk, nz)); B = np.empty((npk, nz))
+    reps = max(1, min(args.steps, 3))
+    ev.compute_peak_AB(npk, *a[:3], nz, *a[3:], A, B)          # first call: buffers and stream
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ev.compute_peak_AB(npk, *a[:3], nz, *a[3:], A, B)
+    cpab_s = (time.perf_counter() - t0) / reps
+    names, wt = pipeline.reference_weight_vectors(sub.gamma)
+    hu = np.empty(npk); hp = np.empty(npk); dd = np.empty(npk)
+    t0 = time.perf_counter()
+    for m in range(len(names)):
+        ev.polish_peaks_with_weights(npk, nz, A, B, sub.gamma, wt[m], 2.5e-4, 40, hu, hp, dd)
+    pol_s = time.perf_counter() - t0
+    # A/B rows weighted by the Gaussian against the fused evaluator at the same points (the
+    # reference words round gamma/2pi and arg w differently: agreement to about 1e-11)
+    k = 1.5e-10
+    wk = np.exp(-k * sub.gamma ** 2)
+    t2 = ev.K2Table(sub)
+    K, dK, _ = t2.eval(Vd[:64], k, flags=ev.VALUE | ev.D1)
+    err = float(np.max(np.abs(A[:64] @ wk - K)))
+    res = {"value": npk * nz / cpab_s, "unit": "candidate-zero evals/s (A, B in host memory)",
+           "compute_peak_AB_s": cpab_s, "reference_published_s": 0.737,
+           "speedup_vs_published": 0.737 / cpab_s,
+           "polish_five_vectors_s": pol_s,
+           "config": {"workload": f"compute_peak_AB {npk:,} peaks x {nz:,} zeros + "
+                                  f"polish_peaks_with_weights x {len(names)} weight vectors",
+                      "reference": "search/results/path1_report.txt:34"},
+           "accuracy": {"max_abs_dK2_rowsum_vs_fused_evaluator": err, "sample": 64,
+                        "bar": 1e-9, "ok": err <= 1e-9},
+           "note": "A and B (16 B per evaluation) cross PCIe into pageable host memory: the "
+                   "time is the transfer's, not the kernel's"}
+    del A, B
+    return res
+
+
 def sub_record(res, keep=("value", "unit", "ms_per_step", "dtype", "config", "accuracy",
                           "roofline", "per_request_loop", "polish", "screen", "wall_s")):
     """The fields of a workload's own line that go into the default line as a sub-record."""
@@ -736,7 +815,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["metric", "mixed", "batched", "polish", "small",
-                                           "enclose", "weights", "screen", "copy"],
+                                           "enclose", "weights", "screen", "compat", "copy"],
                     default="metric")
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
     ap.add_argument("--n-cand", type=int, default=0)
@@ -764,6 +843,8 @@ def main():
         res = run_weights(args)
     elif args.workload == "screen":
         res = run_screen(args)
+    elif args.workload == "compat":
+        res = run_compat(args)
     else:
         res = run_copy(args)
     print(json.dumps(res))

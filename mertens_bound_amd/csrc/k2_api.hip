@@ -852,6 +852,47 @@ extern "C" int k2_kernel_time(k2_table* t, double* ms, int64_t* launches) {
 }
 
 // ---- reference signatures ---------------------------------------------------------------------
+// State of the reference-signature entry points per device: one stream and one device buffer,
+// kept across calls (the buffer grows to the largest chunk seen), so that a driver's call loop
+// allocates and creates nothing after its first call.  The reference functions hold no state and
+// are reentrant; here concurrent calls on one device take turns on the context's mutex.
+#define K2_COMPAT_CHUNK ((size_t)256 << 20)   // bytes of A/B rows per chunk on the device
+
+struct CompatCtx {
+    hipStream_t st = nullptr;
+    char* buf = nullptr;
+    size_t cap = 0;
+    std::mutex mu;
+};
+
+static CompatCtx* compat_ctx(hipError_t* err) {
+    static std::mutex g_mu;
+    static CompatCtx* g_ctx[K2_MAX_DEVICES] = {};
+    int dev = 0;
+    *err = hipGetDevice(&dev);
+    if (*err != hipSuccess) return nullptr;
+    if (dev < 0 || dev >= K2_MAX_DEVICES) { *err = hipErrorInvalidDevice; return nullptr; }
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ctx[dev]) {
+        CompatCtx* c = new CompatCtx();
+        *err = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+        if (*err != hipSuccess) { delete c; return nullptr; }
+        g_ctx[dev] = c;
+    }
+    return g_ctx[dev];
+}
+
+static hipError_t compat_reserve(CompatCtx* c, size_t need) {
+    if (c->cap >= need) return hipSuccess;
+    if (c->buf) (void)hipFree(c->buf);
+    c->buf = nullptr;
+    c->cap = 0;
+    hipError_t e = hipMalloc((void**)&c->buf, need);
+    if (e == hipSuccess) c->cap = need;
+    return e;
+}
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 static void fill_nan(double* p, size_t n) {
     if (!p) return;
@@ -866,43 +907,46 @@ extern "C" void compute_peak_AB(int n_peaks, const uint64_t* x_lo_arr, const uin
                                 double* out_B) {
     if (n_peaks <= 0 || n_zeros <= 0) return;
     const size_t np = (size_t)n_peaks, nz = (size_t)n_zeros, nab = np * nz;
-    if (n_peaks > 65535) {
-        fail(K2_E_ARG, "compute_peak_AB: n_peaks > 65535");
-        fill_nan(out_A, nab); fill_nan(out_B, nab);
-        return;
-    }
-    // device layout: x_lo x_hi (np) | 6 zero words (nz each) | dy0 (np) | alpha gamma (nz) | A B
-    const size_t bytes = np * 8 * 3 + nz * 8 * 8 + nab * 16;
-    char* d = nullptr;
-    hipStream_t st = nullptr;
-    hipError_t e = hipMalloc((void**)&d, bytes);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    if (e == hipSuccess) {
-        uint64_t* xl = (uint64_t*)d;
-        uint64_t* xh = xl + np;
-        uint64_t* zw = xh + np;
-        double* ddy = (double*)(zw + 6 * nz);
-        double* dal = ddy + np;
+    // peaks in chunks of at most K2_COMPAT_CHUNK bytes of A/B (at least one row) and 65,535 rows
+    // (the grid's second dimension); each chunk's rows are copied back before the next is formed
+    const size_t rows = std::max<size_t>(1, std::min<size_t>({np, 65535, K2_COMPAT_CHUNK / (16 * nz)}));
+    // device layout: 6 zero words, alpha, gamma (nz each) | x_lo, x_hi, dy0 (rows each) | A B
+    const size_t off_x = align256(nz * 8 * 8), off_ab = off_x + align256(rows * 8 * 3);
+    hipError_t e = hipSuccess;
+    CompatCtx* c = compat_ctx(&e);
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        hipStream_t st = c->st;
+        e = compat_reserve(c, off_ab + rows * nz * 16);
+        uint64_t* zw = (uint64_t*)c->buf;
+        double* dal = (double*)(zw + 6 * nz);
         double* dga = dal + nz;
-        double* dA = dga + nz;
-        double* dB = dA + nab;
+        uint64_t* xl = (uint64_t*)(c->buf + off_x);
+        uint64_t* xh = xl + rows;
+        double* ddy = (double*)(xh + rows);
+        double* dA = (double*)(c->buf + off_ab);
+        double* dB = dA + rows * nz;
         const uint64_t* hs[6] = {ghi_lo, ghi_hi, glo_lo, glo_hi, psi_lo, psi_hi};
-        e = hipMemcpyAsync(xl, x_lo_arr, np * 8, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(xh, x_hi_arr, np * 8, hipMemcpyHostToDevice, st);
         for (int q = 0; q < 6 && e == hipSuccess; ++q)
             e = hipMemcpyAsync(zw + q * nz, hs[q], nz * 8, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(ddy, dy0_arr, np * 8, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(dal, alpha, nz * 8, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(dga, gamma, nz * 8, hipMemcpyHostToDevice, st);
         const uint64_t* ptrs[8] = {xl, xh, zw, zw + nz, zw + 2 * nz, zw + 3 * nz, zw + 4 * nz,
                                    zw + 5 * nz};
-        if (e == hipSuccess) e = launch_ref_peak_ab(n_peaks, ptrs, ddy, n_zeros, dal, dga, dA, dB, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(out_A, dA, nab * 8, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(out_B, dB, nab * 8, hipMemcpyDeviceToHost, st);
+        for (size_t p0 = 0; p0 < np && e == hipSuccess; p0 += rows) {
+            const size_t r = std::min(rows, np - p0);
+            e = hipMemcpyAsync(xl, x_lo_arr + p0, r * 8, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(xh, x_hi_arr + p0, r * 8, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(ddy, dy0_arr + p0, r * 8, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = launch_ref_peak_ab((int)r, ptrs, ddy, n_zeros, dal, dga, dA, dB, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(out_A + p0 * nz, dA, r * nz * 8, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(out_B + p0 * nz, dB, r * nz * 8, hipMemcpyDeviceToHost, st);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
-    if (st) (void)hipStreamDestroy(st);
-    (void)hipFree(d);
     if (e != hipSuccess) {
         fail(K2_E_HIP, std::string("compute_peak_AB: ") + hipGetErrorString(e));
         fill_nan(out_A, nab); fill_nan(out_B, nab);
@@ -915,33 +959,42 @@ extern "C" void polish_peaks_with_weights(int n_peaks, int n_zeros, const double
                                           double* out_h_unpol, double* out_h_pol,
                                           double* out_ddy_pol) {
     if (n_peaks <= 0) return;
-    const size_t np = (size_t)n_peaks, nz = (size_t)std::max(n_zeros, 0), nab = np * nz;
-    const size_t bytes = nab * 16 + nz * 16 + np * 24 + 64;
-    char* d = nullptr;
-    hipStream_t st = nullptr;
-    hipError_t e = hipMalloc((void**)&d, bytes);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    if (e == hipSuccess) {
-        double* dA = (double*)d;
-        double* dB = dA + nab;
-        double* dg = dB + nab;
+    const size_t np = (size_t)n_peaks, nz = (size_t)std::max(n_zeros, 0);
+    // peaks in chunks of at most K2_COMPAT_CHUNK bytes of A/B rows (at least one row)
+    const size_t rows = nz ? std::max<size_t>(1, std::min<size_t>(np, K2_COMPAT_CHUNK / (16 * nz)))
+                           : np;
+    // device layout: gamma, w (nz each) | h_unpol, h_pol, ddy (rows each) | A B (rows x nz)
+    const size_t off_o = align256(nz * 16), off_ab = off_o + align256(rows * 24);
+    hipError_t e = hipSuccess;
+    CompatCtx* c = compat_ctx(&e);
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        hipStream_t st = c->st;
+        e = compat_reserve(c, off_ab + rows * nz * 16 + 64);
+        double* dg = (double*)c->buf;
         double* dw = dg + nz;
-        double* o = dw + nz;
-        if (nab) e = hipMemcpyAsync(dA, A, nab * 8, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess && nab) e = hipMemcpyAsync(dB, B, nab * 8, hipMemcpyHostToDevice, st);
+        double* o = (double*)(c->buf + off_o);
+        double* dA = (double*)(c->buf + off_ab);
+        double* dB = dA + rows * nz;
         if (e == hipSuccess && nz) e = hipMemcpyAsync(dg, gamma, nz * 8, hipMemcpyHostToDevice, st);
         if (e == hipSuccess && nz) e = hipMemcpyAsync(dw, w, nz * 8, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = launch_ref_polish(n_peaks, (int)nz, dA, dB, dg, dw, grid_half_width, grid_steps, o,
-                                  o + np, o + 2 * np, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(out_h_unpol, o, np * 8, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(out_h_pol, o + np, np * 8, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(out_ddy_pol, o + 2 * np, np * 8, hipMemcpyDeviceToHost, st);
+        for (size_t p0 = 0; p0 < np && e == hipSuccess; p0 += rows) {
+            const size_t r = std::min(rows, np - p0);
+            if (nz) e = hipMemcpyAsync(dA, A + p0 * nz, r * nz * 8, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess && nz)
+                e = hipMemcpyAsync(dB, B + p0 * nz, r * nz * 8, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = launch_ref_polish((int)r, (int)nz, dA, dB, dg, dw, grid_half_width, grid_steps,
+                                      o, o + r, o + 2 * r, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(out_h_unpol + p0, o, r * 8, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(out_h_pol + p0, o + r, r * 8, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(out_ddy_pol + p0, o + 2 * r, r * 8, hipMemcpyDeviceToHost, st);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
-    if (st) (void)hipStreamDestroy(st);
-    (void)hipFree(d);
     if (e != hipSuccess) {
         fail(K2_E_HIP, std::string("polish_peaks_with_weights: ") + hipGetErrorString(e));
         fill_nan(out_h_unpol, np); fill_nan(out_h_pol, np); fill_nan(out_ddy_pol, np);

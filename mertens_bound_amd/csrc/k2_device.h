@@ -430,8 +430,14 @@ __device__ __forceinline__ void sincos_window_f(uint32_t hi, uint32_t lo, float 
 // ---- general FP64 sincos (|x| < 2^20 pi/2) for the reference-format entry points -----------
 // Cody-Waite reduction by pi/2 with a three-part constant (W. J. Cody and W. Waite,
 // "Software Manual for the Elementary Functions", 1980).  PIO2_1 and PIO2_2 carry 33 bits each,
-// so n * PIO2_1 and n * PIO2_2 are exact for |n| < 2^20.
+// so n * PIO2_1 and n * PIO2_2 are exact for |n| < 2^20, i.e. |x| < 2^20 pi/2 ~ 1.65e6; larger
+// arguments (none in the reference's use: |theta| <= 2 pi + gamma |dy0| there) go to the device
+// math library's sincos, which reduces any double.
 __device__ __forceinline__ void sincos_general(double x, double& s, double& c) {
+    if (!(fabs(x) < 1.6e6)) {
+        ::sincos(x, &s, &c);
+        return;
+    }
     const double PIO2_1 = 1.5707963267341256e+00;   // first 33 bits of pi/2
     const double PIO2_2 = 6.0771005063039660e-11;   // next 33 bits
     const double PIO2_3 = 2.0222662487959506e-21;   // remainder

@@ -5,18 +5,25 @@
 // 2 pi G_j / 2^110 exactly, w_j the stored complex128 values, k and v as given (v exact in
 // 64.64).  The result is a midpoint K and radius R with |K - K2_exact| <= R.
 //
-// Error budget (u = 2^-53; x_j = k g_j^2; per zero, independent of the candidate):
-//   g_j     from G_j in FP64 (two conversions, one sum, one product)         rel <= 2.5 u
-//   x_j     k g g                                                             rel <= 6 u
+// Error budget (u = 2^-53 the unit roundoff, every rounding at most u relative; x_j = k g_j^2;
+// per zero, independent of the candidate; first order, higher orders in the final 1.01):
+//   g_j     s = fl(G_hi) + G_lo 2^-64 (conversion u, sum u; the low word's conversion and the
+//           power-of-two scaling add < 2^-63 u), times C = fl(2 pi / 2^46) (0.35 u: the rounding
+//           of 2 pi), one product (u)                                         rel <= 3.5 u
+//   x_j     (k g) g: two products and 2 x g's error                          rel <= 9 u
+//           -> e^{-x} moves by e^{-x} x 9 u
 //   e^{-x}  exp_neg_bounded                                                   rel <= 2.5 u
-//   2 e^{-x} w  -> a_j (one product per component)                            |a - A| <= |A| (3 + 6.1 x) u
+//   a_j     2 e^{-x} w (the factor 2 exact, one product per component)
+//                                                            |a - A| <= |A| (3.5 + 9 x) u
+//   b_j     g a (one product per component, g's 3.5 u)       |b - B| <= |B| (8 + 9 x) u
 //   phase   W = floor(G V / 2^110) mod 2^64 exact; the remainder drops W bits [0,3) and the
 //           fraction below W: < 2 pi 2^-61 rad = 2.8e-18 (0.03 u)
 //   sin/cos sincos_tab64_at: table rounding 0.5 u, remainder polynomial 1.2 u, combine 0.55 u,
 //           the dropped bits 0.03 u -> |S - sin|, |C - cos| <= 2.3 u (bounded here by 3 u)
-//   term    fma(a.x, C, -a.y S): sqrt(2) |a| 3u + 0.5 |a| u + 0.5 |a| u = 5.25 |a| u
-//   =>      |term_j - T_j| <= |a_j| (9 + 6.5 x_j) u                  (K2)
-//           |term'_j - T'_j| <= g_j |a_j| (12 + 6.5 x_j) u           (K2', b = g a adds 3.5 u)
+//   term    fma(a.x, C, -a.y S): sqrt(2) |a| 3 u for S, C, then two roundings of at most |a| u
+//           each: 6.25 |a| u
+//   =>      |term_j - T_j| <= |A_j| (10 + 9 x_j) u                  (K2: 6.25 + 3.5 + 9 x)
+//           |term'_j - T'_j| <= g_j |A_j| (14.5 + 9 x_j) u         (K2': 6.25 + 8 + 9 x)
 //   sums    TwoSum (Knuth) per lane and across splits: every addition's rounding error is kept
 //           exactly in a second word; only the second word's own sum rounds, which adds at most
 //           (m u)^2 sum|a_j| (m = zeros + 2 splits + 2), and the final hi + lo rounds by u |K|.
@@ -24,7 +31,9 @@
 //   every absolute (subnormal) rounding.  The bound sums are formed in FP64 and the whole radius
 //   is scaled by 1.01, which covers their own rounding (< 1e-9 relative) and |A| vs |a|.
 // The radius is R = 1.01 (u B + (m u)^2 sum|a| + u |K|) + 1e-300 with B = sum_{j<N} |a_j|
-// (9 + 6.5 x_j); the zeros of the table past N contribute at most tail = 1.01 sum_{j>=N} |a_j|.
+// (10 + 9 x_j); the zeros of the table past N contribute at most tail = 1.01 sum_{j>=N} |a_j|.
+// tests/test_enclose_host.py replays g, x, a and b on the host and checks each line's bound
+// against mpmath for x in [0, 708].
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -77,8 +86,8 @@ __global__ __launch_bounds__(256) void k2_enclose_weights_kernel(
             vb = make_double2(g * va.x, g * va.y);
             mag = sqrt(va.x * va.x + va.y * va.y);
             magb = g * mag;
-            m = mag * (9.0 + 6.5 * x);
-            mb = magb * (12.0 + 6.5 * x);
+            m = mag * (10.0 + 9.0 * x);
+            mb = magb * (14.5 + 9.0 * x);
         } else {
             mag = 2.0 * sqrt(ww.x * ww.x + ww.y * ww.y) * 1e-307;
             magb = g * mag;

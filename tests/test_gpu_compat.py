@@ -46,3 +46,48 @@ def test_polish_peaks_with_weights(oracle_lib):
     np.testing.assert_allclose(hu, g["h_unpol"], rtol=0, atol=1e-13)
     np.testing.assert_allclose(hp, g["h_pol"], rtol=0, atol=1e-12)
     np.testing.assert_allclose(dd, g["ddy_pol"], rtol=0, atol=1e-10)
+
+
+def test_compute_peak_AB_many_peaks_and_large_arguments(oracle_lib):
+    """More peaks than one grid dimension holds (70,000 > 65,535: two chunks of rows) with the
+    reference's own call, and sub-grid offsets large enough that the angle leaves the range of
+    the three-part pi/2 reduction (|g dy0| up to 2.6e6): the reference function accepts any int
+    and any double (search/path1_eval_fast.cpp:15-53)."""
+    tb, rw, args = ref_inputs()
+    nz = 64
+    npk = 70_000
+    rng = np.random.default_rng(17)
+    sel = rng.integers(0, args[0].shape[0], npk)
+    dy0 = rng.uniform(-4e-3, 4e-3, npk)
+    dy0[:500] = rng.uniform(-2500.0, 2500.0, 500)            # |theta| far beyond 1.6e6
+    a = [args[0][sel], args[1][sel], dy0] + [x[:nz] for x in args[3:]]
+    A0, B0 = oracle_lib.ref_compute_peak_AB(*a)
+    A = np.empty_like(A0); B = np.empty_like(B0)
+    ev.compute_peak_AB(npk, *a[:3], nz, *a[3:], A, B)
+    th_max = 2 * np.pi + rw["gamma"][None, :nz] * np.abs(dy0)[:, None]
+    tol = 2 * rw["alpha"][None, :nz] * (th_max * 2.3e-16 + 4.5e-16)
+    assert np.all(np.isfinite(A)) and np.all(np.isfinite(B))
+    assert np.all(np.abs(A - A0) <= tol) and np.all(np.abs(B - B0) <= tol)
+    print(f"70,000 peaks: max |dA| {np.abs(A - A0).max():.2e}, large-argument rows "
+          f"{np.abs(A - A0)[:500].max():.2e}")
+
+
+def test_polish_peaks_with_weights_repeated_calls(oracle_lib):
+    """The driver's loop over weight vectors (search/eval_tech1_600k.py:252-267): repeated calls
+    with different weights on the same A/B give what each gives alone (the library keeps its
+    device buffers and stream across calls)."""
+    tb, rw, args = ref_inputs()
+    A0, B0 = _reference(oracle_lib, args)
+    npk, nz = A0.shape
+    outs = []
+    for kk in (3e-9, 1.5e-10, 1e-10, 3e-9):
+        w = np.exp(-kk * rw["gamma"] ** 2)
+        hu = np.empty(npk); hp = np.empty(npk); dd = np.empty(npk)
+        ev.polish_peaks_with_weights(npk, nz, A0, B0, rw["gamma"], w, 2.5e-4, 40, hu, hp, dd)
+        outs.append((hu, hp, dd))
+    for x, y in zip(outs[0], outs[3]):
+        assert np.array_equal(x, y)
+    hu_r, hp_r, dd_r = oracle_lib.ref_polish_peaks_with_weights(
+        A0, B0, rw["gamma"], np.exp(-1.5e-10 * rw["gamma"] ** 2), 2.5e-4, 40)
+    np.testing.assert_allclose(outs[1][0], hu_r, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(outs[1][1], hp_r, rtol=0, atol=1e-10)
