@@ -436,6 +436,7 @@ def run_eval(args, form):
         res["polish"] = {"plain": sub_record(pp), "mixed": sub_record(pm),
                          "mixed_speedup": pp["ms_per_step"] / pm["ms_per_step"]}
         res["screen"] = sub_record(run_screen(args, ctx))
+        res["screen_survivors"] = sub_record(run_screen(args, ctx, peaks=True))
         res["compat"] = run_compat(args, ctx)
     if not args.no_cpu and args.workload == "metric":
         try:
@@ -747,12 +748,39 @@ def run_weights(args):
     return res
 
 
-def run_screen(args, ctx=None):
+def screen_peak_candidates(t, n_cand, seed=0, every=16, sigma=3e-4):
+    """2^24-style screen input with survivors: uniform candidates in [3.4e18, 3.6e18], every
+    `every`-th replaced by a point near one of the 1,024 best of 2^18 uniform candidates on the
+    256-zero stage value (moved to its best shift, then v + N(0, sigma))."""
+    import numpy as np
+    import torch
+    from mertens_bound_amd import autograd as ag
+    from mertens_bound_amd import evaluator as ev
+    from mertens_bound_amd import synth
+    c = synth.W65536x2325572
+    pool = ev.v_tensor(synth.make_candidates(1 << 18, c["v_lo"], c["v_hi"], seed + 5))
+    val, best, _ = t.screen_device(pool, 3e-9, 0.0014, 3, ((64, 0.0), (128, 0.0), (256, 0.0)))
+    top = torch.argsort(-val[:, 2], stable=True)[:1024]
+    peaks = ag.offset_points(pool[top], best[top].to(torch.float64) * 0.0014)
+    V = ev.v_tensor(synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"]))
+    pos = torch.arange(0, n_cand, every, device=V.device)
+    rng = np.random.default_rng(seed)
+    off = torch.tensor(rng.normal(0.0, sigma, pos.numel()), device=V.device)
+    V[pos] = ag.offset_points(peaks[torch.arange(pos.numel(), device=V.device) % 1024], off)
+    torch.cuda.synchronize()
+    return V
+
+
+def run_screen(args, ctx=None, peaks=False):
     """The staged leaf screen (k2_screen_device) of search/gpu_mertens_stream.cu:186-250 with its
     constants: 64/128/256 zeros, thresholds 0.78/0.855/0.892, 7 shifts of 0.0014, on 2^24
     candidates v in [3.4e18, 3.6e18] against the first zeros of the full table (k = 3e-9).  A
     zero evaluation is one candidate against one zero with all 7 shifted terms; a candidate that
-    fails a stage stops there."""
+    fails a stage stops there.  peaks=True: the survivor case -- every 16th candidate near a peak
+    of this table's 256-zero sum and the thresholds halved (0.39/0.4275/0.446), so that a known
+    share of the candidates reaches the later stages; per-stage times from calls with the first
+    one, two and three stages, and the survivors' stage values checked bit for bit against a run
+    in which every candidate takes every stage."""
     import numpy as np
     import torch
     from mertens_bound_amd import evaluator as ev
@@ -760,10 +788,15 @@ def run_screen(args, ctx=None):
     c = synth.W65536x2325572
     tb, t = full_table(ctx)
     n_cand = args.n_cand if (args.n_cand and ctx is None) else (1 << 24)
-    V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
-    Vd = ev.v_tensor(V)
     k, d, m = 3e-9, 0.0014, 3
-    stages = pipeline.REFERENCE_SCREEN
+    if peaks:
+        Vd = screen_peak_candidates(t, n_cand)
+        V = None
+        stages = tuple((a, b / 2) for a, b in pipeline.REFERENCE_SCREEN)
+    else:
+        V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
+        Vd = ev.v_tensor(V)
+        stages = pipeline.REFERENCE_SCREEN
     state = {}
 
     def step():
@@ -775,16 +808,47 @@ def run_screen(args, ctx=None):
     nz = np.array([a for a, _ in stages])
     reached = (val[:, :len(stages)] >= 0).sum(axis=0)            # candidates reaching each stage
     zero_evals = float(np.sum(reached * np.diff(np.concatenate([[0], nz]))))
+    name = "screen_peaks" if peaks else "screen"
     res = base_record(args, "screened candidates/sec (64/128/256-zero staged screen, 7 shifts)",
                       n_cand / (ms * 1e-3), "candidates/s", ms, "u32+f32",
-                      {"workload": f"leaf screen, {n_cand:,} candidates, stages "
+                      {"workload": f"leaf screen, {n_cand:,} candidates"
+                                   f"{' (every 16th near a peak)' if peaks else ''}, stages "
                                    f"{'/'.join(str(a) for a, _ in stages)} zeros, thresholds "
                                    f"{'/'.join(str(b) for _, b in stages)}, {2 * m + 1} shifts "
-                                   f"of {d}, k={k}", "form": "screen", "n_cand": n_cand})
+                                   f"of {d}, k={k}", "form": name, "n_cand": n_cand})
     res["screen"] = {"reached_per_stage": [int(x) for x in reached], "passed": passed,
                      "zero_evals_per_step": zero_evals,
                      "zero_evals_per_s": zero_evals / (ms * 1e-3)}
-    g = None if args.no_check else golden_npz("screen_sample.npz")
+    if peaks:
+        # per-stage times: calls with the first one and two stages beside the full call (the
+        # later stages recompute the earlier zeros of their survivors)
+        ms_pre = []
+        for ns in range(1, len(stages)):
+            ms_pre.append(timed(lambda: t.screen_device(Vd, k, d, m, stages[:ns]), args.steps,
+                                args.warmup, torch)[0])
+        ms_pre.append(ms)
+        per = []
+        for s_ in range(len(stages)):
+            dt = ms_pre[s_] - (ms_pre[s_ - 1] if s_ else 0.0)
+            ev_s = float(reached[s_]) * nz[s_]             # zeros the stage's launch sums
+            per.append({"stage": s_ + 1, "zeros": int(nz[s_]), "candidates": int(reached[s_]),
+                        "ms": dt, "zero_evals_per_s": ev_s / (dt * 1e-3) if dt > 0 else None})
+        res["screen"]["per_stage"] = per
+        # every candidate through every stage: the survivors' values must be the same bits
+        allv, _, _ = t.screen_device(Vd, k, d, m, tuple((a, 0.0) for a, _ in stages))
+        allv = allv.cpu().numpy()
+        got = val[:, :len(stages)]
+        mask = got >= 0
+        thr = np.array([b for _, b in stages], dtype=np.float32)
+        want_pass = np.all(allv[:, :len(stages)] >= thr[None, :], axis=1)
+        res["accuracy"] = {"survivor_values_equal_all_stage_run":
+                               bool(np.array_equal(got[mask], allv[:, :len(stages)][mask])),
+                           "pass_flags_match_thresholds":
+                               bool(np.array_equal(state["r"][2].cpu().numpy() == 1, want_pass)),
+                           "ok": None}
+        res["accuracy"]["ok"] = bool(res["accuracy"]["survivor_values_equal_all_stage_run"] and
+                                     res["accuracy"]["pass_flags_match_thresholds"])
+    g = None if (args.no_check or peaks) else golden_npz("screen_sample.npz")
     if g is not None and np.array_equal(V[:g["V"].shape[0]], g["V"]):
         n = g["V"].shape[0]
         got = val[:n, :3]
@@ -794,7 +858,7 @@ def run_screen(args, ctx=None):
                            "oracle_sample": "screen_sample.npz (first 512 candidates, FP64 "
                                             "oracle, every stage reached)",
                            "bar": 2e-6, "ok": bool(err.max() <= 2e-6)}
-    res["roofline"] = roofline("screen", zero_evals, ms, n_cand * 40, "screen")
+    res["roofline"] = roofline("screen", zero_evals, ms, n_cand * 40, name)
     res["wall_s"] = wall
     return res
 
@@ -815,7 +879,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["metric", "mixed", "batched", "polish", "small",
-                                           "enclose", "weights", "screen", "compat", "copy"],
+                                           "enclose", "weights", "screen", "screen_peaks", "compat",
+                                           "copy"],
                     default="metric")
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
     ap.add_argument("--n-cand", type=int, default=0)
@@ -843,6 +908,8 @@ def main():
         res = run_weights(args)
     elif args.workload == "screen":
         res = run_screen(args)
+    elif args.workload == "screen_peaks":
+        res = run_screen(args, peaks=True)
     elif args.workload == "compat":
         res = run_compat(args)
     else:

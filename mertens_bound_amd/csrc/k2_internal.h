@@ -125,7 +125,8 @@ struct k2_table {
     void* polish_buf = nullptr;          // compaction: indices, gathered points, their sums
     size_t polish_buf_bytes = 0;
     int64_t polish_evals = 0;            // candidate evaluations of the last polish
-    void* screen_params = nullptr;       // k2_screen_device: the zero records of the last call
+    void* screen_params = nullptr;       // k2_screen_device: zero records, survivor lists
+    size_t screen_bytes = 0;
     size_t polish_cap = 0;
     // enclosure mode (k2_enclose_device), allocated on first use: weights with a stated error
     // bound, per-block bound sums, the final bound sums, split partials (hi, lo pairs)

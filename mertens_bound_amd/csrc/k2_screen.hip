@@ -14,11 +14,25 @@
 // scaled floats, the FP32 weight 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once
 // per call by k2_screen_records_kernel; every lane reads the same record at the same time, so the
 // screen kernel takes them through the scalar cache, and LDS holds only the 16 KiB sin/cos table.
-// A lane that fails a stage stops accumulating; its later stage values stay -1.  The stage zero
-// counts and thresholds are kernel arguments: the call does not synchronise.
+// Each stage is its own launch over the candidates still in play: the first over all candidates
+// (one lane each), the later ones over the survivors of the stage before, which each launch
+// appends to lists in device memory (the lists' order does not matter, since every candidate
+// writes only its own outputs).  A later stage starts its candidates from the first zero again
+// (the earlier stages' sums are recomputed, bit for bit the same) and its waves are full however
+// few candidates survive; the alternative, one launch in which a failed lane idles while its
+// wave's survivors continue, leaves most lanes of the later stages masked off when survivors are
+// rare.  The survivors go to K2_SCREEN_NSUB sub-lists, each with its own counter on its own
+// cache line: in the first stage a wavefront appends to sub-list (wave index mod NSUB) with one
+// atomic, and a later stage gives each sub-list K2_SCREEN_PER_SUB workgroups whose survivors stay
+// in that sub-list (so a sub-list never outgrows its first-stage size).  One counter for all
+// would take one atomic per wavefront at a single address (262,144 of them for 2^24
+// candidates), which serialises.  A candidate's later stage values stay -1 once it fails.  The
+// stage zero counts and thresholds are kernel arguments and the list lengths stay on the
+// device: the call does not synchronise.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -31,6 +45,9 @@ using namespace k2dev;
 #define K2_SCREEN_MAX 1024   // zeros the screen can stage
 #define K2_SCREEN_STAGES 4
 #define K2_SCREEN_MAXM 8     // shifts -M..M, M <= 8
+#define K2_SCREEN_NSUB 256   // survivor sub-lists (and counters) per stage
+#define K2_SCREEN_PER_SUB 8  // workgroups per sub-list in the later stages
+#define K2_SCREEN_CSTRIDE 16 // ints between counters (one 64-byte line each)
 
 // stage zero counts and thresholds, passed by value (kernel arguments)
 struct ScreenStages {
@@ -56,8 +73,10 @@ struct ScreenRec {
 template <int M>
 __global__ __launch_bounds__(256) void k2_screen_records_kernel(
     const uint4* __restrict__ zg, const double* __restrict__ gamma, const double2* __restrict__ w,
-    double k, double d, int n_zeros, ScreenRec<M>* __restrict__ rec) {
+    double k, double d, int n_zeros, ScreenRec<M>* __restrict__ rec, int* __restrict__ counts) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    // survivor counts of this call's stages (the launch has at least this many threads)
+    if (j < K2_SCREEN_STAGES * K2_SCREEN_NSUB) counts[j * K2_SCREEN_CSTRIDE] = 0;
     if (j >= n_zeros) return;
     const double g = gamma[j];
     const double f = 2.0 * exp(-k * g * g);
@@ -76,30 +95,47 @@ __global__ __launch_bounds__(256) void k2_screen_records_kernel(
     rec[j] = z;
 }
 
+// Stage s of the screen.  s = 0: candidate i = lane i of the grid, for i < n_cand.  s > 0:
+// workgroup b walks sub-list b mod NSUB of the stage before, entries (b / NSUB) 256 + lane in
+// steps of PER_SUB 256.  Each candidate sums zeros [0, nz[s]) at all shifts, writes its stage-s
+// value, and either stops (pass 0, later values -1), passes the last stage (pass 1) or joins its
+// sub-list for stage s + 1 (sub-list q holds entries [q cap, q cap + counts[q])).  The best shift
+// is that of its last stage.
 template <int M>
 __global__ __launch_bounds__(256) void k2_screen_kernel(
     const ScreenRec<M>* __restrict__ rec, const uint4* __restrict__ cand, int n_cand,
-    int n_stages, ScreenStages sp,
-    const Tab32* __restrict__ tab, float* __restrict__ out_val, int* __restrict__ out_shift,
-    int* __restrict__ out_pass) {
+    int n_stages, ScreenStages sp, int s, const int* __restrict__ list_in,
+    const int* __restrict__ count_in, int* __restrict__ list_out, int* __restrict__ count_out,
+    int cap, const Tab32* __restrict__ tab, float* __restrict__ out_val,
+    int* __restrict__ out_shift, int* __restrict__ out_pass) {
     __shared__ Tab32 s_tab[K2_TAB32];
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB32; i += blockDim.x) s_tab[i] = tab[i];
     __syncthreads();
-    const int c = blockIdx.x * blockDim.x + tid;
-    if (c >= n_cand) return;
-    const uint4 cv = cand[c];
-    const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
-    // the lane's side of the window: V's limbs as floats (unscaled) and column 5's packed limbs
-    const float4 vf = make_float4((float)v.x0, (float)v.x1, (float)v.x2, (float)v.x3);
-    const uint32_t vp = pack_lo16(v.x2, v.x3);
-    float acc[2 * M + 1];
+    int q, i0, step, n;
+    if (s == 0) {
+        i0 = blockIdx.x * blockDim.x + tid;
+        step = gridDim.x * blockDim.x;
+        n = n_cand;
+        q = 0;   // per wavefront below
+    } else {
+        q = blockIdx.x % K2_SCREEN_NSUB;
+        i0 = (blockIdx.x / K2_SCREEN_NSUB) * blockDim.x + tid;
+        step = K2_SCREEN_PER_SUB * blockDim.x;
+        n = count_in[q * K2_SCREEN_CSTRIDE];
+    }
+    const int nz = sp.nz[s];
+    for (int i = i0; i < n; i += step) {
+        const int c = s == 0 ? i : list_in[(size_t)q * cap + i];
+        const uint4 cv = cand[c];
+        const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
+        // the lane's side of the window: V's limbs as floats (unscaled) and column 5's packed limbs
+        const float4 vf = make_float4((float)v.x0, (float)v.x1, (float)v.x2, (float)v.x3);
+        const uint32_t vp = pack_lo16(v.x2, v.x3);
+        float acc[2 * M + 1];
 #pragma unroll
-    for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
-    int j = 0, best = M, passed = 1;
-    for (int s = 0; s < n_stages; ++s) {
-        const int nz = sp.nz[s];
-        for (; j < nz; ++j) {
+        for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
+        for (int j = 0; j < nz; ++j) {
             const ScreenRec<M> z = rec[j];   // the same j in every lane
             uint32_t hi, lo;
             float t3;
@@ -119,51 +155,80 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
             }
         }
         float mv = -1.0f;
+        int best = M;
 #pragma unroll
         for (int m = 0; m < 2 * M + 1; ++m) {
             const float a = fabsf(acc[m]);
             if (a > mv) { mv = a; best = m; }
         }
-        out_val[(size_t)c * K2_SCREEN_STAGES + s] = mv;
-        if (mv < sp.thr[s]) {
-            passed = 0;
-            for (int t = s + 1; t < K2_SCREEN_STAGES; ++t)
-                out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
-            break;
+        float* val = out_val + (size_t)c * K2_SCREEN_STAGES;
+        val[s] = mv;
+        const bool pass = mv >= sp.thr[s];
+        const bool next = pass && s + 1 < n_stages;
+        if (!next) {
+            for (int t = s + 1; t < K2_SCREEN_STAGES; ++t) val[t] = -1.0f;
+            out_shift[c] = best - M;
+            out_pass[c] = pass ? 1 : 0;
+        }
+        // survivors join a sub-list of the next stage: one atomic per wavefront
+        const uint64_t bal = __ballot(next);
+        if (bal) {
+            const int qo = s == 0 ? (i >> 6) % K2_SCREEN_NSUB : q;
+            const int lane = __lane_id();
+            const int leader = __ffsll((unsigned long long)bal) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(count_out + qo * K2_SCREEN_CSTRIDE, __popcll(bal));
+            base = __shfl(base, leader, 64);
+            if (next) list_out[(size_t)qo * cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = c;
         }
     }
-    for (int t = n_stages; t < K2_SCREEN_STAGES; ++t)
-        out_val[(size_t)c * K2_SCREEN_STAGES + t] = -1.0f;
-    out_shift[c] = best - M;
-    out_pass[c] = passed;
 }
 
 namespace k2impl {
 
 // records of n_zeros zeros into rec (room for K2_SCREEN_MAX records of ScreenRec<M>), then the
 // screen
+// the screen's device buffer: the zero records, the stages' survivor counts, two survivor lists
+// (stage s reads list (s - 1) & 1 and writes list s & 1)
+struct ScreenBufs {
+    void* rec;
+    int* counts;    // [stage][sub-list], K2_SCREEN_CSTRIDE ints apart
+    int* list[2];   // [sub-list][cap]
+    int cap;        // entries per sub-list
+};
+
+// records of n_zeros zeros into rec (room for K2_SCREEN_MAX records of ScreenRec<M>), then one
+// launch per stage
 template <int M>
 static hipError_t launch_screen_m(const uint4* zg, const double* gamma,
-                                  const double2* w, double k, double d, int n_zeros, void* rec,
-                                  const uint4* cand, int n_cand, int n_stages, ScreenStages sp,
-                                  const Tab32* tab, float* val, int* shift, int* pass,
-                                  hipStream_t st) {
-    hipLaunchKernelGGL((k2_screen_records_kernel<M>), dim3((n_zeros + 255) / 256), dim3(256), 0,
-                       st, zg, gamma, w, k, d, n_zeros, (ScreenRec<M>*)rec);
+                                  const double2* w, double k, double d, int n_zeros,
+                                  const ScreenBufs& B, const uint4* cand, int n_cand,
+                                  int n_stages, ScreenStages sp, const Tab32* tab, float* val,
+                                  int* shift, int* pass, hipStream_t st) {
+    // at least one thread per survivor counter (they are reset here)
+    const int rgrid = std::max((n_zeros + 255) / 256, K2_SCREEN_STAGES * K2_SCREEN_NSUB / 256);
+    hipLaunchKernelGGL((k2_screen_records_kernel<M>), dim3(rgrid), dim3(256), 0, st, zg, gamma,
+                       w, k, d, n_zeros, (ScreenRec<M>*)B.rec, B.counts);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st,
-                       (const ScreenRec<M>*)rec, cand, n_cand, n_stages, sp, tab, val, shift,
-                       pass);
-    return hipGetLastError();
+    const int cs = K2_SCREEN_NSUB * K2_SCREEN_CSTRIDE;   // ints of one stage's counters
+    for (int s = 0; s < n_stages && e == hipSuccess; ++s) {
+        const int grid = s == 0 ? (n_cand + 255) / 256 : K2_SCREEN_NSUB * K2_SCREEN_PER_SUB;
+        hipLaunchKernelGGL((k2_screen_kernel<M>), dim3(grid), dim3(256), 0, st,
+                           (const ScreenRec<M>*)B.rec, cand, n_cand, n_stages, sp, s,
+                           s > 0 ? B.list[(s - 1) & 1] : nullptr,
+                           s > 0 ? B.counts + (s - 1) * cs : nullptr, B.list[s & 1],
+                           B.counts + s * cs, B.cap, tab, val, shift, pass);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 hipError_t launch_screen(int M, const uint4* zg, const double* gamma,
-                         const double2* w, double k, double d, int n_zeros, void* rec,
+                         const double2* w, double k, double d, int n_zeros, const ScreenBufs& B,
                          const uint4* cand, int n_cand, int n_stages, ScreenStages sp,
                          const Tab32* tab, float* val, int* shift, int* pass, hipStream_t st) {
     switch (M) {
-#define K2_M(X) case X: return launch_screen_m<X>(zg, gamma, w, k, d, n_zeros, rec, cand, \
+#define K2_M(X) case X: return launch_screen_m<X>(zg, gamma, w, k, d, n_zeros, B, cand, \
                                                   n_cand, n_stages, sp, tab, val, shift, pass, \
                                                   st);
         K2_M(0) K2_M(1) K2_M(2) K2_M(3) K2_M(4) K2_M(5) K2_M(6) K2_M(7) K2_M(8)
@@ -205,20 +270,34 @@ extern "C" int k2_screen_device(k2_table* t, uint32_t n_cand, const uint64_t* V_
     hipStream_t st = (hipStream_t)stream;
     CallOrder order(t, st);   // the zero records are shared with earlier calls on the table
     if (e == hipSuccess) e = order.err();
-    // the zero records live in a device buffer owned by the table; the stage parameters travel
-    // as kernel arguments, so the call stays asynchronous
-    if (e == hipSuccess && t->screen_params == nullptr)
-        e = hipMalloc((void**)&t->screen_params, K2_SCREEN_MAX * K2_SCREEN_REC_MAX);
+    // the zero records, survivor counts and lists live in a device buffer owned by the table
+    // (grown to the largest call); the stage parameters travel as kernel arguments, so the call
+    // stays asynchronous
+    // a first-stage sub-list takes the survivors of every NSUB-th wavefront: at most cap entries
+    const int cap = (int)(((size_t)n_cand + 64 * K2_SCREEN_NSUB - 1) / (64 * K2_SCREEN_NSUB) * 64);
+    const size_t off_cnt = K2_SCREEN_MAX * K2_SCREEN_REC_MAX;
+    const size_t off_l0 = off_cnt + (size_t)K2_SCREEN_STAGES * K2_SCREEN_NSUB * K2_SCREEN_CSTRIDE * 4;
+    const size_t list_bytes = (size_t)K2_SCREEN_NSUB * cap * 4;
+    const size_t need = off_l0 + 2 * list_bytes;
+    if (e == hipSuccess && need > t->screen_bytes) {
+        (void)hipFree(t->screen_params);
+        t->screen_params = nullptr;
+        t->screen_bytes = 0;
+        e = hipMalloc((void**)&t->screen_params, need);
+        if (e == hipSuccess) t->screen_bytes = need;
+    }
     ScreenStages sp{};
     for (int s = 0; s < n_stages; ++s) {
         sp.nz[s] = stage_zeros[s];
         sp.thr[s] = stage_thresholds[s];
     }
+    char* base = (char*)t->screen_params;
+    const ScreenBufs B{base, (int*)(base + off_cnt),
+                       {(int*)(base + off_l0), (int*)(base + off_l0 + list_bytes)}, cap};
     if (e == hipSuccess)
         e = launch_screen(n_shift, t->g, t->gamma, (const double2*)t->w, k, shift_step,
-                          stage_zeros[n_stages - 1], t->screen_params, (const uint4*)V_dev,
-                          (int)n_cand, n_stages, sp, t->tab32, val_dev, best_shift_dev, pass_dev,
-                          st);
+                          stage_zeros[n_stages - 1], B, (const uint4*)V_dev, (int)n_cand,
+                          n_stages, sp, t->tab32, val_dev, best_shift_dev, pass_dev, st);
     if (e != hipSuccess)
         return set_error(K2_E_HIP, std::string("k2_screen_device: ") + hipGetErrorString(e));
     return K2_OK;

@@ -82,6 +82,45 @@ def test_screen_calls_queue_without_host_sync(setup):
     assert not torch.equal(ra[0], rb[0])
 
 
+def test_screen_sparse_survivors_staged(setup):
+    """Later stages run over compacted survivor lists (appended with atomics, in any order): with
+    thresholds that stop most candidates, every output equals the single-stage-at-a-time
+    definition -- a candidate's stage values are those of a run where every candidate takes every
+    stage, it reaches stage s exactly when the values before reach their thresholds, and two
+    calls give the same bits."""
+    tb, t, torch = setup
+    V = synth.make_candidates(20_000, 3.4e18, 3.6e18, 25)
+    Vt = ev.v_tensor(V)
+    k, d = 3e-9, 0.0014
+    allv, allb, _ = t.screen_device(Vt, k, d, 3, ((64, 0.0), (128, 0.0), (256, 0.0)))
+    allv = allv.cpu().numpy()
+    # stage 1 stops 90% of the candidates; each later threshold is the median of the values of
+    # the candidates that reach it
+    thr = np.zeros(3, dtype=np.float32)
+    alive = np.ones(len(V), dtype=bool)
+    for s, q in enumerate((0.9, 0.5, 0.5)):
+        thr[s] = np.quantile(allv[alive, s], q)
+        alive &= allv[:, s] >= thr[s]
+    stages = tuple((n, float(x)) for n, x in zip((64, 128, 256), thr))
+    r1 = t.screen_device(Vt, k, d, 3, stages)
+    r2 = t.screen_device(Vt, k, d, 3, stages)
+    torch.cuda.synchronize()
+    for x, y in zip(r1, r2):
+        assert torch.equal(x, y)
+    val, best, ok = (x.cpu().numpy() for x in r1)
+    reach = np.ones(len(V), dtype=bool)
+    for s in range(3):
+        assert np.array_equal(val[reach, s], allv[reach, s])
+        assert np.all(val[~reach, s] == -1)
+        reach = reach & (allv[:, s] >= thr[s])
+    assert np.array_equal(ok == 1, reach)
+    print(f"staged screen: {np.sum(val[:, 1] >= 0)} of {len(V)} reach stage 2, "
+          f"{np.sum(val[:, 2] >= 0)} stage 3, {reach.sum()} pass")
+    assert 0 < reach.sum() < np.sum(val[:, 1] >= 0) < len(V)
+    # the best shift of a candidate that completes every stage is that of its last stage
+    assert np.array_equal(best[reach], allb.cpu().numpy()[reach])
+
+
 def test_grid_then_newton_matches_reference_polish(setup, oracle_lib):
     """Canonical grid + Newton vs the reference's polish_peaks_with_weights on A/B rows of the
     same table (reference-format words, dy0 = 0)."""
