@@ -1,13 +1,21 @@
 // Batched evaluation of uneven requests in one persistent launch, captured in a HIP graph.
 //
 // A request r has its own zero cutoff (the first n_r zeros), Gaussian constant k_r and candidate
-// slice.  The host cuts every request into work items:
+// slice.  The host cuts the requests into work items:
 //   * C items: a tile of 256*NC candidates (NC per lane) x a chunk of zeros, the same body as the
-//     one-launch kernel;
+//     one-launch kernel.  A request's full tiles are its own; the candidates left over from all
+//     requests with the same k are packed together, in order of decreasing cutoff, into shared
+//     tiles (candidates fetched through an index list).  A shared tile's zero range is cut at
+//     every member's cutoff, and a candidate's sum takes only the items below its own cutoff (the
+//     lanes of members whose cutoff has passed still compute, and their slots are never read);
+//     ordered by cutoff, neighbours in a tile have similar cutoffs, so few lanes idle that way;
 //   * Z items: a tile of CZ candidates x a longer chunk of zeros, one zero per lane at a time,
-//     for the candidates left over once a request's full C tiles are taken (a C tile would leave
-//     most of its lanes idle);
-//   * with K2_MIXED, the zeros below the FP64 head and the FP32 tail are separate items.
+//     only for the last shared tile of a k when it holds fewer than c_min_fill candidates.  Each
+//     Z item re-reads its zero chunk once per pass over its candidates, so packing the leftovers
+//     into shared C tiles (each zero read once per 256*NC candidates) is what keeps the launch's
+//     memory traffic near the algorithmic bytes;
+//   * with K2_MIXED, the zeros below the FP64 head and the FP32 tail are separate items (a shared
+//     tile takes the largest head among its members: more FP64, never less).
 // Items are sorted longest first and pulled from an atomic counter by a grid of resident
 // workgroups; each item writes its partial sums to its own slot, and a reduction kernel adds a
 // candidate's slots in zero order, so results do not depend on the execution order.
@@ -28,9 +36,10 @@
 using namespace k2dev;
 
 struct BatchItem {
-    uint32_t flags;     // bit0: Z tile, bit1: FP32, bits 8-9: k slot
+    uint32_t flags;     // bit0: Z tile, bit1: FP32, bit2: candidates through the index list,
+                        // bits 8-9: k slot
     uint32_t z0, z1;
-    uint32_t c_first;   // global index of the tile's first candidate
+    uint32_t c_first;   // global index of the tile's first candidate (or of its first index)
     uint32_t c_count;
     uint32_t pad;
     uint64_t poff;      // partial[poff + q * width + pos]
@@ -54,8 +63,9 @@ __global__ void k2_counter_reset_kernel(int* counter) {
 template <int NC, int CZ, int FL, bool ALLOW32>
 __global__ __launch_bounds__(256, 4) void k2_batch_kernel(
     BatchViews views, const BatchItem* __restrict__ items, int n_items, int* counter,
-    const uint4* __restrict__ cand, double* __restrict__ partial,
-    const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
+    const uint4* __restrict__ cand, const uint32_t* __restrict__ cidx,
+    double* __restrict__ partial, const Tab64* __restrict__ tab64,
+    const Tab32* __restrict__ tab32) {
     // one raw LDS block serves the C-tile records, the Z-tile scratch and the item index: an item
     // is either kind, and the index is read into a register before the item's work begins
     constexpr size_t RAW = ctile_smem_bytes<FL>() > sizeof(ZTileSmem<CZ>)
@@ -107,11 +117,12 @@ __global__ __launch_bounds__(256, 4) void k2_batch_kernel(
             continue;
         }
         Limbs4 v[NC];
+        const bool indirect = (I.flags & 4) != 0;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
             const uint32_t c = i * K2_WG + tid;
             uint4 cv = make_uint4(0, 0, 0, 0);
-            if (c < I.c_count) cv = cand[I.c_first + c];
+            if (c < I.c_count) cv = cand[indirect ? cidx[I.c_first + c] : I.c_first + c];
             v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
         }
         double* out = partial + I.poff;
@@ -167,6 +178,7 @@ namespace k2impl {
 struct BatchPlan {
     std::vector<BatchItem> items;
     std::vector<CandDesc> desc;
+    std::vector<uint32_t> cidx;   // candidate index lists of the shared tiles
     uint64_t partial_doubles = 0;
     double useful_evals = 0, padded_evals = 0;
 };
@@ -180,7 +192,46 @@ std::string plan_batch(uint32_t n_req, const k2_request* req, uint64_t n_cand_to
     const uint32_t CW = 256u * NC;
     P.desc.assign(n_cand_total, CandDesc{0, 0, 0, 0});
     std::vector<uint8_t> seen(n_cand_total, 0);
-    struct Tile { uint32_t c_first, c_count, width; bool z; };
+    // items of one tile over the zero range [0, nz_max): cut at every boundary in `cuts` (the
+    // members' cutoffs and the FP64 head), chunked; returns the tile's first partial slot
+    auto emit_tile = [&](uint32_t c_first, uint32_t c_count, uint32_t width, bool z, bool indirect,
+                         int ks, uint32_t head, std::vector<uint32_t> cuts) -> uint64_t {
+        const uint64_t chunk = z ? (uint64_t)ti.chunk * CW / CZ : (uint64_t)ti.chunk;
+        cuts.push_back(head);
+        std::sort(cuts.begin(), cuts.end());
+        cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+        const uint64_t base = P.partial_doubles;
+        uint32_t h = 0, a = 0;
+        for (uint32_t b : cuts) {
+            for (uint64_t zz = a; zz < b; zz += chunk) {
+                BatchItem it{};
+                const bool f32 = mixed && zz >= head;
+                it.flags = (z ? 1u : 0u) | (f32 ? 2u : 0u) | (indirect ? 4u : 0u) |
+                           ((uint32_t)ks << 8);
+                it.z0 = (uint32_t)zz;
+                it.z1 = (uint32_t)std::min<uint64_t>(b, zz + chunk);
+                it.c_first = c_first;
+                it.c_count = c_count;
+                it.poff = base + (uint64_t)h * 3 * width;
+                P.items.push_back(it);
+                P.padded_evals += (double)width * (it.z1 - it.z0);
+                ++h;
+            }
+            a = std::max(a, b);
+        }
+        P.partial_doubles += (uint64_t)h * 3 * width;
+        return base;
+    };
+    // items of the tile emitted last whose zeros end at or below nz (a member's own items)
+    auto items_below = [&](uint64_t base, uint32_t nz) {
+        uint32_t n = 0;
+        for (size_t i = P.items.size(); i-- > 0 && P.items[i].poff >= base;)
+            if (P.items[i].z1 <= nz) ++n;
+        return n;
+    };
+    // leftover candidates per k slot: (cutoff, head, candidate index)
+    struct Left { uint32_t nz, head, c; };
+    std::vector<std::vector<Left>> left(kslots.size());
     for (uint32_t r = 0; r < n_req; ++r) {
         const k2_request& q = req[r];
         if (q.n_cand == 0) continue;
@@ -195,47 +246,56 @@ std::string plan_batch(uint32_t n_req, const k2_request* req, uint64_t n_cand_to
         const int ks = (int)(std::find(kslots.begin(), kslots.end(), q.k) - kslots.begin());
         const uint32_t head =
             mixed ? mixed_head_zeros(kw[ks].abs_blocks, nz, et.mixed_head, et.mixed_tail) : nz;
-        std::vector<Tile> tiles;
+        P.useful_evals += (double)q.n_cand * nz;
+        // the request's full tiles
         const uint32_t n_full = q.n_cand / CW, rem = q.n_cand % CW;
-        for (uint32_t i = 0; i < n_full; ++i)
-            tiles.push_back({(uint32_t)q.cand_offset + i * CW, CW, CW, false});
-        if (rem >= (uint32_t)ti.c_min_fill && rem > 0) {
-            tiles.push_back({(uint32_t)q.cand_offset + n_full * CW, rem, CW, false});
-        } else {
-            for (uint32_t o = 0; o < rem; o += CZ)
-                tiles.push_back({(uint32_t)q.cand_offset + n_full * CW + o,
-                                 std::min<uint32_t>(CZ, rem - o), (uint32_t)CZ, true});
+        for (uint32_t i = 0; i < n_full; ++i) {
+            const uint32_t c0 = (uint32_t)q.cand_offset + i * CW;
+            const uint64_t base = emit_tile(c0, CW, CW, false, false, ks, head, {nz});
+            const uint32_t H = items_below(base, nz);
+            for (uint32_t j = 0; j < CW; ++j) P.desc[c0 + j] = CandDesc{base, H, (uint16_t)CW, (uint16_t)j};
         }
-        for (const Tile& tl : tiles) {
-            // zero chunks: same work per item for both tile kinds
-            const uint64_t chunk = tl.z ? (uint64_t)ti.chunk * CW / CZ : (uint64_t)ti.chunk;
-            struct Region { uint32_t a, b; bool f32; };
-            Region regs[2] = {{0, head, false}, {head, nz, true}};
-            std::vector<std::pair<uint32_t, uint32_t>> spans;
-            std::vector<bool> f32s;
-            for (const Region& rg : regs) {
-                for (uint64_t z = rg.a; z < rg.b; z += chunk) {
-                    spans.push_back({(uint32_t)z, (uint32_t)std::min<uint64_t>(rg.b, z + chunk)});
-                    f32s.push_back(rg.f32);
+        for (uint32_t j = 0; j < rem; ++j)
+            left[ks].push_back({nz, head, (uint32_t)q.cand_offset + n_full * CW + j});
+    }
+    // the leftovers of each k, by decreasing cutoff, in shared tiles
+    for (size_t ks = 0; ks < left.size(); ++ks) {
+        std::vector<Left>& L = left[ks];
+        std::stable_sort(L.begin(), L.end(), [](const Left& x, const Left& y) { return x.nz > y.nz; });
+        for (size_t t0 = 0; t0 < L.size(); t0 += CW) {
+            const uint32_t n = (uint32_t)std::min<size_t>(CW, L.size() - t0);
+            if (n < (uint32_t)ti.c_min_fill) {
+                // a thin last tile: Z tiles over runs of candidates with one cutoff and contiguous
+                // indices
+                size_t i = t0;
+                while (i < t0 + n) {
+                    size_t e = i + 1;
+                    while (e < t0 + n && e - i < (size_t)CZ && L[e].nz == L[i].nz &&
+                           L[e].c == L[e - 1].c + 1)
+                        ++e;
+                    const uint32_t cnt = (uint32_t)(e - i);
+                    const uint64_t base = emit_tile(L[i].c, cnt, (uint32_t)CZ, true, false, (int)ks,
+                                                    L[i].head, {L[i].nz});
+                    const uint32_t H = items_below(base, L[i].nz);
+                    for (uint32_t j = 0; j < cnt; ++j)
+                        P.desc[L[i].c + j] = CandDesc{base, H, (uint16_t)CZ, (uint16_t)j};
+                    i = e;
                 }
+                continue;
             }
-            const uint64_t base = P.partial_doubles;
-            const uint32_t H = (uint32_t)spans.size();
-            for (uint32_t h = 0; h < H; ++h) {
-                BatchItem it{};
-                it.flags = (tl.z ? 1u : 0u) | (f32s[h] ? 2u : 0u) | ((uint32_t)ks << 8);
-                it.z0 = spans[h].first;
-                it.z1 = spans[h].second;
-                it.c_first = tl.c_first;
-                it.c_count = tl.c_count;
-                it.poff = base + (uint64_t)h * 3 * tl.width;
-                P.items.push_back(it);
-                P.padded_evals += (double)tl.width * (it.z1 - it.z0);
+            const uint32_t i0 = (uint32_t)P.cidx.size();
+            std::vector<uint32_t> cuts;
+            uint32_t head = 0;
+            for (uint32_t j = 0; j < n; ++j) {
+                P.cidx.push_back(L[t0 + j].c);
+                cuts.push_back(L[t0 + j].nz);
+                head = std::max(head, L[t0 + j].head);
             }
-            P.partial_doubles += (uint64_t)H * 3 * tl.width;
-            for (uint32_t i = 0; i < tl.c_count; ++i)
-                P.desc[tl.c_first + i] = CandDesc{base, H, (uint16_t)tl.width, (uint16_t)i};
-            P.useful_evals += (double)tl.c_count * nz;
+            const uint64_t base = emit_tile(i0, n, CW, false, true, (int)ks, head, cuts);
+            for (uint32_t j = 0; j < n; ++j) {
+                const Left& m = L[t0 + j];
+                P.desc[m.c] = CandDesc{base, items_below(base, m.nz), (uint16_t)CW, (uint16_t)j};
+            }
         }
     }
     // longest first (an FP32 item takes about half the time of an FP64 one of the same size)
@@ -270,7 +330,8 @@ const void* batch_kernel(int nc, int cz, unsigned flags, bool allow32) {
 
 hipError_t launch_batch(const void* kernel, int grid, const BatchViews* views,
                         const BatchItem* items, int n_items, int* counter, const uint4* cand,
-                        double* partial, const Tab64* tab64, const Tab32* tab32,
+                        const uint32_t* cidx, double* partial, const Tab64* tab64,
+                        const Tab32* tab32,
                         const CandDesc* desc, uint64_t n_cand, double* K, double* dK, double* d2K,
                         hipStream_t st) {
     hipLaunchKernelGGL(k2_counter_reset_kernel, dim3(1), dim3(64), 0, st, counter);
@@ -278,7 +339,7 @@ hipError_t launch_batch(const void* kernel, int grid, const BatchViews* views,
     if (e != hipSuccess) return e;
     if (n_items > 0 && grid > 0) {
         BatchViews v = *views;
-        void* args[] = {&v, &items, &n_items, &counter, &cand, &partial, &tab64, &tab32};
+        void* args[] = {&v, &items, &n_items, &counter, &cand, &cidx, &partial, &tab64, &tab32};
         e = hipLaunchKernel(kernel, dim3(grid), dim3(256), args, 0, st);
         if (e != hipSuccess) return e;
     }
@@ -302,6 +363,7 @@ struct k2_batch {
     std::vector<WeightArrays> w;   // one per distinct k, owned (the table's cache may evict)
     BatchItem* d_items = nullptr;
     CandDesc* d_desc = nullptr;
+    uint32_t* d_cidx = nullptr;
     double* d_partial = nullptr;
     int* d_counter = nullptr;
     int n_items = 0;
@@ -340,6 +402,7 @@ extern "C" int k2_batch_destroy(k2_batch* b) {
     for (auto& w : b->w) free_weights(w);
     (void)hipFree(b->d_items);
     (void)hipFree(b->d_desc);
+    (void)hipFree(b->d_cidx);
     (void)hipFree(b->d_partial);
     (void)hipFree(b->d_counter);
     if (b->cap) (void)hipStreamDestroy(b->cap);
@@ -423,6 +486,10 @@ extern "C" int k2_batch_create(k2_table* t, uint32_t n_req, const k2_request* re
     BCHK(hipMalloc((void**)&b->d_items, std::max<size_t>(1, P.items.size()) * sizeof(BatchItem)));
     BCHK(hipMalloc((void**)&b->d_desc, std::max<size_t>(1, P.desc.size()) * sizeof(CandDesc)));
     BCHK(hipMalloc((void**)&b->d_partial, std::max<uint64_t>(1, P.partial_doubles) * sizeof(double)));
+    BCHK(hipMalloc((void**)&b->d_cidx, std::max<size_t>(1, P.cidx.size()) * sizeof(uint32_t)));
+    if (!P.cidx.empty())
+        BCHK(hipMemcpyAsync(b->d_cidx, P.cidx.data(), P.cidx.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, b->cap));
     if (!P.items.empty())
         BCHK(hipMemcpyAsync(b->d_items, P.items.data(), P.items.size() * sizeof(BatchItem),
                             hipMemcpyHostToDevice, b->cap));
@@ -455,8 +522,8 @@ extern "C" int k2_batch_create(k2_table* t, uint32_t n_req, const k2_request* re
 static hipError_t batch_launches(k2_batch* b, hipStream_t st) {
     k2_table* t = b->t;
     return launch_batch(b->kernel, b->grid, &b->views, b->d_items, b->n_items, b->d_counter, b->cand,
-                        b->d_partial, t->tab64, t->tab32, b->d_desc, b->n_cand, b->K, b->dK,
-                        b->d2K, st);
+                        b->d_cidx, b->d_partial, t->tab64, t->tab32, b->d_desc, b->n_cand, b->K,
+                        b->dK, b->d2K, st);
 }
 
 extern "C" int k2_batch_run_direct(k2_batch* b, void* stream) {
