@@ -170,33 +170,38 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
     const bool lo_half = NT == K2_WG || tid < K2_WG;   // stages g, a
     const bool hi_half = NT == K2_WG || tid >= K2_WG;  // stages b, e
     const int ntiles = z1 > z0 ? (z1 - z0 + K2_WG - 1) / K2_WG : 0;
-    uint4 rg = make_uint4(0, 0, 0, 0);
-    double2 ra = make_double2(0.0, 0.0), rb = ra, re = ra;
-    auto fetch = [&](int t) {
+    // the next tile's record of this lane, held in registers while the current tile is summed.
+    // Returned by value: assigning captured locals by reference under these conditions made
+    // the compiler keep them in a private (scratch) array, stored and reloaded every tile.
+    struct Rec { uint4 g; double2 a, b, e; };
+    auto fetch = [&](int t) -> Rec {
         const int j = z0 + t * K2_WG + zl;
+        Rec r;
+        r.g = make_uint4(0, 0, 0, 0);
+        r.a = r.b = r.e = make_double2(0.0, 0.0);
         if (j < z1) {
-            if (lo_half) { rg = zv.g[j]; ra = zv.a[j]; }
+            if (lo_half) { r.g = zv.g[j]; r.a = zv.a[j]; }
             if (hi_half) {
-                if (D1) rb = zv.b[j];
-                if (D2) re = zv.e[j];
+                if (D1) r.b = zv.b[j];
+                if (D2) r.e = zv.e[j];
             }
-        } else {
-            rg = make_uint4(0, 0, 0, 0); ra = make_double2(0.0, 0.0); rb = ra; re = ra;
         }
+        return r;
     };
-    auto stage = [&](int buf) {
-        if (lo_half) { sm.gz(buf, zl) = rg; sm.a(buf, zl) = ra; }
+    auto stage = [&](int buf, const Rec& r) {
+        if (lo_half) { sm.gz(buf, zl) = r.g; sm.a(buf, zl) = r.a; }
         if (hi_half) {
-            if (D1) sm.b(buf, zl) = rb;
-            if (D2) sm.e(buf, zl) = re;
+            if (D1) sm.b(buf, zl) = r.b;
+            if (D2) sm.e(buf, zl) = r.e;
         }
     };
-    if (ntiles > 0) { fetch(0); stage(0); }
+    Rec pf;
+    if (ntiles > 0) { pf = fetch(0); stage(0, pf); }
     __syncthreads();
     const SinCosCoef kc = sincos_coef();
     for (int t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
-        if (t + 1 < ntiles) fetch(t + 1);
+        if (t + 1 < ntiles) pf = fetch(t + 1);
         const int zn = min(K2_WG, z1 - z0 - t * K2_WG);   // zeros of this tile (short splits)
 #pragma unroll(NT == K2_WG ? 2 : 1)
         for (int z = 0; z < zn; ++z) {
@@ -213,7 +218,7 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
                 accumulate<FL>(acc[i], S, C, aa, bb, ee);
             }
         }
-        if (t + 1 < ntiles) stage(buf ^ 1);
+        if (t + 1 < ntiles) stage(buf ^ 1, pf);
         __syncthreads();
     }
 }
