@@ -438,6 +438,12 @@ def run_eval(args, form):
         res["screen"] = sub_record(run_screen(args, ctx))
         res["screen_survivors"] = sub_record(run_screen(args, ctx, peaks=True))
         res["compat"] = run_compat(args, ctx)
+        res["weights"] = sub_record(run_weights(args, ctx),
+                                    keep=("value", "unit", "ms_per_step", "config", "accuracy",
+                                          "vs_separate_passes", "wall_s"))
+        res["enclose"] = sub_record(run_enclose(args, ctx),
+                                    keep=("value", "unit", "ms_per_step", "config", "accuracy",
+                                          "radius_max_all", "roofline", "wall_s"))
     if not args.no_cpu and args.workload == "metric":
         try:
             cb = cpu_baseline(tb, V, c["k"])
@@ -671,7 +677,7 @@ def sub_record(res, keep=("value", "unit", "ms_per_step", "dtype", "config", "ac
     return {k: res[k] for k in keep if k in res}
 
 
-def run_enclose(args):
+def run_enclose(args, ctx=None):
     """Enclosure mode (midpoint + rigorous radius of K2 and K2') on the metric workload."""
     import numpy as np
     import torch
@@ -679,10 +685,9 @@ def run_enclose(args):
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import synth
     c = synth.W65536x2325572
-    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
-    n_cand = args.n_cand or c["n_cand"]
+    tb, _ = full_table(ctx)
+    n_cand = args.n_cand if (args.n_cand and ctx is None) else c["n_cand"]
     V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
-    torch.cuda.set_device(0)
     t = ev.K2Table(tb)
     autotune.apply_tuned(t, n_cand, tb.n, "plain")
     Vd = ev.v_tensor(V)
@@ -694,7 +699,8 @@ def run_enclose(args):
         inside = np.abs(K - g["K"]) <= Kr + 2.0 ** -53 * np.abs(g["K"])
         acc = {"max_abs_dK2_vs_oracle": float(np.max(np.abs(K - g["K"]))),
                "max_radius": float(Kr.max()), "max_dK_radius": float(dKr.max()),
-               "oracle_inside_all": bool(inside.all()), "oracle_sample": "w65536x2325572_sub256.npz"}
+               "oracle_inside_all": bool(inside.all()), "oracle_sample": "w65536x2325572_sub256.npz",
+               "bar": 1e-9, "ok": bool(inside.all() and np.max(np.abs(K - g["K"])) <= 1e-9)}
     state = {}
 
     def step():
@@ -717,18 +723,20 @@ def run_enclose(args):
     return res
 
 
-def run_weights(args):
+def run_weights(args, ctx=None):
     """The reference's five weight vectors (JP kernel, Gaussians at 1.5e-10 and 1e-10, raw,
     first 10,000 zeros) for the metric's candidates and table in one pass
-    (k2_eval_weights_device), against five one-vector passes of the same kernel."""
+    (k2_eval_weights_device), against five one-vector passes of the same kernel.  Accuracy: the
+    Gaussian row at k = 1.5e-10 is K2 itself, checked on the committed binary128 sample of the
+    same candidates (w65536x2325572_sub256.npz)."""
+    import numpy as np
     import torch
     from mertens_bound_amd import evaluator as ev
     from mertens_bound_amd import pipeline, synth
     c = synth.W65536x2325572
-    tb = synth.make_zero_table(c["n_zeros"], c["seed"])
-    n_cand = args.n_cand or c["n_cand"]
+    tb, _ = full_table(ctx)
+    n_cand = args.n_cand if (args.n_cand and ctx is None) else c["n_cand"]
     V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
-    torch.cuda.set_device(0)
     t = ev.K2Table(tb)
     t.set_tiling(target_blocks=3072)
     Vd = ev.v_tensor(V)
@@ -744,6 +752,14 @@ def run_weights(args):
                       {"workload": f"K2 for {len(names)} weight vectors ({', '.join(names)}), "
                                    f"{n_cand:,} candidates x {tb.n:,} zeros", "form": "weights"})
     res["vs_separate_passes"] = {"ms": ms1, "speedup": ms1 / ms}
+    g = None if args.no_check else golden_npz("w65536x2325572_sub256.npz")
+    if g is not None and np.array_equal(V[:g["V"].shape[0]], g["V"]):
+        m = g["V"].shape[0]
+        K = t.eval_weights_device(Vd[:m], wtd, flags=ev.VALUE)[0][names.index("g15")]
+        e = float(np.max(np.abs(K.cpu().numpy() - g["K"])))
+        res["accuracy"] = {"max_abs_dK2_gaussian_row_vs_oracle": e,
+                           "oracle_sample": "w65536x2325572_sub256.npz (binary128, k = 1.5e-10)",
+                           "bar": 1e-9, "ok": e <= 1e-9}
     res["wall_s"] = wall
     return res
 
