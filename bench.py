@@ -34,8 +34,10 @@ sys.path.insert(0, ROOT)
 #                          + 15 INT32 multiply-adds per FP32-tail evaluation; its FP64 head at the
 #                          plain constants
 #   enclosure K2 + K2':    the plain work + 12 FP64 additions (TwoSum accumulation of K2, K2')
-#   leaf screen (one zero, 7 shifted sums): 15 INT32 multiply-adds + 2 transcendentals + 31 FP32
-#                          flops (weight rotation 6, unshifted sum 1, three shift pairs 4 FMAs each)
+#   leaf screen (one zero, 7 shifted sums): 15 INT32 multiply-adds + 2 transcendentals + 19 FP32
+#                          flops (weight rotation 6, unshifted sum 1, per shift pair the two FMAs
+#                          of Re q cos(m g d) and Im q sin(m g d), whose sum and difference are the
+#                          pair's terms)
 # roofline.frac is the largest class fraction (achieved class rate / class peak).  Class peaks:
 # FP64 78.6 TFLOP/s and FP32 157.3 TFLOP/s (MI355X published); the INT32 multiply-add
 # (v_mad_u64_u32) and transcendental (v_sin_f32) rates are measured on the same GPU in the same
@@ -44,7 +46,7 @@ ALG = {"plain": {"fp64_flops": 42, "int32_mad": 15},
        "plain_d2": {"fp64_flops": 46, "int32_mad": 15},
        "mixed": {"trans": 2, "fp32_flops": 8, "int32_mad": 15},
        "enclose": {"fp64_flops": 54, "int32_mad": 15},
-       "screen": {"int32_mad": 15, "trans": 2, "fp32_flops": 31}}
+       "screen": {"int32_mad": 15, "trans": 2, "fp32_flops": 19}}
 PUBLISHED_PEAKS = {"fp64_flops": 78.6e12, "fp32_flops": 157.3e12}
 # measured once on gfx950 (k2_valu_rates, lane operations/s); used only where the live
 # measurement fails
@@ -53,12 +55,12 @@ RECORDED_RATES = {"int32_mad": 33.0e12, "trans": 9.8e12}
 # one wave64 instruction per 4 cycles per SIMD = 39.3 T lane-slots/s on 256 CUs at 2.4 GHz;
 # FP32 instructions issue in half a slot.  plain: 15 INT32 multiply-adds + 21 FP64 FMAs; mixed
 # FP32 tail as built: 6 integer instructions + 15 FP32 at half a slot; enclosure: + 12 FP64
-# additions; screen: 7 integer + 28 FP32 instructions per zero.
-SLOTS = {"plain": 36.0, "plain_d2": 38.0, "mixed": 13.5, "enclose": 48.0, "screen": 21.0}
+# additions; screen: 8 integer + 22 FP32 instructions per zero.
+SLOTS = {"plain": 36.0, "plain_d2": 38.0, "mixed": 13.5, "enclose": 48.0, "screen": 19.0}
 VALU_SLOT_PEAK = 39.3e12
 # SQ_INSTS_VALU per evaluation from the committed counter summaries (wave instructions x 64 /
 # evaluations)
-VALU_INSTS = {"plain": 41.7, "mixed": 22.7, "screen": 35.0}
+VALU_INSTS = {"plain": 41.7, "mixed": 22.7, "screen": 30.0}
 _RATES = {}
 
 
@@ -836,19 +838,29 @@ def run_screen(args, ctx=None, peaks=False):
                      "zero_evals_per_step": zero_evals,
                      "zero_evals_per_s": zero_evals / (ms * 1e-3)}
     if peaks:
-        # per-stage times: calls with the first one and two stages beside the full call (the
-        # later stages recompute the earlier zeros of their survivors)
-        ms_pre = []
-        for ns in range(1, len(stages)):
-            ms_pre.append(timed(lambda: t.screen_device(Vd, k, d, m, stages[:ns]), args.steps,
-                                args.warmup, torch)[0])
-        ms_pre.append(ms)
+        # per-stage times: calls with the first one, two and all stages, interleaved over rounds,
+        # each call between its own HIP events; the medians' differences are the stages' times
+        # (the later stages recompute the earlier zeros of their survivors)
+        st = torch.cuda.current_stream()
+        ev_pairs = []
+        for _ in range(max(10, 2 * args.steps)):
+            for ns in range(1, len(stages) + 1):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                t.screen_device(Vd, k, d, m, stages[:ns])
+                e1.record(st)
+                ev_pairs.append((ns, e0, e1))
+        torch.cuda.synchronize()
+        ms_pre = [float(np.median([a.elapsed_time(b) for n_, a, b in ev_pairs if n_ == ns]))
+                  for ns in range(1, len(stages) + 1)]
         per = []
         for s_ in range(len(stages)):
             dt = ms_pre[s_] - (ms_pre[s_ - 1] if s_ else 0.0)
             ev_s = float(reached[s_]) * nz[s_]             # zeros the stage's launch sums
             per.append({"stage": s_ + 1, "zeros": int(nz[s_]), "candidates": int(reached[s_]),
-                        "ms": dt, "zero_evals_per_s": ev_s / (dt * 1e-3) if dt > 0 else None})
+                        "ms": dt, "zero_evals_per_s": ev_s / (dt * 1e-3) if dt > 0 else None,
+                        "median_call_ms_through_this_stage": ms_pre[s_]})
         res["screen"]["per_stage"] = per
         # every candidate through every stage: the survivors' values must be the same bits
         allv, _, _ = t.screen_device(Vd, k, d, m, tuple((a, 0.0) for a, _ in stages))

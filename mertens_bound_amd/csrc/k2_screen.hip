@@ -7,28 +7,33 @@
 // (reference: 64/128/256 zeros, 0.78/0.855/0.892).  Here the phase is the precision form's window
 // (columns 4..5 of G V exact in integers, column 3 in FP32: within 2^-34 turn, not a wrapped
 // 64-bit approximation) and sin/cos come from the FP32 LDS table (not a 12-bit cos lookup); the
-// shifted terms use the per-zero rotations e^{i m g_j d}, formed once per call in FP64:
-// Re(q e^{+-i m g d}) is two FMAs each.
+// shifted terms use the per-zero rotations e^{i m g_j d}, formed once per call in FP64.  With
+// q_j = a_j e^{i g_j v}, Re(q e^{+-i m g d}) = Re q cos(m g d) -+ Im q sin(m g d), so a candidate
+// keeps a0 = sum Re q, A_m = sum Re q cos(m g d) and B_m = sum Im q sin(m g d) (one FMA each per
+// zero, 2M + 1 in all) and forms h_{+-m} = A_m -+ B_m once at the end, where summing each shifted
+// term separately takes two FMAs per shift.
 //
-// One lane per candidate.  The zero records of all stages (<= K2_SCREEN_MAX: G and its limbs as
-// scaled floats, the FP32 weight 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once
-// per call by k2_screen_records_kernel; every lane reads the same record at the same time, so the
-// screen kernel takes them through the scalar cache, and LDS holds only the 16 KiB sin/cos table.
-// Each stage is its own launch over the candidates still in play: the first over all candidates
-// (one lane each), the later ones over the survivors of the stage before, which each launch
-// appends to lists in device memory (the lists' order does not matter, since every candidate
-// writes only its own outputs).  A later stage starts its candidates from the first zero again
-// (the earlier stages' sums are recomputed, bit for bit the same) and its waves are full however
-// few candidates survive; the alternative, one launch in which a failed lane idles while its
-// wave's survivors continue, leaves most lanes of the later stages masked off when survivors are
-// rare.  The survivors go to K2_SCREEN_NSUB sub-lists, each with its own counter on its own
-// cache line: in the first stage a wavefront appends to sub-list (wave index mod NSUB) with one
-// atomic, and a later stage gives each sub-list K2_SCREEN_PER_SUB workgroups whose survivors stay
-// in that sub-list (so a sub-list never outgrows its first-stage size).  One counter for all
-// would take one atomic per wavefront at a single address (262,144 of them for 2^24
-// candidates), which serialises.  A candidate's later stage values stay -1 once it fails.  The
-// stage zero counts and thresholds are kernel arguments and the list lengths stay on the
-// device: the call does not synchronise.
+// The zero records of all stages (<= K2_SCREEN_MAX: G and its limbs as scaled floats, the FP32
+// weight 2 e^{-k g^2} w and the rotations e^{i m g d}, m = 1..M) are formed once per call by
+// k2_screen_records_kernel.  Every lane of a wave reads the same record at the same time, so the
+// kernels take them through the scalar cache, and LDS holds the 16 KiB sin/cos table.  Each stage
+// is its own launch over the candidates still in play:
+// * the first over all candidates, one lane each;
+// * the later ones over the survivors of the stage before, which each launch appends to lists in
+//   device memory (the lists' order does not matter, since every candidate writes only its own
+//   outputs).  A later stage starts its candidates from the first zero again (the earlier
+//   stages' sums are recomputed, bit for bit the same in every run), and its waves are full
+//   however few candidates survive: the four waves of a workgroup share 64 survivors, each wave
+//   summing a quarter of the zeros, so that few survivors still make enough waves to fill the
+//   GPU, and the workgroups take the survivors of all sub-lists as one list, so that survivors
+//   that cluster in some sub-lists (those near one peak do) still spread over all of them.
+// The survivors go to K2_SCREEN_NSUB sub-lists, each with its own counter on its own cache line:
+// a wavefront of the first stage appends to sub-list (wave index mod NSUB), pass P of a later
+// stage to sub-list P mod NSUB, with one atomic per wavefront.  One counter for all would take
+// one atomic per wavefront at a single address (262,144 of them for 2^24 candidates), which
+// serialises.  A candidate's later stage values stay -1 once it fails.  The stage zero counts and
+// thresholds are kernel arguments and the list lengths stay on the device: the call does not
+// synchronise.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -46,8 +51,8 @@ using namespace k2dev;
 #define K2_SCREEN_STAGES 4
 #define K2_SCREEN_MAXM 8     // shifts -M..M, M <= 8
 #define K2_SCREEN_NSUB 256   // survivor sub-lists (and counters) per stage
-#define K2_SCREEN_PER_SUB 8  // workgroups per sub-list in the later stages
 #define K2_SCREEN_CSTRIDE 16 // ints between counters (one 64-byte line each)
+#define K2_SCREEN_GRID 2048  // workgroups of a later stage (each loops over passes)
 
 // stage zero counts and thresholds, passed by value (kernel arguments)
 struct ScreenStages {
@@ -95,92 +100,216 @@ __global__ __launch_bounds__(256) void k2_screen_records_kernel(
     rec[j] = z;
 }
 
-// Stage s of the screen.  s = 0: candidate i = lane i of the grid, for i < n_cand.  s > 0:
-// workgroup b walks sub-list b mod NSUB of the stage before, entries (b / NSUB) 256 + lane in
-// steps of PER_SUB 256.  Each candidate sums zeros [0, nz[s]) at all shifts, writes its stage-s
-// value, and either stops (pass 0, later values -1), passes the last stage (pass 1) or joins its
-// sub-list for stage s + 1 (sub-list q holds entries [q cap, q cap + counts[q])).  The best shift
-// is that of its last stage.
+// a candidate's shifted sums: a0 = sum Re q_j, A[m-1] = sum Re q_j cos(m g_j d),
+// B[m-1] = sum Im q_j sin(m g_j d)
+template <int M>
+struct ShiftSums {
+    float a0;
+    float A[M > 0 ? M : 1], B[M > 0 ? M : 1];
+    __device__ __forceinline__ void zero() {
+        a0 = 0.0f;
+#pragma unroll
+        for (int m = 0; m < M; ++m) A[m] = B[m] = 0.0f;
+    }
+};
+
+// one zero's term for candidate v (v's window operands vp, vf); SB: the record is uniform over
+// the wave and sits in scalar registers
+template <int M, bool SB>
+__device__ __forceinline__ void screen_term(const Limbs4& v, uint32_t vp, const float4& vf,
+                                            const ScreenRec<M>& z, const Tab32* s_tab,
+                                            ShiftSums<M>& acc) {
+    uint32_t hi, lo;
+    float t3;
+    phase_window_f<K2_TAB32_BITS, SB>(v, vp, vf, Limbs4{0u, z.g.y, z.g.z, z.g.w}, z.g.x, z.gvf,
+                                      hi, lo, t3);
+    float S, C;
+    sincos_window_f<K2_TAB32_BITS>(hi, lo, t3, s_tab, S, C);
+    // q = a e^{i th}
+    const float qr = fmaf(z.a.x, C, -z.a.y * S);
+    const float qi = fmaf(z.a.x, S, z.a.y * C);
+    acc.a0 += qr;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        acc.A[m] = fmaf(qr, z.r[m].x, acc.A[m]);
+        acc.B[m] = fmaf(qi, z.r[m].y, acc.B[m]);
+    }
+}
+
+// the stage value max_m |h_m| and the first shift that reaches it (m = -M..M in order)
+template <int M>
+__device__ __forceinline__ float stage_value(const ShiftSums<M>& acc, int& best) {
+    // h_{-m} = A + B, h_{+m} = A - B; shifts in the order -M..M with a strict comparison: the
+    // first maximum wins
+    float mv = -1.0f;
+    best = 0;
+#pragma unroll
+    for (int i = 0; i < 2 * M + 1; ++i) {
+        const int m = i - M;
+        const float h = m == 0 ? acc.a0
+                               : (m < 0 ? acc.A[-m - 1] + acc.B[-m - 1]
+                                        : acc.A[m - 1] - acc.B[m - 1]);
+        const float a = fabsf(h);
+        if (a > mv) { mv = a; best = m; }
+    }
+    return mv;
+}
+
+// a candidate's stage-s outputs; returns whether it goes on to stage s + 1
+__device__ __forceinline__ bool screen_outputs(int c, int s, int n_stages, const ScreenStages& sp,
+                                               float mv, int best, float* __restrict__ out_val,
+                                               int* __restrict__ out_shift,
+                                               int* __restrict__ out_pass) {
+    float* val = out_val + (size_t)c * K2_SCREEN_STAGES;
+    val[s] = mv;
+    const bool pass = mv >= sp.thr[s];
+    const bool next = pass && s + 1 < n_stages;
+    if (!next) {
+        for (int t = s + 1; t < K2_SCREEN_STAGES; ++t) val[t] = -1.0f;
+        out_shift[c] = best;
+        out_pass[c] = pass ? 1 : 0;
+    }
+    return next;
+}
+
+// survivors join sub-list qo of the next stage: one atomic per wavefront (lanes with take set)
+__device__ __forceinline__ void screen_append(bool take, int c, int qo, int* __restrict__ list_out,
+                                              int* __restrict__ count_out, int cap) {
+    const uint64_t bal = __ballot(take);
+    if (!bal) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count_out + qo * K2_SCREEN_CSTRIDE, __popcll(bal));
+    base = __shfl(base, leader, 64);
+    if (take) list_out[(size_t)qo * cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = c;
+}
+
+// Stage 0: candidate i = lane i of the grid, for i < n_cand, summing zeros [0, nz[0]) at all
+// shifts; it writes its stage value and either stops (pass 0, later values -1), passes the last
+// stage (pass 1) or joins sub-list (wave index mod NSUB) of stage 1 (sub-list q holds entries
+// [q cap, q cap + counts[q])).  The best shift is that of its last stage.
 template <int M>
 __global__ __launch_bounds__(256) void k2_screen_kernel(
     const ScreenRec<M>* __restrict__ rec, const uint4* __restrict__ cand, int n_cand,
-    int n_stages, ScreenStages sp, int s, const int* __restrict__ list_in,
-    const int* __restrict__ count_in, int* __restrict__ list_out, int* __restrict__ count_out,
+    int n_stages, ScreenStages sp, int* __restrict__ list_out, int* __restrict__ count_out,
     int cap, const Tab32* __restrict__ tab, float* __restrict__ out_val,
     int* __restrict__ out_shift, int* __restrict__ out_pass) {
     __shared__ Tab32 s_tab[K2_TAB32];
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB32; i += blockDim.x) s_tab[i] = tab[i];
     __syncthreads();
-    int q, i0, step, n;
-    if (s == 0) {
-        i0 = blockIdx.x * blockDim.x + tid;
-        step = gridDim.x * blockDim.x;
-        n = n_cand;
-        q = 0;   // per wavefront below
-    } else {
-        q = blockIdx.x % K2_SCREEN_NSUB;
-        i0 = (blockIdx.x / K2_SCREEN_NSUB) * blockDim.x + tid;
-        step = K2_SCREEN_PER_SUB * blockDim.x;
-        n = count_in[q * K2_SCREEN_CSTRIDE];
-    }
-    const int nz = sp.nz[s];
-    for (int i = i0; i < n; i += step) {
-        const int c = s == 0 ? i : list_in[(size_t)q * cap + i];
-        const uint4 cv = cand[c];
+    const int nz = sp.nz[0];
+    for (int i = blockIdx.x * blockDim.x + tid; i < n_cand; i += gridDim.x * blockDim.x) {
+        const uint4 cv = cand[i];
         const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
         // the lane's side of the window: V's limbs as floats (unscaled) and column 5's packed limbs
         const float4 vf = make_float4((float)v.x0, (float)v.x1, (float)v.x2, (float)v.x3);
         const uint32_t vp = pack_lo16(v.x2, v.x3);
-        float acc[2 * M + 1];
+        ShiftSums<M> acc;
+        acc.zero();
+        for (int j = 0; j < nz; ++j) screen_term<M, true>(v, vp, vf, rec[j], s_tab, acc);
+        int best;
+        const float mv = stage_value<M>(acc, best);
+        const bool next = screen_outputs(i, 0, n_stages, sp, mv, best, out_val, out_shift,
+                                         out_pass);
+        screen_append(next, i, (i >> 6) % K2_SCREEN_NSUB, list_out, count_out, cap);
+    }
+}
+
+// Stage s > 0, over the survivors of stage s - 1 in its NSUB sub-lists, taken as one list: entry
+// e of the concatenation is entry e - pre[q] of sub-list q, pre the prefix sums of the sub-list
+// counts (formed by every workgroup from the counters).  Workgroup pass P (P = b, b + grid, ...)
+// takes entries [64 P, 64 P + 64), one per lane, and its four waves share them: wave w sums zeros
+// [w Z, w Z + Z) of [0, nz[s]), Z = ceil(nz / 4), with the record uniform over the wave (scalar
+// loads, as in stage 0), and wave 0 adds the other waves' sums (in wave order, through LDS) to
+// its own.  Survivors of pass P join sub-list P mod NSUB of stage s + 1: a sub-list receives at
+// most 64 entries from each of ceil(n / (64 NSUB)) passes, within its capacity.  Candidates that
+// cluster in a few sub-lists of the stage before (survivors near one peak do) still spread over
+// every workgroup.
+template <int M>
+__global__ __launch_bounds__(256) void k2_screen_later_kernel(
+    const ScreenRec<M>* __restrict__ rec, const uint4* __restrict__ cand, int n_stages,
+    ScreenStages sp, int s, const int* __restrict__ list_in, const int* __restrict__ count_in,
+    int* __restrict__ list_out, int* __restrict__ count_out, int cap,
+    const Tab32* __restrict__ tab, float* __restrict__ out_val, int* __restrict__ out_shift,
+    int* __restrict__ out_pass) {
+    constexpr int NW = 4;   // waves per workgroup
+    constexpr int NS = 2 * M + 1;
+    static_assert(K2_SCREEN_NSUB == NW * 64, "one sub-list counter per thread");
+    __shared__ Tab32 s_tab[K2_TAB32];
+    __shared__ int s_pre[K2_SCREEN_NSUB + 1];
+    __shared__ int s_wsum[NW];
+    __shared__ float s_part[NW - 1][NS][64];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int i = tid; i < K2_TAB32; i += blockDim.x) s_tab[i] = tab[i];
+    // exclusive prefix sums of the sub-list counts: a scan within each wave, then the wave totals
+    int x = count_in[tid * K2_SCREEN_CSTRIDE];
 #pragma unroll
-        for (int m = 0; m < 2 * M + 1; ++m) acc[m] = 0.0f;
-        for (int j = 0; j < nz; ++j) {
-            const ScreenRec<M> z = rec[j];   // the same j in every lane
-            uint32_t hi, lo;
-            float t3;
-            phase_window_f<K2_TAB32_BITS, true>(v, vp, vf, Limbs4{0u, z.g.y, z.g.z, z.g.w},
-                                                z.g.x, z.gvf, hi, lo, t3);
-            float S, C;
-            sincos_window_f<K2_TAB32_BITS>(hi, lo, t3, s_tab, S, C);
-            // q = a e^{i th}
-            const float qr = fmaf(z.a.x, C, -z.a.y * S);
-            const float qi = fmaf(z.a.x, S, z.a.y * C);
-            acc[M] += qr;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wsum[w] = x;
+    __syncthreads();
+    int off = 0;
+    for (int u = 0; u < w; ++u) off += s_wsum[u];
+    s_pre[tid + 1] = x + off;
+    if (tid == 0) s_pre[0] = 0;
+    const int n = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+    __syncthreads();
+    const int nz = sp.nz[s];
+    const int zq = (nz + NW - 1) / NW;
+    const int z_lo = min(nz, w * zq), z_hi = min(nz, z_lo + zq);
+    for (int P = blockIdx.x; P * 64 < n; P += gridDim.x) {   // uniform over the workgroup
+        const int e = P * 64 + lane;
+        const bool have = e < n;
+        int c = 0;
+        if (have) {
+            int lo = 0, hi = K2_SCREEN_NSUB;   // the sub-list q with pre[q] <= e < pre[q + 1]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= e) lo = mid; else hi = mid;
+            }
+            c = list_in[(size_t)lo * cap + (e - s_pre[lo])];
+        }
+        const uint4 cv = cand[c];
+        const Limbs4 v{cv.x, cv.y, cv.z, cv.w};
+        const float4 vf = make_float4((float)v.x0, (float)v.x1, (float)v.x2, (float)v.x3);
+        const uint32_t vp = pack_lo16(v.x2, v.x3);
+        ShiftSums<M> acc;
+        acc.zero();
+        for (int j = z_lo; j < z_hi; ++j) screen_term<M, true>(v, vp, vf, rec[j], s_tab, acc);
+        if (w > 0) {
+            s_part[w - 1][0][lane] = acc.a0;
 #pragma unroll
-            for (int m = 1; m <= M; ++m) {   // Re(q e^{+i m g d}), Re(q e^{-i m g d})
-                const float2 r = z.r[m - 1];
-                acc[M + m] = fmaf(-qi, r.y, fmaf(qr, r.x, acc[M + m]));
-                acc[M - m] = fmaf(qi, r.y, fmaf(qr, r.x, acc[M - m]));
+            for (int m = 0; m < M; ++m) {
+                s_part[w - 1][1 + m][lane] = acc.A[m];
+                s_part[w - 1][1 + M + m][lane] = acc.B[m];
             }
         }
-        float mv = -1.0f;
-        int best = M;
+        __syncthreads();
+        if (w == 0) {
 #pragma unroll
-        for (int m = 0; m < 2 * M + 1; ++m) {
-            const float a = fabsf(acc[m]);
-            if (a > mv) { mv = a; best = m; }
+            for (int u = 0; u < NW - 1; ++u) {
+                acc.a0 += s_part[u][0][lane];
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    acc.A[m] += s_part[u][1 + m][lane];
+                    acc.B[m] += s_part[u][1 + M + m][lane];
+                }
+            }
+            bool next = false;
+            if (have) {
+                int best;
+                const float mv = stage_value<M>(acc, best);
+                next = screen_outputs(c, s, n_stages, sp, mv, best, out_val, out_shift, out_pass);
+            }
+            screen_append(next, c, P % K2_SCREEN_NSUB, list_out, count_out, cap);
         }
-        float* val = out_val + (size_t)c * K2_SCREEN_STAGES;
-        val[s] = mv;
-        const bool pass = mv >= sp.thr[s];
-        const bool next = pass && s + 1 < n_stages;
-        if (!next) {
-            for (int t = s + 1; t < K2_SCREEN_STAGES; ++t) val[t] = -1.0f;
-            out_shift[c] = best - M;
-            out_pass[c] = pass ? 1 : 0;
-        }
-        // survivors join a sub-list of the next stage: one atomic per wavefront
-        const uint64_t bal = __ballot(next);
-        if (bal) {
-            const int qo = s == 0 ? (i >> 6) % K2_SCREEN_NSUB : q;
-            const int lane = __lane_id();
-            const int leader = __ffsll((unsigned long long)bal) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(count_out + qo * K2_SCREEN_CSTRIDE, __popcll(bal));
-            base = __shfl(base, leader, 64);
-            if (next) list_out[(size_t)qo * cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = c;
-        }
+        __syncthreads();   // wave 0 has read s_part before the next pass writes it
     }
 }
 
@@ -211,12 +340,17 @@ static hipError_t launch_screen_m(const uint4* zg, const double* gamma,
                        w, k, d, n_zeros, (ScreenRec<M>*)B.rec, B.counts);
     hipError_t e = hipGetLastError();
     const int cs = K2_SCREEN_NSUB * K2_SCREEN_CSTRIDE;   // ints of one stage's counters
-    for (int s = 0; s < n_stages && e == hipSuccess; ++s) {
-        const int grid = s == 0 ? (n_cand + 255) / 256 : K2_SCREEN_NSUB * K2_SCREEN_PER_SUB;
-        hipLaunchKernelGGL((k2_screen_kernel<M>), dim3(grid), dim3(256), 0, st,
-                           (const ScreenRec<M>*)B.rec, cand, n_cand, n_stages, sp, s,
-                           s > 0 ? B.list[(s - 1) & 1] : nullptr,
-                           s > 0 ? B.counts + (s - 1) * cs : nullptr, B.list[s & 1],
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL((k2_screen_kernel<M>), dim3((n_cand + 255) / 256), dim3(256), 0, st,
+                           (const ScreenRec<M>*)B.rec, cand, n_cand, n_stages, sp, B.list[0],
+                           B.counts, B.cap, tab, val, shift, pass);
+        e = hipGetLastError();
+    }
+    for (int s = 1; s < n_stages && e == hipSuccess; ++s) {
+        hipLaunchKernelGGL((k2_screen_later_kernel<M>),
+                           dim3(std::min(K2_SCREEN_GRID, (n_cand + 63) / 64)), dim3(256), 0, st,
+                           (const ScreenRec<M>*)B.rec, cand, n_stages, sp, s,
+                           B.list[(s - 1) & 1], B.counts + (s - 1) * cs, B.list[s & 1],
                            B.counts + s * cs, B.cap, tab, val, shift, pass);
         e = hipGetLastError();
     }

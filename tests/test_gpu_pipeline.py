@@ -36,6 +36,14 @@ def test_screen_values_vs_oracle(setup, oracle_lib):
         err = np.max(np.abs(val[:, s] - want)) / scale
         print(f"screen stage {nz}: max error {err:.2e} of sum|a|")
         assert err <= 2e-6
+        # the best shift (that of the last stage) is the oracle's argmax wherever the two
+        # largest |h_m| differ by more than the error bar
+        if s == len(stages) - 1:
+            srt = np.sort(np.abs(hs), axis=1)
+            clear = srt[:, -1] - srt[:, -2] > 4e-6 * scale
+            want_best = np.argmax(np.abs(hs), axis=1) - 3
+            assert clear.sum() > 400
+            np.testing.assert_array_equal(best.cpu().numpy()[clear], want_best[clear])
     np.testing.assert_array_equal(val[:, 3], -1.0)
 
 
