@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "k2_device.h"
@@ -852,16 +853,27 @@ extern "C" int k2_kernel_time(k2_table* t, double* ms, int64_t* launches) {
 }
 
 // ---- reference signatures ---------------------------------------------------------------------
-// State of the reference-signature entry points per device: one stream and one device buffer,
-// kept across calls (the buffer grows to the largest chunk seen), so that a driver's call loop
-// allocates and creates nothing after its first call.  The reference functions hold no state and
-// are reentrant; here concurrent calls on one device take turns on the context's mutex.
+// State of the reference-signature entry points per device: one stream, one device buffer and
+// two pinned host staging buffers, kept across calls (the device buffer grows to the largest
+// chunk seen), so that a driver's call loop allocates and creates nothing after its first call.
+// The reference functions hold no state and are reentrant; here concurrent calls on one device
+// take turns on the context's mutex.
+//
+// A and B live in the caller's pageable host memory (numpy arrays in the reference's drivers),
+// and they are the bulk of these calls' time (16 B per evaluation against a few ns of kernel per
+// thousand evaluations).  A copy between device memory and pageable memory runs at well under the
+// link's rate, so A and B move through the pinned staging buffers in pieces: the GPU copies piece
+// k into one buffer while host threads copy piece k - 1 out of (or into) the other.
 #define K2_COMPAT_CHUNK ((size_t)256 << 20)   // bytes of A/B rows per chunk on the device
+#define K2_COMPAT_STAGE ((size_t)64 << 20)    // bytes per pinned staging buffer (two)
+#define K2_COMPAT_THREADS 8                   // host threads of a staging copy, at most
 
 struct CompatCtx {
     hipStream_t st = nullptr;
     char* buf = nullptr;
     size_t cap = 0;
+    char* pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
     std::mutex mu;
 };
 
@@ -876,7 +888,19 @@ static CompatCtx* compat_ctx(hipError_t* err) {
     if (!g_ctx[dev]) {
         CompatCtx* c = new CompatCtx();
         *err = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
-        if (*err != hipSuccess) { delete c; return nullptr; }
+        for (int i = 0; i < 2 && *err == hipSuccess; ++i)
+            *err = hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming);
+        for (int i = 0; i < 2 && *err == hipSuccess; ++i)
+            *err = hipHostMalloc((void**)&c->pin[i], K2_COMPAT_STAGE, hipHostMallocDefault);
+        if (*err != hipSuccess) {
+            for (int i = 0; i < 2; ++i) {
+                if (c->pin[i]) (void)hipHostFree(c->pin[i]);
+                if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+            }
+            if (c->st) (void)hipStreamDestroy(c->st);
+            delete c;
+            return nullptr;
+        }
         g_ctx[dev] = c;
     }
     return g_ctx[dev];
@@ -889,6 +913,69 @@ static hipError_t compat_reserve(CompatCtx* c, size_t need) {
     c->cap = 0;
     hipError_t e = hipMalloc((void**)&c->buf, need);
     if (e == hipSuccess) c->cap = need;
+    return e;
+}
+
+// host memcpy of n bytes, split over up to K2_COMPAT_THREADS threads for large copies
+static void host_copy(char* dst, const char* src, size_t n) {
+    unsigned t = std::max(1u, std::min<unsigned>(K2_COMPAT_THREADS, std::thread::hardware_concurrency()));
+    if (n < ((size_t)4 << 20)) t = 1;
+    if (t == 1) { memcpy(dst, src, n); return; }
+    const size_t per = (n / t + 4095) / 4096 * 4096;
+    std::vector<std::thread> th;
+    th.reserve(t);
+    size_t done = 0;   // bytes handed to threads; the rest is copied here
+    try {
+        for (unsigned i = 0; i < t; ++i) {
+            const size_t a = std::min(n, i * per), b = std::min(n, a + per);
+            if (a < b) th.emplace_back([=] { memcpy(dst + a, src + a, b - a); });
+            done = b;
+        }
+    } catch (...) {   // a thread could not be started
+    }
+    if (done < n) memcpy(dst + done, src + done, n - done);
+    for (auto& x : th) x.join();
+}
+
+// n bytes from device memory to (pageable) host memory through the two pinned buffers: piece k
+// of K2_COMPAT_STAGE bytes goes to buffer k mod 2 on the stream; the host copies piece k - 1 out of
+// the other buffer meanwhile.  Returns when all n bytes are in dst.
+static hipError_t d2h_staged(CompatCtx* c, char* dst, const char* src, size_t n, hipStream_t st) {
+    const size_t S = K2_COMPAT_STAGE, np = (n + S - 1) / S;
+    hipError_t e = hipSuccess;
+    auto drain = [&](size_t k) {
+        const int b = (int)(k & 1);
+        hipError_t r = hipEventSynchronize(c->ev[b]);
+        if (r == hipSuccess) host_copy(dst + k * S, c->pin[b], std::min(S, n - k * S));
+        return r;
+    };
+    for (size_t k = 0; k < np && e == hipSuccess; ++k) {
+        const int b = (int)(k & 1);
+        if (k >= 2) e = drain(k - 2);   // buffer b held piece k - 2
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->pin[b], src + k * S, std::min(S, n - k * S), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(c->ev[b], st);
+    }
+    for (size_t k = np >= 2 ? np - 2 : 0; k < np && e == hipSuccess; ++k) e = drain(k);
+    return e;
+}
+
+// n bytes from (pageable) host memory to device memory through the pinned buffers: the host
+// fills buffer k mod 2 with piece k once the GPU has read piece k - 2 out of it, and the GPU
+// copies piece k - 1 meanwhile.  The copies are queued on the stream; the last two may still be
+// in flight at return (the buffers are not reused before their events).
+static hipError_t h2d_staged(CompatCtx* c, char* dst, const char* src, size_t n, hipStream_t st) {
+    const size_t S = K2_COMPAT_STAGE, np = (n + S - 1) / S;
+    hipError_t e = hipSuccess;
+    for (size_t k = 0; k < np && e == hipSuccess; ++k) {
+        const int b = (int)(k & 1);
+        const size_t len = std::min(S, n - k * S);
+        e = hipEventSynchronize(c->ev[b]);   // the GPU is done with what the buffer held
+        if (e != hipSuccess) break;
+        host_copy(c->pin[b], src + k * S, len);
+        e = hipMemcpyAsync(dst + k * S, c->pin[b], len, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipEventRecord(c->ev[b], st);
+    }
     return e;
 }
 
@@ -941,9 +1028,9 @@ extern "C" void compute_peak_AB(int n_peaks, const uint64_t* x_lo_arr, const uin
             if (e == hipSuccess)
                 e = launch_ref_peak_ab((int)r, ptrs, ddy, n_zeros, dal, dga, dA, dB, st);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(out_A + p0 * nz, dA, r * nz * 8, hipMemcpyDeviceToHost, st);
+                e = d2h_staged(c, (char*)(out_A + p0 * nz), (const char*)dA, r * nz * 8, st);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(out_B + p0 * nz, dB, r * nz * 8, hipMemcpyDeviceToHost, st);
+                e = d2h_staged(c, (char*)(out_B + p0 * nz), (const char*)dB, r * nz * 8, st);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
@@ -980,9 +1067,9 @@ extern "C" void polish_peaks_with_weights(int n_peaks, int n_zeros, const double
         if (e == hipSuccess && nz) e = hipMemcpyAsync(dw, w, nz * 8, hipMemcpyHostToDevice, st);
         for (size_t p0 = 0; p0 < np && e == hipSuccess; p0 += rows) {
             const size_t r = std::min(rows, np - p0);
-            if (nz) e = hipMemcpyAsync(dA, A + p0 * nz, r * nz * 8, hipMemcpyHostToDevice, st);
+            if (nz) e = h2d_staged(c, (char*)dA, (const char*)(A + p0 * nz), r * nz * 8, st);
             if (e == hipSuccess && nz)
-                e = hipMemcpyAsync(dB, B + p0 * nz, r * nz * 8, hipMemcpyHostToDevice, st);
+                e = h2d_staged(c, (char*)dB, (const char*)(B + p0 * nz), r * nz * 8, st);
             if (e == hipSuccess)
                 e = launch_ref_polish((int)r, (int)nz, dA, dB, dg, dw, grid_half_width, grid_steps,
                                       o, o + r, o + 2 * r, st);
