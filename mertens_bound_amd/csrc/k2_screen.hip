@@ -136,6 +136,23 @@ __device__ __forceinline__ void screen_term(const Limbs4& v, uint32_t vp, const 
     }
 }
 
+// zeros [z0, z1) in order, two per step: the second zero's phase and sin/cos do not wait for the
+// first one's sums, so the two chains interleave (the sums take the terms in the same order).
+// Measured against one zero per step: 0.79 against 0.84 ms for the first stage over 2^24
+// candidates; four per step took 0.83 ms there (and gained 10 us in the later stages).
+template <int M>
+__device__ __forceinline__ void screen_zeros(const Limbs4& v, uint32_t vp, const float4& vf,
+                                             const ScreenRec<M>* __restrict__ rec, int z0, int z1,
+                                             const Tab32* s_tab, ShiftSums<M>& acc) {
+    int j = z0;
+    for (; j + 1 < z1; j += 2) {
+        const ScreenRec<M> r0 = rec[j], r1 = rec[j + 1];
+        screen_term<M, true>(v, vp, vf, r0, s_tab, acc);
+        screen_term<M, true>(v, vp, vf, r1, s_tab, acc);
+    }
+    if (j < z1) screen_term<M, true>(v, vp, vf, rec[j], s_tab, acc);
+}
+
 // the stage value max_m |h_m| and the first shift that reaches it (m = -M..M in order)
 template <int M>
 __device__ __forceinline__ float stage_value(const ShiftSums<M>& acc, int& best) {
@@ -208,7 +225,7 @@ __global__ __launch_bounds__(256) void k2_screen_kernel(
         const uint32_t vp = pack_lo16(v.x2, v.x3);
         ShiftSums<M> acc;
         acc.zero();
-        for (int j = 0; j < nz; ++j) screen_term<M, true>(v, vp, vf, rec[j], s_tab, acc);
+        screen_zeros<M>(v, vp, vf, rec, 0, nz, s_tab, acc);
         int best;
         const float mv = stage_value<M>(acc, best);
         const bool next = screen_outputs(i, 0, n_stages, sp, mv, best, out_val, out_shift,
@@ -281,7 +298,7 @@ __global__ __launch_bounds__(256) void k2_screen_later_kernel(
         const uint32_t vp = pack_lo16(v.x2, v.x3);
         ShiftSums<M> acc;
         acc.zero();
-        for (int j = z_lo; j < z_hi; ++j) screen_term<M, true>(v, vp, vf, rec[j], s_tab, acc);
+        screen_zeros<M>(v, vp, vf, rec, z_lo, z_hi, s_tab, acc);
         if (w > 0) {
             s_part[w - 1][0][lane] = acc.a0;
 #pragma unroll
