@@ -47,6 +47,40 @@ def test_screen_values_vs_oracle(setup, oracle_lib):
     np.testing.assert_array_equal(val[:, 3], -1.0)
 
 
+@pytest.mark.parametrize("n", [1, 63, 65, 257, 3000])
+def test_screen_ragged_sizes(setup, oracle_lib, n):
+    """Candidate counts that fill no whole wave or workgroup: every stage value against the
+    oracle, and with thresholds that stop some candidates, the staged outputs against the run in
+    which every candidate takes every stage (bit for bit)."""
+    tb, t, torch = setup
+    k, d = 3e-9, 0.0014
+    V = synth.make_candidates(n, 3.4e18, 3.6e18, 100 + n)
+    Vt = ev.v_tensor(V)
+    allv, allb, allok = t.screen_device(Vt, k, d, 3, ((64, 0.0), (128, 0.0), (256, 0.0)))
+    allv = allv.cpu().numpy()
+    assert np.all(allok.cpu().numpy() == 1)
+    for s, nz in enumerate((64, 128, 256)):
+        hs = np.stack([oracle_lib.eval_f64(tb, _shift_int(V, m, d), k, flags=1, n_zeros=nz)[0]
+                       for m in range(-3, 4)], axis=1)
+        scale = np.sum(2 * np.exp(-k * tb.gamma[:nz] ** 2) * np.abs(tb.w[:nz]))
+        assert np.max(np.abs(allv[:, s] - np.max(np.abs(hs), axis=1))) <= 2e-6 * scale
+    thr = np.float32(np.median(allv[:, 0])) if n > 1 else np.float32(0.0)
+    stages = ((64, float(thr)), (128, 0.0), (256, 0.0))
+    val, best, ok = (x.cpu().numpy() for x in t.screen_device(Vt, k, d, 3, stages))
+    reach = allv[:, 0] >= thr
+    assert np.array_equal(ok == 1, reach)
+    assert np.array_equal(val[reach, :3], allv[reach, :3])
+    assert np.all(val[~reach, 1:] == -1)
+    assert np.array_equal(best[reach], allb.cpu().numpy()[reach])
+
+
+def test_screen_no_candidates(setup):
+    tb, t, torch = setup
+    Vt = ev.v_tensor(synth.make_candidates(1, 3.4e18, 3.6e18, 1))[:0]
+    val, best, ok = t.screen_device(Vt, 3e-9, 0.0014, 3, ((64, 0.0), (128, 0.0), (256, 0.0)))
+    assert val.shape[0] == 0 and ok.shape[0] == 0
+
+
 def test_screen_staging_and_survivors(setup):
     tb, t, torch = setup
     V = synth.make_candidates(4096, 3.4e18, 3.6e18, 22)
