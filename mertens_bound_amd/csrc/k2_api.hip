@@ -869,11 +869,14 @@ extern "C" int k2_kernel_time(k2_table* t, double* ms, int64_t* launches) {
 #define K2_COMPAT_THREADS 8                   // host threads of a staging copy, at most
 
 struct CompatCtx {
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;       // kernels and the copies back
+    hipStream_t up = nullptr;       // uploads of A and B (polish_peaks_with_weights)
     char* buf = nullptr;
     size_t cap = 0;
     char* pin[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};        // the pinned buffers' last copies
+    hipEvent_t loaded[2] = {nullptr, nullptr};    // a device half's upload done
+    hipEvent_t used[2] = {nullptr, nullptr};      // a device half's kernel done
     std::mutex mu;
 };
 
@@ -888,15 +891,22 @@ static CompatCtx* compat_ctx(hipError_t* err) {
     if (!g_ctx[dev]) {
         CompatCtx* c = new CompatCtx();
         *err = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
-        for (int i = 0; i < 2 && *err == hipSuccess; ++i)
+        if (*err == hipSuccess) *err = hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking);
+        for (int i = 0; i < 2 && *err == hipSuccess; ++i) {
             *err = hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming);
+            if (*err == hipSuccess) *err = hipEventCreateWithFlags(&c->loaded[i], hipEventDisableTiming);
+            if (*err == hipSuccess) *err = hipEventCreateWithFlags(&c->used[i], hipEventDisableTiming);
+        }
         for (int i = 0; i < 2 && *err == hipSuccess; ++i)
             *err = hipHostMalloc((void**)&c->pin[i], K2_COMPAT_STAGE, hipHostMallocDefault);
         if (*err != hipSuccess) {
             for (int i = 0; i < 2; ++i) {
                 if (c->pin[i]) (void)hipHostFree(c->pin[i]);
                 if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+                if (c->loaded[i]) (void)hipEventDestroy(c->loaded[i]);
+                if (c->used[i]) (void)hipEventDestroy(c->used[i]);
             }
+            if (c->up) (void)hipStreamDestroy(c->up);
             if (c->st) (void)hipStreamDestroy(c->st);
             delete c;
             return nullptr;
@@ -1047,40 +1057,52 @@ extern "C" void polish_peaks_with_weights(int n_peaks, int n_zeros, const double
                                           double* out_ddy_pol) {
     if (n_peaks <= 0) return;
     const size_t np = (size_t)n_peaks, nz = (size_t)std::max(n_zeros, 0);
-    // peaks in chunks of at most K2_COMPAT_CHUNK bytes of A/B rows (at least one row)
+    // peaks in chunks of at most K2_COMPAT_CHUNK bytes of A/B rows (at least one row), in two
+    // device halves: the upload stream fills half (i mod 2) with chunk i while the kernel stream
+    // runs chunk i - 1 out of the other half
     const size_t rows = nz ? std::max<size_t>(1, std::min<size_t>(np, K2_COMPAT_CHUNK / (16 * nz)))
                            : np;
-    // device layout: gamma, w (nz each) | h_unpol, h_pol, ddy (rows each) | A B (rows x nz)
-    const size_t off_o = align256(nz * 16), off_ab = off_o + align256(rows * 24);
+    // device layout: gamma, w (nz each) | h_unpol, h_pol, ddy (np each) | 2 x (A B) (rows x nz)
+    const size_t off_o = align256(nz * 16), off_ab = off_o + align256(np * 24);
+    const size_t half = align256(rows * nz * 16);
     hipError_t e = hipSuccess;
     CompatCtx* c = compat_ctx(&e);
     if (c) {
         std::lock_guard<std::mutex> lk(c->mu);
-        hipStream_t st = c->st;
-        e = compat_reserve(c, off_ab + rows * nz * 16 + 64);
+        hipStream_t st = c->st, up = c->up;
+        e = compat_reserve(c, off_ab + 2 * half + 64);
         double* dg = (double*)c->buf;
         double* dw = dg + nz;
-        double* o = (double*)(c->buf + off_o);
-        double* dA = (double*)(c->buf + off_ab);
-        double* dB = dA + rows * nz;
+        double* oh = (double*)(c->buf + off_o);
+        double* op = oh + np;
+        double* od = op + np;
         if (e == hipSuccess && nz) e = hipMemcpyAsync(dg, gamma, nz * 8, hipMemcpyHostToDevice, st);
         if (e == hipSuccess && nz) e = hipMemcpyAsync(dw, w, nz * 8, hipMemcpyHostToDevice, st);
-        for (size_t p0 = 0; p0 < np && e == hipSuccess; p0 += rows) {
+        size_t i = 0;
+        for (size_t p0 = 0; p0 < np && e == hipSuccess; p0 += rows, ++i) {
             const size_t r = std::min(rows, np - p0);
-            if (nz) e = h2d_staged(c, (char*)dA, (const char*)(A + p0 * nz), r * nz * 8, st);
+            const int h = (int)(i & 1);
+            double* dA = (double*)(c->buf + off_ab + h * half);
+            double* dB = dA + r * nz;
+            if (i >= 2) e = hipStreamWaitEvent(up, c->used[h], 0);   // chunk i - 2 is done with it
             if (e == hipSuccess && nz)
-                e = h2d_staged(c, (char*)dB, (const char*)(B + p0 * nz), r * nz * 8, st);
+                e = h2d_staged(c, (char*)dA, (const char*)(A + p0 * nz), r * nz * 8, up);
+            if (e == hipSuccess && nz)
+                e = h2d_staged(c, (char*)dB, (const char*)(B + p0 * nz), r * nz * 8, up);
+            if (e == hipSuccess) e = hipEventRecord(c->loaded[h], up);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, c->loaded[h], 0);
             if (e == hipSuccess)
                 e = launch_ref_polish((int)r, (int)nz, dA, dB, dg, dw, grid_half_width, grid_steps,
-                                      o, o + r, o + 2 * r, st);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(out_h_unpol + p0, o, r * 8, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(out_h_pol + p0, o + r, r * 8, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(out_ddy_pol + p0, o + 2 * r, r * 8, hipMemcpyDeviceToHost, st);
+                                      oh + p0, op + p0, od + p0, st);
+            if (e == hipSuccess) e = hipEventRecord(c->used[h], st);
         }
+        if (e == hipSuccess) e = hipMemcpyAsync(out_h_unpol, oh, np * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(out_h_pol, op, np * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(out_ddy_pol, od, np * 8, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
+        // a failed call leaves no upload or kernel behind that could still read the buffers
+        if (e != hipSuccess) (void)hipStreamSynchronize(st);
+        (void)hipStreamSynchronize(up);
     }
     if (e != hipSuccess) {
         fail(K2_E_HIP, std::string("polish_peaks_with_weights: ") + hipGetErrorString(e));
