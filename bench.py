@@ -33,7 +33,8 @@ sys.path.insert(0, ROOT)
 #   precision form:        2 transcendentals (sin, cos) + 8 FP32 flops (weight, compensated sum)
 #                          + 15 INT32 multiply-adds per FP32-tail evaluation; its FP64 head at the
 #                          plain constants
-#   enclosure K2 + K2':    the plain work + 12 FP64 additions (TwoSum accumulation of K2, K2')
+#   enclosure K2 + K2':    the plain work + 1.5 FP64 additions (TwoSum of the K2 and K2' group
+#                          sums, one per 8 zeros)
 #   leaf screen (one zero, 7 shifted sums): 15 INT32 multiply-adds + 2 transcendentals + 19 FP32
 #                          flops (weight rotation 6, unshifted sum 1, per shift pair the two FMAs
 #                          of Re q cos(m g d) and Im q sin(m g d), whose sum and difference are the
@@ -45,7 +46,7 @@ sys.path.insert(0, ROOT)
 ALG = {"plain": {"fp64_flops": 42, "int32_mad": 15},
        "plain_d2": {"fp64_flops": 46, "int32_mad": 15},
        "mixed": {"trans": 2, "fp32_flops": 8, "int32_mad": 15},
-       "enclose": {"fp64_flops": 54, "int32_mad": 15},
+       "enclose": {"fp64_flops": 43.5, "int32_mad": 15},
        "screen": {"int32_mad": 15, "trans": 2, "fp32_flops": 19}}
 PUBLISHED_PEAKS = {"fp64_flops": 78.6e12, "fp32_flops": 157.3e12}
 # measured once on gfx950 (k2_valu_rates, lane operations/s); used only where the live
@@ -54,9 +55,9 @@ RECORDED_RATES = {"int32_mad": 33.0e12, "trans": 9.8e12}
 # As-executed VALU issue model (not the roofline): full-rate slot-equivalents per evaluation,
 # one wave64 instruction per 4 cycles per SIMD = 39.3 T lane-slots/s on 256 CUs at 2.4 GHz;
 # FP32 instructions issue in half a slot.  plain: 15 INT32 multiply-adds + 21 FP64 FMAs; mixed
-# FP32 tail as built: 6 integer instructions + 15 FP32 at half a slot; enclosure: + 12 FP64
+# FP32 tail as built: 6 integer instructions + 15 FP32 at half a slot; enclosure: + 1.5 FP64
 # additions; screen: 8 integer + 22 FP32 instructions per zero.
-SLOTS = {"plain": 36.0, "plain_d2": 38.0, "mixed": 13.5, "enclose": 48.0, "screen": 19.0}
+SLOTS = {"plain": 36.0, "plain_d2": 38.0, "mixed": 13.5, "enclose": 37.5, "screen": 19.0}
 VALU_SLOT_PEAK = 39.3e12
 # SQ_INSTS_VALU per evaluation from the committed counter summaries (wave instructions x 64 /
 # evaluations)

@@ -20,18 +20,25 @@
 //           fraction below W: < 2 pi 2^-61 rad = 2.8e-18 (0.03 u)
 //   sin/cos sincos_tab64_at: table rounding 0.5 u, remainder polynomial 1.2 u, combine 0.55 u,
 //           the dropped bits 0.03 u -> |S - sin|, |C - cos| <= 2.3 u (bounded here by 3 u)
-//   term    fma(a.x, C, -a.y S): sqrt(2) |a| 3 u for S, C, then two roundings of at most |a| u
-//           each: 6.25 |a| u
-//   =>      |term_j - T_j| <= |A_j| (10 + 9 x_j) u                  (K2: 6.25 + 3.5 + 9 x)
-//           |term'_j - T'_j| <= g_j |A_j| (14.5 + 9 x_j) u         (K2': 6.25 + 8 + 9 x)
-//   sums    TwoSum (Knuth) per lane and across splits: every addition's rounding error is kept
-//           exactly in a second word; only the second word's own sum rounds, which adds at most
-//           (m u)^2 sum|a_j| (m = zeros + 2 splits + 2), and the final hi + lo rounds by u |K|.
+//   term    the exact products a.x C, a.y S (K2) and b.x S, b.y C (K2') at the computed S, C:
+//           against a.x cos - a.y sin, sqrt(2) |a| 3 u = 4.25 |a| u
+//   group   the products of K2_ENC_GROUP = 8 consecutive zeros enter a group sum by FMAs, 16
+//           roundings: at most gamma_16 = 16 u / (1 - 16 u) times the sum of their magnitudes,
+//           |a.x| |C| + |a.y| |S| <= sqrt(2) |a| (1 + 3 u), i.e. 22.7 |a| u per zero (the same
+//           with b for K2')
+//   =>      |term_j - T_j| <= |A_j| (30.5 + 9 x_j) u            (K2: 4.25 + 3.5 + 22.7 + 9 x)
+//           |term'_j - T'_j| <= g_j |A_j| (35 + 9 x_j) u         (K2': 4.25 + 8 + 22.7 + 9 x)
+//   sums    TwoSum (Knuth) of each group sum per lane and across splits: every addition's
+//           rounding error is kept exactly in a second word; only the second word's own sum
+//           rounds, which adds at most (m u)^2 sum|a_j| (m = zeros + 2 splits + 2 bounds the
+//           additions), and the final hi + lo rounds by u |K|.
 //   Zeros with x_j > 708 get weight 0 and contribute 2 |w_j| 1e-307 to the bound; 1e-300 covers
 //   every absolute (subnormal) rounding.  The bound sums are formed in FP64 and the whole radius
 //   is scaled by 1.01, which covers their own rounding (< 1e-9 relative) and |A| vs |a|.
 // The radius is R = 1.01 (u B + (m u)^2 sum|a| + u |K|) + 1e-300 with B = sum_{j<N} |a_j|
-// (10 + 9 x_j); the zeros of the table past N contribute at most tail = 1.01 sum_{j>=N} |a_j|.
+// (30.5 + 9 x_j); the zeros of the table past N contribute at most tail = 1.01 sum_{j>=N} |a_j|.
+// Summing each term by TwoSum (two FMAs and six additions per sum and zero) gave a radius a
+// third of this size at 29% more time (221 against 171 ms on 65,536 x 2,325,572).
 // tests/test_enclose_host.py replays g, x, a and b on the host and checks each line's bound
 // against mpmath for x in [0, 708].
 #include <hip/hip_runtime.h>
@@ -86,8 +93,8 @@ __global__ __launch_bounds__(256) void k2_enclose_weights_kernel(
             vb = make_double2(g * va.x, g * va.y);
             mag = sqrt(va.x * va.x + va.y * va.y);
             magb = g * mag;
-            m = mag * (10.0 + 9.0 * x);
-            mb = magb * (14.5 + 9.0 * x);
+            m = mag * (30.5 + 9.0 * x);
+            mb = magb * (35.0 + 9.0 * x);
         } else {
             mag = 2.0 * sqrt(ww.x * ww.x + ww.y * ww.y) * 1e-307;
             magb = g * mag;
@@ -146,7 +153,7 @@ __global__ __launch_bounds__(256, 2) void k2_enclose_kernel(
     const int tid = threadIdx.x;
     for (int i = tid; i < K2_TAB; i += K2_WG) s_tab[i] = tab64[i];
     Limbs4 v[NC];
-    AccDD acc[NC];
+    AccGD acc[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         const int c = ct * (K2_WG * NC) + i * K2_WG + tid;

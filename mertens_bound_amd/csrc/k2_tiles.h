@@ -96,6 +96,39 @@ __device__ __forceinline__ void accumulate(AccDD& A, double S, double C, double2
     if (FL & K2_D1) two_sum_acc(A.h1, A.l1, -fma(b.x, S, b.y * C));
 }
 
+// Grouped error-free sums (enclosure mode): the products of up to K2_ENC_GROUP consecutive zeros
+// go into a plain FP64 group sum by FMAs (2 K2_ENC_GROUP roundings per group, bounded in the
+// radius), and each group sum enters hi + lo by TwoSum (exact).  flush() ends a group.
+#define K2_ENC_GROUP 8
+struct AccGD {
+    double g0, g1, h0, l0, h1, l1;
+    __device__ __forceinline__ void zero() { g0 = g1 = h0 = l0 = h1 = l1 = 0.0; }
+    template <int FL>
+    __device__ __forceinline__ void flush() {
+        two_sum_acc(h0, l0, g0);
+        g0 = 0.0;
+        if (FL & K2_D1) {
+            two_sum_acc(h1, l1, g1);
+            g1 = 0.0;
+        }
+    }
+};
+
+template <int FL>
+__device__ __forceinline__ void accumulate(AccGD& A, double S, double C, double2 a, double2 b,
+                                           double2 e) {
+    A.g0 = fma(a.x, C, A.g0);
+    A.g0 = fma(-a.y, S, A.g0);
+    if (FL & K2_D1) {
+        A.g1 = fma(-b.x, S, A.g1);
+        A.g1 = fma(-b.y, C, A.g1);
+    }
+}
+
+// zeros per group of an accumulator (0: no groups)
+template <class ACC> struct AccGroup { static constexpr int value = 0; };
+template <> struct AccGroup<AccGD> { static constexpr int value = K2_ENC_GROUP; };
+
 // one zero's FP32 terms (K, K', K'') added into a group sum with two FMAs each
 template <int FL>
 __device__ __forceinline__ void add_terms(float3& p, float S, float C, float2 a, float2 b,
@@ -153,8 +186,9 @@ constexpr size_t ctile_smem_bytes() {
                                                                : sizeof(CTileSmem32<FL>);
 }
 
-// Sums zeros [z0, z1) into acc for the NC candidates of this lane (ACC: Acc<false>, or AccDD
-// in the enclosure mode).  Enters and leaves with sm free (the last action is a barrier).
+// Sums zeros [z0, z1) into acc for the NC candidates of this lane (ACC: Acc<false>, or AccGD
+// in the enclosure mode: its group sums are flushed every K2_ENC_GROUP zeros and at tile ends).
+// Enters and leaves with sm free (the last action is a barrier).
 // tab: the (cos, sin) table in LDS of the matching width.
 // NT threads per workgroup (256, or 512: two threads per zero slot share the staging, the first
 // half g and a, the second half b and e, and twice the candidates read each staged record).
@@ -217,6 +251,16 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
                 sincos_tab64_at(phase_rem(g, v[i]), tab, S, C, kc);
                 accumulate<FL>(acc[i], S, C, aa, bb, ee);
             }
+            if constexpr (AccGroup<ACC>::value > 0) {
+                if (z % AccGroup<ACC>::value == AccGroup<ACC>::value - 1) {   // uniform: z is
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) acc[i].template flush<FL>();
+                }
+            }
+        }
+        if constexpr (AccGroup<ACC>::value > 0) {   // a group never spans two tiles
+#pragma unroll
+            for (int i = 0; i < NC; ++i) acc[i].template flush<FL>();
         }
         if (t + 1 < ntiles) stage(buf ^ 1, pf);
         __syncthreads();

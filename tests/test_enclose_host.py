@@ -49,7 +49,7 @@ def test_weight_lines_within_stated_bounds():
     a = 2 e^{-x} w per component, b = g a) against mpmath at 60 digits, line by line, for
     gamma over the table's range and x over [0, 708]: |g - gamma| <= 3.5 u gamma,
     |x - X| <= 9 u X, |a - A| <= (3.5 + 9 X) u |A|, |b - B| <= (8 + 9 X) u |B| (the per-term
-    budget of k2_enclose.hip adds the sin/cos's 3 u and the term's two roundings)."""
+    budget of k2_enclose.hip adds the sin/cos's 3 u and the group sums' roundings)."""
     import mpmath
     rng = np.random.default_rng(11)
     n = 3000
