@@ -551,6 +551,13 @@ def run_polish(args, form=None, ctx=None):
     V0t = ev.v_tensor(V0)
     k = c["k"]
     mixed = (form or args.form) == "mixed"
+    # the tuned tiling of the polish's passes (the table may be shared with other sub-records)
+    from mertens_bound_amd import autotune
+    prev_tiling = t.get_tiling()
+    ti = autotune.polish_tiling("mixed" if mixed else "plain")
+    if ti:
+        t.set_tiling(nc=ti["nc"], target_blocks=ti["target_blocks"],
+                     wg_threads=ti.get("wg_threads", 256))
 
     def step():
         ag.polish(t, V0t, k, n_steps=8, mixed=mixed)
@@ -602,7 +609,10 @@ def run_polish(args, form=None, ctx=None):
                                tb.n * 64 + V0.shape[0] * 40, "polish_" + ("mixed" if mixed else
                                                                           "plain"),
                                t.mixed_head(k) / tb.n if mixed else 0.0)
+    res["config"]["tiling"] = t.get_tiling()
     res["wall_s"] = wall
+    t.set_tiling(nc=prev_tiling["nc"], target_blocks=prev_tiling["target_blocks"],
+                 wg_threads=prev_tiling["wg_threads"])
     return res
 
 

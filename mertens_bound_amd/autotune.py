@@ -6,10 +6,13 @@ Tunable shapes:
                          zero splits aim at, i.e. zeros per split)
   batched plan           nc x cz (candidates per Z tile) x chunk (zeros per work item) x
                          wg_per_cu (resident workgroups of the persistent grid)
+  Newton polish          nc x wg_threads x target_blocks of the K2/K2'/K2'' passes, timed over
+                         whole polishes (4,096 candidates near peaks, 8 steps), whose passes
+                         shrink as candidates converge
 
 Each candidate shape is timed with HIP events on the GPU; the fastest per shape class is written
 to tuned_tiles.json beside this file, which the host layer consults (``apply_tuned``,
-``batch_tiling``).  Shape classes bucket the candidate and zero counts by powers of two.
+``batch_tiling``, ``polish_tiling``).  Shape classes bucket the candidate and zero counts by powers of two.
 
 Run on the GPU:  python -m mertens_bound_amd.autotune [--out PATH] [--quick]
 """
@@ -30,6 +33,7 @@ EVAL_GRID = {"nc": [1, 2, 4], "target_blocks": [1536, 3072, 6144, 12288],
              "wg_threads": [256, 512]}
 BATCH_GRID = {"nc": [2, 4], "cz": [8, 16], "chunk": [4096, 8192, 16384], "wg_per_cu": [0],
               "c_min_fill": [128, 256, 384]}
+POLISH_GRID = {"nc": [2, 4], "target_blocks": [3072, 6144, 12288], "wg_threads": [256, 512]}
 
 _cache = None
 
@@ -79,6 +83,12 @@ def apply_tuned(table, n_cand: int, n_zeros: int, form: str = "plain") -> bool:
 
 def batch_tiling(form: str = "plain"):
     rec = load().get("batch", {}).get(form)
+    return None if rec is None else rec["tiling"]
+
+
+def polish_tiling(form: str = "plain"):
+    """Tuned tiling of the Newton polish's passes (nc, target_blocks, wg_threads), or None."""
+    rec = load().get("polish", {}).get(form)
     return None if rec is None else rec["tiling"]
 
 
@@ -138,17 +148,34 @@ def tune_batch(table, reqs, V, form, grid=BATCH_GRID, log=print):
     return results
 
 
+def tune_polish(table, V0, k, form, grid=POLISH_GRID, log=print):
+    import torch
+    from . import autograd as ag
+    results = []
+    for nc, tb, wg in itertools.product(grid["nc"], grid["target_blocks"],
+                                        grid.get("wg_threads", [256])):
+        if wg == 512 and nc > 2:
+            continue
+        table.set_tiling(nc=nc, target_blocks=tb, wg_threads=wg)
+        ms = _time(lambda: ag.polish(table, V0, k, n_steps=8, mixed=(form == "mixed")), torch)
+        results.append({"tiling": {"nc": nc, "target_blocks": tb, "wg_threads": wg}, "ms": ms})
+        log(f"  polish {form} nc={nc} wg={wg} target_blocks={tb}: {ms:.3f} ms")
+    table.set_tiling(nc=2, target_blocks=3072, wg_threads=256)
+    results.sort(key=lambda r: r["ms"])
+    return results
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=TUNED)
     ap.add_argument("--quick", action="store_true", help="smaller grids")
-    ap.add_argument("--only", choices=("eval", "batch"), default=None,
-                    help="tune one kind; the other kind's entries are kept from the current file")
+    ap.add_argument("--only", choices=("eval", "batch", "polish"), default=None,
+                    help="tune one kind; the other kinds' entries are kept from the current file")
     args = ap.parse_args()
     import numpy as np
     import torch
     from . import evaluator as ev
-    from . import synth
+    from . import synth, workloads
 
     def log(msg):
         print(msg, flush=True)
@@ -166,12 +193,14 @@ def main():
     small = synth.make_zero_table(10_000, 0)
     t_small = ev.K2Table(small)
     prev = load()
-    out = {"eval": dict(prev.get("eval", {})) if args.only == "batch" else {},
-           "batch": dict(prev.get("batch", {})) if args.only == "eval" else {},
+    keep = lambda kind: args.only not in (None, kind)
+    out = {"eval": dict(prev.get("eval", {})) if keep("eval") else {},
+           "batch": dict(prev.get("batch", {})) if keep("batch") else {},
+           "polish": dict(prev.get("polish", {})) if keep("polish") else {},
            "device": torch.cuda.get_device_name(0),
            "created": time.strftime("%Y-%m-%d %H:%M:%S")}
     shapes = [(t_big, big.n, 65_536), (t_big, big.n, 4_096), (t_small, small.n, 1_024)]
-    if args.only == "batch":
+    if args.only not in (None, "eval"):
         shapes = []
     for form in ("plain", "mixed"):
         for t, nz, nc in shapes:
@@ -183,10 +212,16 @@ def main():
     nzr, ks, cnt, offs, Vb = synth.make_batched_requests(big, n_req=512, seed=0)
     reqs = [(int(a), int(b), int(o), float(k)) for a, b, o, k in zip(nzr, cnt, offs, ks)]
     Vbt = ev.v_tensor(Vb)
-    for form in (() if args.only == "eval" else ("plain", "mixed")):
+    for form in (("plain", "mixed") if args.only in (None, "batch") else ()):
         res = tune_batch(t_big, reqs, Vbt, form, bg, log)
         out["batch"][form] = {"tiling": res[0]["tiling"], "ms": res[0]["ms"], "all": res}
         log(f"best batch {form}: {res[0]}")
+    if args.only in (None, "polish"):
+        V0 = ev.v_tensor(workloads.polish_candidates(t_big, 4096, seed=0))
+        for form in ("plain", "mixed"):
+            res = tune_polish(t_big, V0, c2["k"], form, POLISH_GRID, log)
+            out["polish"][form] = {"tiling": res[0]["tiling"], "ms": res[0]["ms"], "all": res}
+            log(f"best polish {form}: {res[0]}")
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)
