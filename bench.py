@@ -447,6 +447,16 @@ def run_eval(args, form):
         res["enclose"] = sub_record(run_enclose(args, ctx),
                                     keep=("value", "unit", "ms_per_step", "config", "accuracy",
                                           "radius_max_all", "roofline", "wall_s"))
+        # the CPU-runnable case (1,024 candidates x 10,000 zeros) in both forms: every
+        # candidate against binary128 (plain) and against the FP64 kernel (precision form)
+        a1 = argparse.Namespace(**vars(args))
+        a1.workload, a1.n_cand, a1.steps, a1.warmup = "small", 0, max(args.steps, 50), 5
+        keep1 = ("value", "unit", "ms_per_step", "dtype", "config", "accuracy", "roofline",
+                 "vs_plain", "wall_s")
+        s1p = sub_record(run_eval(a1, "plain"), keep=keep1)
+        s1m = sub_record(run_eval(a1, "mixed"), keep=keep1)
+        res["config1"] = {"plain": s1p, "mixed": s1m,
+                          "mixed_speedup": s1p["ms_per_step"] / s1m["ms_per_step"]}
     if not args.no_cpu and args.workload == "metric":
         try:
             cb = cpu_baseline(tb, V, c["k"])
