@@ -91,3 +91,41 @@ def test_polish_peaks_with_weights_repeated_calls(oracle_lib):
         A0, B0, rw["gamma"], np.exp(-1.5e-10 * rw["gamma"] ** 2), 2.5e-4, 40)
     np.testing.assert_allclose(outs[1][0], hu_r, rtol=0, atol=1e-12)
     np.testing.assert_allclose(outs[1][1], hp_r, rtol=0, atol=1e-10)
+
+
+def test_shims_in_many_chunks_and_pieces(oracle_lib):
+    """Arrays larger than the device staging: 2,000 peaks x 20,000 zeros are three device chunks
+    of 838 rows (A and B of a chunk fill the 256 MiB buffer), each array chunk 134 MB moving
+    through the two 64 MiB pinned buffers in three pieces, and the polish uploads the chunks
+    into alternating device halves on its second stream.  Rows on both sides of every chunk
+    boundary and a random sample against the oracle's restatement of the reference."""
+    from mertens_bound_amd import formats, synth
+    tb = synth.make_zero_table(20_000, 9)
+    rw = formats.reference_words(tb)
+    npk, nz = 2000, tb.n
+    V = synth.make_candidates(npk, 3.4e18, 3.6e18, 9)
+    xs, dy = formats.candidates_to_x_dy(V)
+    rng = np.random.default_rng(9)
+    dy = dy + rng.uniform(-4e-3, 4e-3, npk)
+    xlo, xhi = formats.x_words(xs)
+    zw = (rw["alpha"], rw["gamma"], rw["ghi_lo"], rw["ghi_hi"], rw["glo_lo"], rw["glo_hi"],
+          rw["psi_lo"], rw["psi_hi"])
+    A = np.empty((npk, nz)); B = np.empty((npk, nz))
+    ev.compute_peak_AB(npk, xlo, xhi, dy, nz, *zw, A, B)
+    sel = np.unique(np.concatenate([[0, 1, 837, 838, 839, 1675, 1676, 1677, 1999],
+                                    rng.choice(npk, 24, replace=False)]))
+    A0, B0 = oracle_lib.ref_compute_peak_AB(xlo[sel], xhi[sel], dy[sel], *zw)
+    th_max = 2 * np.pi + rw["gamma"][None, :] * np.abs(dy[sel])[:, None]
+    tol = 2 * rw["alpha"][None, :] * (th_max * 2.3e-16 + 4.5e-16)
+    assert np.all(np.abs(A[sel] - A0) <= tol) and np.all(np.abs(B[sel] - B0) <= tol)
+    w = np.exp(-3e-9 * rw["gamma"] ** 2)
+    hu = np.empty(npk); hp = np.empty(npk); dd = np.empty(npk)
+    ev.polish_peaks_with_weights(npk, nz, A, B, rw["gamma"], w, 2.5e-4, 5, hu, hp, dd)
+    hu0, hp0, dd0 = oracle_lib.ref_polish_peaks_with_weights(A[sel], B[sel], rw["gamma"], w,
+                                                             2.5e-4, 5)
+    print(f"2,000 x 20,000 shims: max |dA| {np.abs(A[sel] - A0).max():.2e}, h_unpol "
+          f"{np.abs(hu[sel] - hu0).max():.2e}, h_pol {np.abs(hp[sel] - hp0).max():.2e}, ddy "
+          f"{np.abs(dd[sel] - dd0).max():.2e}")
+    np.testing.assert_allclose(hu[sel], hu0, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(hp[sel], hp0, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(dd[sel], dd0, rtol=0, atol=1e-9)
