@@ -61,7 +61,7 @@ SLOTS = {"plain": 36.0, "plain_d2": 38.0, "mixed": 13.5, "enclose": 37.5, "scree
 VALU_SLOT_PEAK = 39.3e12
 # SQ_INSTS_VALU per evaluation from the committed counter summaries (wave instructions x 64 /
 # evaluations)
-VALU_INSTS = {"plain": 41.7, "mixed": 22.7, "screen": 30.0}
+VALU_INSTS = {"plain": 41.7, "mixed": 22.7, "enclose": 44.8, "screen": 30.7}
 _RATES = {}
 
 
