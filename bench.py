@@ -575,9 +575,9 @@ def run_polish(args, form=None, ctx=None):
     ms, wall = timed(step, args.steps, args.warmup, torch)
     V, K, dK, steps = ag.polish(t, V0t, k, n_steps=8, mixed=mixed)
     torch.cuda.synchronize()
-    # candidate evaluations per polish: the start value, then the Newton passes (converged
-    # candidates drop out after each step) and the final value
-    cand_evals = V0.shape[0] + t.polish_stats()
+    # candidate evaluations per polish: the Newton passes (the first, over all candidates, also
+    # gives the start values; converged candidates drop out after each step) and the final value
+    cand_evals = t.polish_stats()
     evals = float(cand_evals) * tb.n
     res = base_record(args, "candidate-zero evals/sec, Newton polish (K2, K2', K2'')",
                       evals / (ms * 1e-3), "candidate-zero evals/s", ms,

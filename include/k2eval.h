@@ -159,10 +159,12 @@ int k2_batch_destroy(k2_batch *b);
  * clamped to +-clamp, stop once |step| < 1e-13); the FP64 step is applied exactly to the
  * 64.64 word.  With K2_MIXED the sums run in the precision form and a candidate stops once
  * |step| < 1e-11, the form's noise floor in v (FP32 rounding in K2' moves converged points by
- * ~5e-12).  K/dK/d2K receive the values at the final point; steps_dev the iterations taken. */
+ * ~5e-12).  K/dK/d2K receive the values at the final point; steps_dev the iterations taken;
+ * K0_dev (may be NULL) K at the starting points, in the same form, taken from the first pass
+ * (the reference's unpolished h0, path1_eval_fast.cpp:77-81, without a pass of its own). */
 int k2_polish_device(k2_table *t, uint32_t n_cand, uint64_t *V_dev, double k, uint32_t n_zeros,
                      int n_steps, double clamp, uint32_t flags, double *K_dev, double *dK_dev,
-                     double *d2K_dev, int32_t *steps_dev, void *stream);
+                     double *d2K_dev, int32_t *steps_dev, double *K0_dev, void *stream);
 /* Candidate evaluations (K, K', K'' passes x candidates) of the last k2_polish_device call on t:
  * after the first step only the candidates still moving are evaluated again. */
 int k2_polish_stats(k2_table *t, int64_t *cand_evals);

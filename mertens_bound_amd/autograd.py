@@ -145,11 +145,11 @@ def polish(table: ev.K2Table, V0: torch.Tensor, k: float, n_steps: int = 8, clam
     the next pass).  Same rule and return values as :func:`newton_polish`; with
     mixed=True the sums run in the precision form and a candidate stops once |step| < 1e-11, the
     noise floor that form's FP32 rounding leaves in v."""
-    K0, _, _ = table.eval_device(V0, k, n_zeros=n_zeros, flags=ev.VALUE)
-    K0 = K0.clone()
     V = V0.clone()
-    K, dK, d2K, steps = table.polish_device(V, k, n_steps=n_steps, clamp=clamp, n_zeros=n_zeros,
-                                            flags=ev.MIXED if mixed else 0)
+    # K0 (the unpolished value) comes from the polish's first pass, which is at V0
+    K, dK, d2K, steps, K0 = table.polish_device(V, k, n_steps=n_steps, clamp=clamp,
+                                                n_zeros=n_zeros, flags=ev.MIXED if mixed else 0,
+                                                start=True)
     keep = K0.abs() > K.abs()
     V = torch.where(keep[:, None], V0, V)
     Kf = torch.where(keep, K0, K)

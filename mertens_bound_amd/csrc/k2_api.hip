@@ -671,7 +671,7 @@ extern "C" int k2_eval(k2_table* t, uint32_t n_cand, const uint64_t* V, double k
 extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, double k,
                                 uint32_t n_zeros, int n_steps, double clamp, uint32_t flags,
                                 double* K_dev, double* dK_dev, double* d2K_dev, int32_t* steps_dev,
-                                void* stream) {
+                                double* K0_dev, void* stream) {
     if (!t || (n_cand > 0 && (!V_dev || !K_dev || !dK_dev || !d2K_dev || !steps_dev)))
         return fail(K2_E_ARG, "k2_polish_device: null pointer");
     if (n_steps < 0 || !(clamp > 0.0) || clamp > 0.25)
@@ -725,6 +725,10 @@ extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, d
         if (n_act == (int)n_cand) {
             int rc = k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
             if (rc) return rc;
+            // the first pass is at the starting points: their K, without a pass of its own
+            if (it == 0 && K0_dev)
+                HIPCHK(hipMemcpyAsync(K0_dev, K_dev, (size_t)n_cand * 8, hipMemcpyDeviceToDevice,
+                                      st));
         } else {
             HIPCHK(hipMemcpyAsync(idx_d, idx.data(), (size_t)n_act * 4, hipMemcpyHostToDevice, st));
             HIPCHK(launch_gather((const uint4*)V_dev, idx_d, n_act, Vc, st));
@@ -743,7 +747,10 @@ extern "C" int k2_polish_device(k2_table* t, uint32_t n_cand, uint64_t* V_dev, d
     }
     evals += n_cand;
     t->polish_evals = evals;
-    return k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
+    int rc = k2_eval_device(t, n_cand, V_dev, k, n_zeros, fl, K_dev, dK_dev, d2K_dev, st);
+    if (!rc && n_steps == 0 && K0_dev)   // no Newton pass: the final pass is at the start
+        HIPCHK(hipMemcpyAsync(K0_dev, K_dev, (size_t)n_cand * 8, hipMemcpyDeviceToDevice, st));
+    return rc;
 }
 
 extern "C" int k2_polish_stats(k2_table* t, int64_t* cand_evals) {
@@ -1043,6 +1050,8 @@ extern "C" void compute_peak_AB(int n_peaks, const uint64_t* x_lo_arr, const uin
                 e = d2h_staged(c, (char*)(out_B + p0 * nz), (const char*)dB, r * nz * 8, st);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
+        // a failed call leaves no copy or kernel behind that could still use the buffers
+        if (e != hipSuccess) (void)hipStreamSynchronize(st);
     }
     if (e != hipSuccess) {
         fail(K2_E_HIP, std::string("compute_peak_AB: ") + hipGetErrorString(e));

@@ -96,7 +96,7 @@ def _declare(L):
                                 ctypes.POINTER(ctypes.c_uint32)]
     L.k2_polish_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                    ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp, _vp,
-                                   _vp, _vp]
+                                   _vp, _vp, _vp]
     L.k2_phase_words.argtypes = [_vp, ctypes.c_uint32, _u64p, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_int, _u64p]
     L.k2_sincos_words.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_int, _f64p]
@@ -251,18 +251,21 @@ class K2Table:
             st), "k2_eval_batched_device")
         return K, dK, d2K
 
-    def polish_device(self, V, k, n_steps=8, clamp=1e-4, n_zeros=None, flags=0, stream=None):
-        """Newton polish in place on V (tensor).  Returns (K, dK, d2K, steps) at the final point."""
+    def polish_device(self, V, k, n_steps=8, clamp=1e-4, n_zeros=None, flags=0, stream=None,
+                      start=False):
+        """Newton polish in place on V (tensor).  Returns (K, dK, d2K, steps) at the final point,
+        and with start=True also K at the starting points (from the first pass)."""
         import torch
         n = V.shape[0]
         K, dK, d2K = (torch.empty(n, dtype=torch.float64, device=V.device) for _ in range(3))
+        K0 = torch.empty(n, dtype=torch.float64, device=V.device) if start else None
         steps = torch.zeros(n, dtype=torch.int32, device=V.device)
         st = stream if stream is not None else torch.cuda.current_stream(V.device).cuda_stream
         _check(load_library().k2_polish_device(
             self._h, n, V.data_ptr(), float(k), int(n_zeros or 0), int(n_steps), float(clamp),
-            int(flags), K.data_ptr(), dK.data_ptr(), d2K.data_ptr(), steps.data_ptr(), st),
-            "k2_polish_device")
-        return K, dK, d2K, steps
+            int(flags), K.data_ptr(), dK.data_ptr(), d2K.data_ptr(), steps.data_ptr(),
+            K0.data_ptr() if start else None, st), "k2_polish_device")
+        return (K, dK, d2K, steps, K0) if start else (K, dK, d2K, steps)
 
     def polish_stats(self):
         """Candidate evaluations of the last polish_device call (converged candidates drop out)."""

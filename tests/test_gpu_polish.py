@@ -29,6 +29,26 @@ def test_polish_matches_oracle():
     assert np.array_equal(steps.cpu().numpy(), g["steps"])
 
 
+@pytest.mark.parametrize("flags, tol", [(0, 1e-13), (ev.MIXED, 1e-9)])
+def test_polish_start_values_from_first_pass(flags, tol):
+    """K0 of k2_polish_device (K at the starting points, taken from its first pass) against a
+    separate evaluation at those points in the same form; with no Newton step the final pass is
+    at the start and K0 equals K."""
+    g = golden("polish_small.npz")
+    tb = synth.make_zero_table(3000, 11)
+    t = ev.K2Table(tb)
+    k = float(g["k"])
+    V0 = g["V0"]
+    Kref, _, _ = t.eval(V0, k, flags=ev.VALUE | flags)
+    K, dK, d2K, steps, K0 = t.polish_device(ev.v_tensor(V0), k, n_steps=8, clamp=1e-4,
+                                            flags=flags, start=True)
+    np.testing.assert_allclose(K0.cpu().numpy(), Kref, rtol=0, atol=tol)
+    K, dK, d2K, steps, K0 = t.polish_device(ev.v_tensor(V0), k, n_steps=0, flags=flags,
+                                            start=True)
+    assert np.array_equal(K0.cpu().numpy(), K.cpu().numpy())
+    np.testing.assert_allclose(K.cpu().numpy(), Kref, rtol=0, atol=tol)
+
+
 def test_polish_compaction_counts_and_matches_full_passes():
     """After each step only the candidates still moving are evaluated again; the result must not
     depend on that (a per-candidate sum does not depend on which other candidates share a pass
