@@ -236,11 +236,33 @@ __global__ __launch_bounds__(256) void k2_reduce_kernel(
     const int c = blockIdx.x * CB + cl;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     if (c < n_cand) {
-        for (int s = sg; s < n_splits; s += SG) {
-            const double* row = partial + (size_t)s * 3 * n_cand_pad;
-            s0 += row[c];
-            if (flags & K2_D1) s1 += row[n_cand_pad + c];
-            if (flags & K2_D2) s2 += row[2 * n_cand_pad + c];
+        // four rows' loads in flight per step, added in row order (the same sums as one row at
+        // a time): the loop is bound by load latency, not bandwidth
+        const bool d1 = (flags & K2_D1) != 0, d2 = (flags & K2_D2) != 0;
+        const size_t rs = (size_t)3 * n_cand_pad;
+        const double* p0 = partial + c;
+        int s = sg;
+        for (; s + 3 * SG < n_splits; s += 4 * SG) {
+            double a[4], b[4], e[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double* row = p0 + (size_t)(s + u * SG) * rs;
+                a[u] = row[0];
+                b[u] = d1 ? row[n_cand_pad] : 0.0;
+                e[u] = d2 ? row[2 * n_cand_pad] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                s0 += a[u];
+                s1 += b[u];
+                s2 += e[u];
+            }
+        }
+        for (; s < n_splits; s += SG) {
+            const double* row = p0 + (size_t)s * rs;
+            s0 += row[0];
+            if (d1) s1 += row[n_cand_pad];
+            if (d2) s2 += row[2 * n_cand_pad];
         }
     }
     if constexpr (SG > 1) {
