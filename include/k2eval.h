@@ -214,11 +214,13 @@ int k2_screen_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double
  * out[c*nz + j]).  mode 0 = evaluator path, 1 = schoolbook product, 2 = the exact-column phase
  * window: bits [128,174) of the product's columns 3..5, i.e. floor(W / 2^18) or up to 4 below
  * it (mod 2^46); 3 = the phase the FP32 kernels use (columns 4..5 exact, column 3 summed in
- * FP32 and rounded to a unit here), in the same units, within 2^12.6 of floor(W / 2^18). */
+ * FP32 and rounded to a unit here), in the same units, within 2^12.6 of floor(W / 2^18); 4 = the
+ * one-launch FP64 kernel's remainder words reassembled into W with bit 0 (not used) clear. */
 int k2_phase_words(k2_table *t, uint32_t n_cand, const uint64_t *V, uint32_t z0, uint32_t nz,
                    int mode, uint64_t *out);
 
-/* Debug: the evaluator's sin/cos of 2 pi W / 2^64 (mode 0, FP64) or the FP32 form's of the
+/* Debug: the evaluator's sin/cos of 2 pi W / 2^64 (mode 0, FP64 with the 1,024-point table;
+ * mode 2, FP64 with the 2,048-point table of the one-launch kernel) or the FP32 form's of the
  * window floor(W / 2^18) (mode 1, widened); out[2i] = sin, out[2i+1] = cos.  Host buffers. */
 int k2_sincos_words(uint32_t n, const uint64_t *W, int mode, double *out);
 

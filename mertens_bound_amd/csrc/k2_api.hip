@@ -32,6 +32,16 @@ void k2impl::sincos_tables(k2dev::Tab64* t64, k2dev::Tab32* t32) {
         const long double a = two_pi * ((long double)k + 0.5L) / (long double)K2_TAB - shift;
         t64[k] = k2dev::Tab64{(double)cosl(a), (double)sinl(a)};
     }
+    // the split 2,048-point table after them: K2_TABW cosines, a zero pad, K2_TABW sines
+    double* tw = reinterpret_cast<double*>(t64 + K2_TAB);
+    tw[K2_TABW] = 0.0;
+    const long double shiftw =
+        6755399441055744.0L * (long double)K2_REMW_K - (long double)K2_REMW_M;
+    for (int k = 0; k < K2_TABW; ++k) {
+        const long double a = two_pi * ((long double)k + 0.5L) / (long double)K2_TABW - shiftw;
+        tw[k] = (double)cosl(a);
+        tw[K2_TABW_SIN + k] = (double)sinl(a);
+    }
     for (int k = 0; k < K2_TAB32; ++k) {
         const long double a = two_pi * (long double)k / (long double)K2_TAB32;
         t32[k] = k2dev::Tab32{(float)cosl(a), (float)sinl(a)};
@@ -39,10 +49,11 @@ void k2impl::sincos_tables(k2dev::Tab64* t64, k2dev::Tab32* t32) {
 }
 
 int k2impl::upload_tables(k2dev::Tab64** t64, k2dev::Tab32** t32, hipStream_t st) {
-    std::vector<k2dev::Tab64> h64(K2_TAB);
+    std::vector<k2dev::Tab64> h64(K2_TAB64_ENTRIES);
     std::vector<k2dev::Tab32> h32(K2_TAB32);
     sincos_tables(h64.data(), h32.data());
-    const size_t b64 = K2_TAB * sizeof(k2dev::Tab64), b32 = K2_TAB32 * sizeof(k2dev::Tab32);
+    const size_t b64 = K2_TAB64_ENTRIES * sizeof(k2dev::Tab64);
+    const size_t b32 = K2_TAB32 * sizeof(k2dev::Tab32);
     hipError_t e = hipMalloc((void**)t64, b64);
     if (e == hipSuccess) e = hipMalloc((void**)t32, b32);
     if (e == hipSuccess) e = hipMemcpyAsync(*t64, h64.data(), b64, hipMemcpyHostToDevice, st);
@@ -779,7 +790,7 @@ extern "C" int k2_phase_words(k2_table* t, uint32_t n_cand, const uint64_t* V, u
                               uint32_t nz, int mode, uint64_t* out) {
     if (!t || (n_cand && nz && (!V || !out))) return fail(K2_E_ARG, "k2_phase_words: null pointer");
     if ((uint64_t)z0 + nz > t->n) return fail(K2_E_ARG, "k2_phase_words: zero range out of table");
-    if (mode < 0 || mode > 3) return fail(K2_E_ARG, "k2_phase_words: mode must be 0..3");
+    if (mode < 0 || mode > 4) return fail(K2_E_ARG, "k2_phase_words: mode must be 0..4");
     if ((uint64_t)n_cand * nz > ((uint64_t)1 << 31))
         return fail(K2_E_ARG, "k2_phase_words: n_cand * nz too large");
     if (n_cand == 0 || nz == 0) return K2_OK;
@@ -804,7 +815,7 @@ extern "C" int k2_sincos_words(uint32_t n, const uint64_t* W, int mode, double* 
     if (n && (!W || !out)) return fail(K2_E_ARG, "k2_sincos_words: null pointer");
     if (n == 0) return K2_OK;
     if (n > (1u << 28)) return fail(K2_E_ARG, "k2_sincos_words: n too large");
-    if (mode < 0 || mode > 1) return fail(K2_E_ARG, "k2_sincos_words: mode must be 0 or 1");
+    if (mode < 0 || mode > 2) return fail(K2_E_ARG, "k2_sincos_words: mode must be 0, 1 or 2");
     uint64_t* dW = nullptr;
     double* dO = nullptr;
     k2dev::Tab64* t64 = nullptr;

@@ -131,8 +131,9 @@ __device__ __forceinline__ PhaseRem phase_rem_of(uint64_t W) {
 // the limbs' low halves packed in pairs, one v_dot2_u32_u16 (each 16x16-bit product exact mod
 // 2^16) in place of two 24-bit multiply-adds.  pack_lo16(x, y) = (x mod 2^16) | (y << 16); a zero
 // packs (g2, g3), a candidate (v3, v2).  The FP32 window uses it (3% off the FP32 tail's time,
-// the packed zero word staged in the tile); in the FP64 loops, which would pack each zero in the
-// loop, it measured no faster than the two multiply-adds.
+// the packed zero word staged in the tile), and so does the one-launch FP64 C tile
+// (phase_columns_dp, the packed word staged with each zero); packing each zero inside an FP64
+// loop instead measured no faster than the two multiply-adds.
 __device__ __forceinline__ uint32_t pack_lo16(uint32_t x, uint32_t y) {
     return __builtin_amdgcn_perm(y, x, 0x05040100u);
 }
@@ -225,6 +226,55 @@ __device__ __forceinline__ PhaseRem phase_rem(Limbs4 g, Limbs4 v) {
         r = phase_rem_of(W);
     }
     return r;
+}
+
+// The columns of phase_columns with column 5 as one packed dot product: gp packs the zero's
+// (g2, g3) and vp the candidate's (v3, v2) low halves (pack_lo16), and dot2_lo16 gives
+// g2 v3 + g3 v2 mod 2^16 added to the column-4 carry word (bits [0,14) are needed).  The C tile
+// stages gp with each zero, so the pair costs one instruction per evaluation instead of two.
+__device__ __forceinline__ void phase_columns_dp(Limbs4 g, uint32_t gp, Limbs4 v, uint32_t vp,
+                                                 uint32_t& s3lo, uint32_t& s4lo, uint32_t& s5,
+                                                 uint32_t& tlo) {
+    uint32_t k2;
+    uint64_t t = mul32(g.x0, v.x2);
+    t = mad_carry0(g.x1, v.x1, t, k2);
+    t = mad_carry(g.x2, v.x0, t, k2);
+    tlo = (uint32_t)t;
+    uint64_t s3 = (t >> 32) | ((uint64_t)k2 << 32);
+    uint32_t k3;
+    s3 = mad_carry0(g.x0, v.x3, s3, k3);
+    s3 = mad_carry(g.x1, v.x2, s3, k3);
+    s3 = mad_carry(g.x2, v.x1, s3, k3);
+    s3 = mad_carry(g.x3, v.x0, s3, k3);
+    uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
+    s4 = mad64(g.x1, v.x3, s4);
+    s4 = mad64(g.x2, v.x2, s4);
+    s4 = mad64(g.x3, v.x1, s4);
+    s5 = dot2_lo16(gp, vp, (uint32_t)(s4 >> 32));
+    s3lo = (uint32_t)s3;
+    s4lo = (uint32_t)s4;
+}
+
+// the split table's remainder words straight from the columns: W bits [1,33) are product bits
+// [111,143), W bits [33,53) bits [143,163), and 8 (W bits [53,64)) = s5 & 0x3FF8
+__device__ __forceinline__ PhaseRem phase_rem_w_cols(uint32_t s3lo, uint32_t s4lo, uint32_t s5) {
+    return PhaseRem{__builtin_amdgcn_alignbit(s4lo, s3lo, 15),
+                    __builtin_amdgcn_alignbit(s5, s4lo, 15), s5 & 0x3FF8u};
+}
+
+// The carry case for the split table's words, which drop W bit 0: a carry from columns 0-1
+// (at most one unit at bit 98) changes W bits [1,64) only if product bits [98,111) are all ones
+// (probability 2^-13, half the rate of phase_carry_case).
+__device__ __forceinline__ bool phase_carry_case_w(uint32_t s3lo) {
+    return (s3lo & 0x7FFCu) == 0x7FFCu;
+}
+
+// the carry case's exact W from the columns (phase_carry_case holds)
+__device__ __forceinline__ uint64_t phase_word_fixed(Limbs4 g, Limbs4 v, uint32_t s3lo,
+                                                     uint32_t s4lo, uint32_t s5, uint32_t tlo) {
+    return ((uint64_t)__builtin_amdgcn_alignbit(s5, s4lo, 14) << 32 |
+            __builtin_amdgcn_alignbit(s4lo, s3lo, 14)) +
+           phase_low_carry(g, v, s3lo, tlo);
 }
 
 // ---- FP32 form: the phase window ------------------------------------------------------------
@@ -382,6 +432,63 @@ __device__ __forceinline__ void sincos_tab64_at(const PhaseRem& r, const Tab64* 
 
 __device__ __forceinline__ void sincos_tab64(uint64_t W, const Tab64* tab, double& S, double& C) {
     sincos_tab64_at(phase_rem_of(W), tab, S, C, SinCosCoef K2_SINCOS_COEF);
+}
+
+// ---- FP64 sin/cos with a 2,048-point split table (the one-launch C tile) ----------------------
+// The same construction with twice the intervals: k = W bits [53,64), r = W bits [1,53) (W bit 0
+// dropped: < 3.4e-19 rad), x = fma(2^52 + r, KW, -MW) with KW = 2 pi / 2^63, |x| <= pi/2048 =
+// 1.53e-3.  That halves x and drops the x^5 term: sin x = x - x^3/6 (truncation x^5/120 <= 7.1e-17
+// = 0.64 u, odd in x; x is centred in its interval, so it averages out over the zeros) and
+// cos x = 1 - x^2/2 + x^4/24 (< 1.8e-20) -- five FMAs where the 1,024-point table takes six.
+// The table holds the K2_TABW cosines, one unused double, and the K2_TABW sines of the
+// midpoints (8-byte entries, so the byte offset 8k is W bits [53,64) masked in place; the two
+// halves are read at that offset and 16 KiB + 8 above it -- the extra 8 bytes put a lane's sine
+// in the two LDS banks after its cosine's, as in a 16-byte entry, instead of the same two).
+// 32 KiB + 8 in LDS (K2_TABW_DOUBLES).  It sits in the FP64 table buffer after the K2_TAB
+// interleaved entries (sincos_tables).
+#define K2_TABW_BITS 11
+#define K2_TABW (1 << K2_TABW_BITS)
+#define K2_TABW_SIN (K2_TABW + 1)             // index of the first sine
+#define K2_TABW_DOUBLES (2 * K2_TABW + 1)
+#define K2_REMW_K 6.8122431601731090e-19   // 2 pi / 2^63, rounded to double
+#define K2_REMW_M 0.0046019423636569235    // (2^52 + 2^51) K2_REMW_K, rounded to double
+
+struct SinCosCoefW {
+    double k, nm, s1, c2;   // KW, -MW, -1/6, 1/24
+};
+
+#define K2_SINCOSW_COEF {K2_REMW_K, -K2_REMW_M, -1.0 / 6.0, 1.0 / 24.0}
+
+__device__ __forceinline__ SinCosCoefW sincos_coef_w() {
+    SinCosCoefW k K2_SINCOSW_COEF;
+    asm volatile("" : "+v"(k.nm), "+v"(k.s1));
+    return k;
+}
+
+// the remainder words for the split table: rlo = W bits [1,33), rhi bits [0,20) = W bits
+// [33,53) (higher bits unused), off = 8 * (W bits [53,64))
+__device__ __forceinline__ PhaseRem phase_rem_w_of(uint64_t W) {
+    return PhaseRem{(uint32_t)(W >> 1), (uint32_t)(W >> 33), (uint32_t)(W >> 50) & 0x3FF8u};
+}
+
+// tab: the split table (K2_TABW cosines, a pad, K2_TABW sines)
+__device__ __forceinline__ void sincos_tabw_at(const PhaseRem& r, const double* tab, double& S,
+                                               double& C, const SinCosCoefW& kc) {
+    uint32_t hi;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(hi) : "v"(r.rhi), "s"(0xFFFFFu), "v"(0x43300000u));
+    const double x = fma(__hiloint2double((int)hi, (int)r.rlo), kc.k, kc.nm);
+    const double z = fma(x, x, 0.0);
+    const double sp = fma(fma(x, z, 0.0), kc.s1, x);
+    const double cp = fma(z, fma(z, kc.c2, -0.5), 1.0);
+    const char* tb = reinterpret_cast<const char*>(tab) + r.off;
+    const double tc = *reinterpret_cast<const double*>(tb);
+    const double ts = *reinterpret_cast<const double*>(tb + K2_TABW_SIN * sizeof(double));
+    C = fma(tc, cp, fma(-ts, sp, 0.0));
+    S = fma(ts, cp, fma(tc, sp, 0.0));
+}
+
+__device__ __forceinline__ void sincos_tabw(uint64_t W, const double* tab, double& S, double& C) {
+    sincos_tabw_at(phase_rem_w_of(W), tab, S, C, SinCosCoefW K2_SINCOSW_COEF);
 }
 
 // FP32 sin/cos of the window's phase with a 2^TB-point table (every 2^(11-TB)-th entry of the

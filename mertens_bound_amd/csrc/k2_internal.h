@@ -59,8 +59,11 @@ struct LaunchArgs {
 };
 
 // host-side sin/cos tables, rounded from long double: K2_TAB FP64 entries at the interval
-// midpoints 2 pi (k + 1/2) / K2_TAB (see sincos_tab64_at) and K2_TAB32 FP32 entries of
+// midpoints 2 pi (k + 1/2) / K2_TAB (see sincos_tab64_at) followed by the split K2_TABW-point
+// table (sincos_tabw_at) in t64, which holds K2_TAB64_ENTRIES, and K2_TAB32 FP32 entries of
 // 2 pi k / K2_TAB32
+// Tab64-sized entries of the FP64 table buffer: K2_TAB interleaved, then K2_TABW_DOUBLES
+#define K2_TAB64_ENTRIES (K2_TAB + (K2_TABW_DOUBLES + 1) / 2)
 void sincos_tables(k2dev::Tab64* t64, k2dev::Tab32* t32);
 // device copies of those tables
 int upload_tables(k2dev::Tab64** t64, k2dev::Tab32** t32, hipStream_t st);

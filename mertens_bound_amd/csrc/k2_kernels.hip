@@ -78,9 +78,11 @@ __global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : (F32 && NC == 
     ZeroView zv, int z_begin, int z_end, int zps, const uint4* __restrict__ cand, int n_cand,
     int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
     const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
-    __shared__ typename std::conditional<F32, CTileSmem32<FL>, CTileSmem64<FL>>::type sm;
-    constexpr int NTAB = F32 ? K2_TAB32 : K2_TAB;
-    __shared__ typename std::conditional<F32, Tab32, Tab64>::type s_tab[NTAB];
+    // FP32: the 2,048-point FP32 table; FP64: the split 2,048-point table after the K2_TAB
+    // interleaved entries of the FP64 table buffer
+    __shared__ typename std::conditional<F32, CTileSmem32<FL>, CTileSmemW<FL, ctw_tz<FL, NT>()>>::type sm;
+    constexpr int NTAB = F32 ? K2_TAB32 : K2_TABW_DOUBLES;
+    __shared__ typename std::conditional<F32, Tab32, double>::type s_tab[NTAB];
     const int L = xcd_linear_block();
     const int split = L / n_ctiles, ct = L - split * n_ctiles;
     const int z0 = z_begin + split * zps;
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : (F32 && NC == 
     const int tid = threadIdx.x;
     for (int i = tid; i < NTAB; i += NT) {
         if constexpr (F32) s_tab[i] = tab32[i];
-        else s_tab[i] = tab64[i];
+        else s_tab[i] = reinterpret_cast<const double*>(tab64 + K2_TAB)[i];
     }
     Limbs4 v[NC];
     Acc<F32> acc[NC];
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : (F32 && NC == 
         acc[i].zero();
     }
     if constexpr (F32) ctile_f32<NC, FL, NT>(zv, z0, z1, v, acc, sm, s_tab);
-    else ctile_f64<NC, FL, Acc<false>, NT>(zv, z0, z1, v, acc, sm, s_tab);
+    else ctile_f64w<NC, FL, NT>(zv, z0, z1, v, acc, sm, s_tab);
     double* prow = partial + (size_t)(row_base + split) * 3 * n_cand_pad;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
@@ -295,6 +297,16 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
     uint64_t W;
     if (mode == 0) W = phase_word(g, v);
     else if (mode == 1) W = phase_word_exact(g, v);
+    else if (mode == 4) {
+        // the one-launch C tile's remainder words (W bits [1,64)) reassembled into W, bit 0 clear
+        uint32_t s3lo, s4lo, s5, tlo;
+        phase_columns_dp(g, pack_lo16(g.x2, g.x3), v, pack_lo16(v.x3, v.x2), s3lo, s4lo, s5, tlo);
+        PhaseRem r = phase_rem_w_cols(s3lo, s4lo, s5);
+        if (phase_carry_case_w(s3lo))
+            r = phase_rem_w_of(phase_word_fixed(g, v, s3lo, s4lo, s5, tlo));
+        W = ((uint64_t)(r.off >> 3) << 53) | ((uint64_t)(r.rhi & 0xFFFFFu) << 33) |
+            ((uint64_t)r.rlo << 1);
+    }
     else if (mode == 2) {
         uint32_t hi, lo;
         phase_window<K2_TAB32_BITS>(g, v, hi, lo);
@@ -312,8 +324,9 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
 }
 
 // ---- debug: the evaluators' sin/cos of phase words ---------------------------------------------
-// mode 0: FP64 table path of the plain kernel, 1: FP32 table path of the mixed kernel (the
-// window of W: bits [18,64) plus half a table step)
+// mode 0: FP64 1,024-point table path (enclosure, batched, Z-tile and weight-vector kernels),
+// 1: FP32 table path of the mixed kernel (the window of W: bits [18,64) plus half a table step),
+// 2: FP64 2,048-point split table path of the one-launch C tile
 __global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __restrict__ W,
                                                               int n, int mode,
                                                               const Tab64* __restrict__ tab64,
@@ -324,6 +337,10 @@ __global__ __launch_bounds__(256) void k2_sincos_words_kernel(const uint64_t* __
     if (mode == 0) {
         double s, c;
         sincos_tab64(W[i], tab64, s, c);
+        out[2 * i] = s; out[2 * i + 1] = c;
+    } else if (mode == 2) {
+        double s, c;
+        sincos_tabw(W[i], reinterpret_cast<const double*>(tab64 + K2_TAB), s, c);
         out[2 * i] = s; out[2 * i + 1] = c;
     } else {
         float s, c;

@@ -43,6 +43,31 @@ def test_phase_fixup_cases_hit_and_exact(small, oracle_lib):
     assert hits > 0
 
 
+def test_split_table_words_bit_exact(small, oracle_lib):
+    """The one-launch kernel's remainder words (packed column 5, carry case -- product bits
+    [98,111) all ones, rate 2^-13 -- fixed up from the columns) give W bits [1,64) exactly, on
+    the pairs of the fix-up test (about 125 carry cases) and on edge words."""
+    tb, t = small
+    V = synth.make_candidates(512, 3.4e18, 3.6e18, 2)
+    Wo = oracle_lib.phase_words(tb.G, V)
+    assert np.array_equal(t.phase_words(V, mode=4), Wo & np.uint64(M64 - 1))
+    hits = 0
+    for v in synth.v_to_int(V[:128]):
+        vl = [(v >> (32 * i)) & 0xFFFFFFFF for i in range(4)]
+        for g in synth.v_to_int(tb.G):
+            gl = [(g >> (32 * i)) & 0xFFFFFFFF for i in range(4)]
+            U = sum(gl[i] * vl[j2] << (32 * (i + j2)) for i in range(4) for j2 in range(4)
+                    if i + j2 >= 2)
+            hits += ((U >> 98) & 0x1FFF) == 0x1FFF
+    assert hits > 0   # the fix-up path ran (about 30 of these 256,000 pairs)
+    edge = [0, 1, (1 << 128) - 1, 1 << 110, (1 << 127) + 12345, M64, 1 << 64, (1 << 98) - 1]
+    G = synth.v_from_int(edge)
+    te = ev.K2Table(synth.ZeroTable(G=G, gamma=np.ones(len(edge)), w=np.zeros(len(edge), complex)))
+    Ve = synth.v_from_int(edge + [(3 << 64) + (1 << 62)])
+    assert np.array_equal(te.phase_words(Ve, mode=4),
+                          oracle_lib.phase_words(G, Ve) & np.uint64(M64 - 1))
+
+
 def test_phase_words_edges(oracle_lib):
     edge = [0, 1, (1 << 128) - 1, 1 << 110, (1 << 127) + 12345, M64, 1 << 64, (1 << 98) - 1]
     G = synth.v_from_int(edge)
@@ -94,6 +119,35 @@ def test_sincos_of_words():
     assert np.max(np.abs(s32 - np.sin(th))) < 3e-7 and np.max(np.abs(c32 - np.cos(th))) < 3e-7
 
 
+def test_sincos_split_table_of_words():
+    """The one-launch kernel's sin/cos (2,048-point split table, sin x = x - x^3/6): against long
+    double over random words and both edges of every interval (the largest remainders, where the
+    dropped x^5/120 is 0.64 u), and against mpmath on a sample."""
+    U = 2.0 ** -53
+    rng = np.random.default_rng(5)
+    W = rng.integers(0, 2**64, size=1 << 20, dtype=np.uint64)
+    k = np.arange(2048, dtype=np.uint64)
+    step = np.uint64(1 << 53)
+    W[:2048] = k << np.uint64(53)                               # largest negative remainder
+    W[2048:4096] = (k << np.uint64(53)) + step - np.uint64(1)   # largest positive remainder
+    W[4096:4104] = np.array([0, 1, 1 << 62, 1 << 63, 3 << 62, M64, 1 << 53, (1 << 53) - 1],
+                            dtype=np.uint64)
+    s, c = ev.sincos_words(W, mode=2)
+    two_pi = np.longdouble("6.283185307179586476925286766559005768")
+    th = two_pi * (W.astype(np.longdouble) / np.longdouble(2.0) ** 64)
+    es = np.abs(s.astype(np.longdouble) - np.sin(th)) / U
+    ec = np.abs(c.astype(np.longdouble) - np.cos(th)) / U
+    print(f"sincos_tabw max error: sin {float(es.max()):.3f} u, cos {float(ec.max()):.3f} u")
+    assert float(es.max()) <= 3.0 and float(ec.max()) <= 3.0
+    # a wrong interval, half-step or sign misses by 1e-3 or more
+    import mpmath
+    with mpmath.workdps(40):
+        for i in list(range(0, 4104, 257)) + list(range(4104, 1 << 20, 9973)):
+            t = 2 * mpmath.pi * mpmath.mpf(int(W[i])) / mpmath.mpf(2) ** 64
+            assert abs(s[i] - float(mpmath.sin(t))) <= 3.5e-16
+            assert abs(c[i] - float(mpmath.cos(t))) <= 3.5e-16
+
+
 def _check_window_f(win, Wo):
     """The FP32 kernels' window sums column 3 in FP32: four products of floats rounded to 24
     bits, three FMAs, then truncation to units of 2^130.  Worst case |S3 - t 2^34| <= 6 2^-24 S3
@@ -119,7 +173,8 @@ def test_phase_window_f_within_bound(small, oracle_lib):
 
 def test_phase_words_full_table(oracle_lib):
     """All 2,325,572 zeros of the full-size table against candidates spread over
-    [3.4e18, 3.6e18]: every 64-bit word bit-exact, every FP32 window within 4 units."""
+    [3.4e18, 3.6e18]: every 64-bit word bit-exact (the one-launch kernel's words in W bits
+    [1,64)), every FP32 window within 4 units."""
     c = synth.W65536x2325572
     tb = synth.make_zero_table(c["n_zeros"], c["seed"])
     t = ev.K2Table(tb)
@@ -128,5 +183,6 @@ def test_phase_words_full_table(oracle_lib):
                         synth.make_candidates(1, c["v_lo"], c["v_hi"], 77)])
     Wo = oracle_lib.phase_words(tb.G, V)
     assert np.array_equal(t.phase_words(V, mode=0), Wo)
+    assert np.array_equal(t.phase_words(V, mode=4), Wo & np.uint64(M64 - 1))
     _check_window(t.phase_words(V, mode=2), Wo)
     _check_window_f(t.phase_words(V, mode=3), Wo)
