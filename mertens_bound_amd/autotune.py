@@ -29,11 +29,12 @@ import time
 PKG = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(PKG, "tuned_tiles.json")
 
-EVAL_GRID = {"nc": [1, 2, 4], "target_blocks": [1536, 3072, 6144, 12288],
+EVAL_GRID = {"nc": [1, 2, 4], "target_blocks": [3072, 6144, 12288, 24576, 49152, 98304],
              "wg_threads": [256, 512]}
 BATCH_GRID = {"nc": [2, 4], "cz": [8, 16], "chunk": [4096, 8192, 16384], "wg_per_cu": [0],
               "c_min_fill": [128, 256, 384]}
-POLISH_GRID = {"nc": [2, 4], "target_blocks": [3072, 6144, 12288], "wg_threads": [256, 512]}
+POLISH_GRID = {"nc": [2, 4], "target_blocks": [6144, 12288, 24576, 49152],
+               "wg_threads": [256, 512]}
 
 _cache = None
 

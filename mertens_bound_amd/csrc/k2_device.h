@@ -232,27 +232,37 @@ __device__ __forceinline__ PhaseRem phase_rem(Limbs4 g, Limbs4 v) {
 // (g2, g3) and vp the candidate's (v3, v2) low halves (pack_lo16), and dot2_lo16 gives
 // g2 v3 + g3 v2 mod 2^16 added to the column-4 carry word (bits [0,14) are needed).  The C tile
 // stages gp with each zero, so the pair costs one instruction per evaluation instead of two.
+// SG: the zero's limbs (and gp) are uniform over the wave and sit in scalar registers; they
+// take the multiply-adds' scalar operand slot.
+template <bool SG = false>
 __device__ __forceinline__ void phase_columns_dp(Limbs4 g, uint32_t gp, Limbs4 v, uint32_t vp,
                                                  uint32_t& s3lo, uint32_t& s4lo, uint32_t& s5,
                                                  uint32_t& tlo) {
     uint32_t k2;
     uint64_t t = mul32(g.x0, v.x2);
-    t = mad_carry0(g.x1, v.x1, t, k2);
-    t = mad_carry(g.x2, v.x0, t, k2);
+    t = mad_carry0<SG>(v.x1, g.x1, t, k2);
+    t = mad_carry<SG>(v.x0, g.x2, t, k2);
     tlo = (uint32_t)t;
     uint64_t s3 = (t >> 32) | ((uint64_t)k2 << 32);
     uint32_t k3;
-    s3 = mad_carry0(g.x0, v.x3, s3, k3);
-    s3 = mad_carry(g.x1, v.x2, s3, k3);
-    s3 = mad_carry(g.x2, v.x1, s3, k3);
-    s3 = mad_carry(g.x3, v.x0, s3, k3);
+    s3 = mad_carry0<SG>(v.x3, g.x0, s3, k3);
+    s3 = mad_carry<SG>(v.x2, g.x1, s3, k3);
+    s3 = mad_carry<SG>(v.x1, g.x2, s3, k3);
+    s3 = mad_carry<SG>(v.x0, g.x3, s3, k3);
     uint64_t s4 = (s3 >> 32) | ((uint64_t)k3 << 32);
-    s4 = mad64(g.x1, v.x3, s4);
-    s4 = mad64(g.x2, v.x2, s4);
-    s4 = mad64(g.x3, v.x1, s4);
+    s4 = mad64<SG>(v.x3, g.x1, s4);
+    s4 = mad64<SG>(v.x2, g.x2, s4);
+    s4 = mad64<SG>(v.x1, g.x3, s4);
     s5 = dot2_lo16(gp, vp, (uint32_t)(s4 >> 32));
     s3lo = (uint32_t)s3;
     s4lo = (uint32_t)s4;
+}
+
+// pack_lo16(x, y) on the scalar unit, for values uniform over the wave
+__device__ __forceinline__ uint32_t pack_lo16_s(uint32_t x, uint32_t y) {
+    uint32_t r;
+    asm("s_pack_ll_b32_b16 %0, %1, %2" : "=s"(r) : "s"(x), "s"(y));
+    return r;
 }
 
 // the split table's remainder words straight from the columns: W bits [1,33) are product bits
@@ -442,8 +452,10 @@ __device__ __forceinline__ void sincos_tab64(uint64_t W, const Tab64* tab, doubl
 // cos x = 1 - x^2/2 + x^4/24 (< 1.8e-20) -- five FMAs where the 1,024-point table takes six.
 // The table holds the K2_TABW cosines, one unused double, and the K2_TABW sines of the
 // midpoints (8-byte entries, so the byte offset 8k is W bits [53,64) masked in place; the two
-// halves are read at that offset and 16 KiB + 8 above it -- the extra 8 bytes put a lane's sine
-// in the two LDS banks after its cosine's, as in a 16-byte entry, instead of the same two).
+// halves are read at that offset and 16 KiB + 8 above it, by two ds_read_b64).  Without the
+// extra 8 bytes the compiler joins the two reads into one ds_read2st64_b64 whose two addresses
+// per lane fall in the same LDS banks (twice the bank-conflict cycles, measured slower); 16-byte
+// (cos, sin) entries read by one ds_read_b128 at the doubled offset measured 4% slower still.
 // 32 KiB + 8 in LDS (K2_TABW_DOUBLES).  It sits in the FP64 table buffer after the K2_TAB
 // interleaved entries (sincos_tables).
 #define K2_TABW_BITS 11

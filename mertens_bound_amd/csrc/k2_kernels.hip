@@ -68,19 +68,22 @@ __global__ __launch_bounds__(256) void k2_abs_blocks_kernel(const double2* __res
 // Split s covers zeros [z_begin + s*zps, ...) up to z_end and writes partial row row_base + s.
 // F32 selects the precision form (phase window, FP32 sin/cos and weights, compensated FP32
 // sums); the mixed form launches an FP64 head and an FP32 tail side by side.
-// NC <= 2 keeps to 128 VGPRs so that four workgroups share a CU; NT = 512 puts twice the waves
-// on one staged tile and one sin/cos table (LDS is what limits the workgroups per CU), walks the
-// zeros without unrolling and stays within 80 VGPRs (six waves per SIMD; 96 and five waves with
-// K2'', whose third sum would spill at 80)
+// FP64: the zero records come into scalar registers (ctile_f64s) and LDS holds only the
+// 32 KiB table, so VGPRs set the occupancy: about 100 at NC = 4 (four waves per SIMD).
+// FP32: NC <= 2 keeps to 128 VGPRs so that four workgroups share a CU; NT = 512 puts twice the
+// waves on one staged tile and one sin/cos table (LDS is what limits the workgroups per CU),
+// walks the zeros without unrolling and stays within 80 VGPRs (six waves per SIMD; 96 and five
+// waves with K2'', whose third sum would spill at 80)
 template <int NC, int FL, bool F32, int NT>
 __global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : (F32 && NC == 2 ? 8 : 6))
                                             : (NC <= 2 ? 4 : 2))) void k2_eval_kernel(
     ZeroView zv, int z_begin, int z_end, int zps, const uint4* __restrict__ cand, int n_cand,
     int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
     const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
-    // FP32: the 2,048-point FP32 table; FP64: the split 2,048-point table after the K2_TAB
-    // interleaved entries of the FP64 table buffer
-    __shared__ typename std::conditional<F32, CTileSmem32<FL>, CTileSmemW<FL, ctw_tz<FL, NT>()>>::type sm;
+    // FP32: the C tile's records and the 2,048-point FP32 table; FP64: only the split
+    // 2,048-point table (after the K2_TAB interleaved entries of the FP64 table buffer) -- the
+    // FP64 C tile reads its zero records into scalar registers
+    __shared__ typename std::conditional<F32, CTileSmem32<FL>, char>::type sm;
     constexpr int NTAB = F32 ? K2_TAB32 : K2_TABW_DOUBLES;
     __shared__ typename std::conditional<F32, Tab32, double>::type s_tab[NTAB];
     const int L = xcd_linear_block();
@@ -103,7 +106,11 @@ __global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : (F32 && NC == 
         acc[i].zero();
     }
     if constexpr (F32) ctile_f32<NC, FL, NT>(zv, z0, z1, v, acc, sm, s_tab);
-    else ctile_f64w<NC, FL, NT>(zv, z0, z1, v, acc, sm, s_tab);
+    else {
+        (void)sm;
+        __syncthreads();   // the table
+        ctile_f64s<NC, FL>(zv.g, zv.a, zv.b, zv.e, z0, z1, v, acc, s_tab);
+    }
     double* prow = partial + (size_t)(row_base + split) * 3 * n_cand_pad;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {

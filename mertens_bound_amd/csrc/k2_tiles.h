@@ -268,37 +268,21 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
 }
 
 // ---- C tile of the one-launch FP64 kernel ---------------------------------------------------
-// As ctile_f64, with three changes that take instructions out of the evaluation loop:
-//  * the 2,048-point split sin/cos table (sincos_tabw_at: one FMA fewer per evaluation), 32 KiB;
+// As ctile_f64, with four changes that take instructions and LDS traffic out of the loop:
+//  * the zero records come straight from global memory into scalar registers: the record index
+//    is the same in every lane, so the loads are scalar (s_load, through the scalar cache) and
+//    the VALU instructions take the record as their one scalar operand -- no LDS staging, no
+//    barriers, no per-zero address arithmetic, and LDS holds only the table;
+//  * the 2,048-point split sin/cos table (sincos_tabw_at: one FMA fewer per evaluation);
 //  * column 5 of the phase as one packed dot product: each zero's (g2, g3) low halves are packed
-//    when the zero is staged (gp), the candidate's once per call (phase_columns_dp);
+//    on the scalar unit (s_pack_ll_b32_b16), the candidate's once per call (phase_columns_dp);
 //  * one carry-case test for the lane's NC candidates: the largest of their masked bits
 //    [98,111) equals the mask exactly when one of them is all ones (NC - 1 full-rate maxima and
 //    one compare instead of NC compares), and only then does any lane take the fix-up branch
 //    (phase_carry_case_w: the words drop W bit 0, so bit 110 joins the test and halves the rate).
-// To keep four workgroups per CU with the larger table the zero records are staged TZ at a time
-// into one LDS buffer (two barriers per tile; a tile is hundreds of thousands of cycles of work
-// per wave, so the barriers cost nothing measurable); NT / TZ lanes share the staging of one
-// zero.  Every field sits in an array of 16-byte entries (gp in the first word of its entry), so
-// one address register and fixed offsets reach all of a zero's fields.
-template <int FL, int NT>
-constexpr int ctw_tz() {   // zeros per staged tile (4 lanes per zero)
-    return NT / 4;
-}
 
-template <int FL, int TZ>
-struct CTileSmemW {
-    static constexpr int NB = (FL & K2_D1) ? 1 : 0, NE = (FL & K2_D2) ? 1 : 0;
-    uint4 g[TZ];
-    double2 w[(1 + NB + NE) * TZ];   // a, then b, then e
-    uint4 gp[TZ];                    // .x = pack_lo16(g2, g3)
-    __device__ __forceinline__ double2& a(int z) { return w[z]; }
-    __device__ __forceinline__ double2& b(int z) { return w[TZ + z]; }
-    __device__ __forceinline__ double2& e(int z) { return w[(1 + NB) * TZ + z]; }
-};
-
-// One zero's terms for the lane's NC candidates (the body of ctile_f64w's loop).
-template <int NC, int FL>
+// One zero's terms for the lane's NC candidates.  SG: the record sits in scalar registers.
+template <int NC, int FL, bool SG = false>
 __device__ __forceinline__ void ctw_zero(const uint4 gg, uint32_t gp, const double2 aa,
                                          const double2 bb, const double2 ee,
                                          const Limbs4 (&v)[NC], const uint32_t (&vp)[NC],
@@ -307,7 +291,8 @@ __device__ __forceinline__ void ctw_zero(const uint4 gg, uint32_t gp, const doub
     const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
     uint32_t c3[NC], c4[NC], c5[NC], ct[NC];
 #pragma unroll
-    for (int i = 0; i < NC; ++i) phase_columns_dp(g, gp, v[i], vp[i], c3[i], c4[i], c5[i], ct[i]);
+    for (int i = 0; i < NC; ++i)
+        phase_columns_dp<SG>(g, gp, v[i], vp[i], c3[i], c4[i], c5[i], ct[i]);
     uint32_t m = c3[0] & 0x7FFCu;
 #pragma unroll
     for (int i = 1; i < NC; ++i) m = max(m, c3[i] & 0x7FFCu);
@@ -328,65 +313,40 @@ __device__ __forceinline__ void ctw_zero(const uint4 gg, uint32_t gp, const doub
     }
 }
 
-// Sums zeros [z0, z1) into acc for the NC candidates of this lane; tab: the split table in LDS
-// (staged by the caller before the call; the first barrier here covers it).  Leaves sm free.
-template <int NC, int FL, int NT, int TZ = ctw_tz<FL, NT>()>
-__device__ __forceinline__ void ctile_f64w(const ZeroView& zv, int z0, int z1,
+// Sums zeros [z0, z1) (uniform over the workgroup) into acc for the NC candidates of this lane;
+// tab: the split table in LDS, staged and made visible by the caller.  The next zero's record
+// is loaded while the current one is summed.
+template <int NC, int FL>
+__device__ __forceinline__ void ctile_f64s(const uint4* __restrict__ zg,
+                                           const double2* __restrict__ za,
+                                           const double2* __restrict__ zb,
+                                           const double2* __restrict__ ze, int z0, int z1,
                                            const Limbs4 (&v)[NC], Acc<false> (&acc)[NC],
-                                           CTileSmemW<FL, TZ>& sm, const double* tab) {
-    constexpr int P = NT / TZ;   // lanes per staged zero
-    static_assert(P == 1 || P == 2 || P == 4, "NT / TZ of 1, 2 or 4");
+                                           const double* tab) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
-    const int tid = threadIdx.x;
-    const int zl = tid % TZ, part = tid / TZ;
-    // which fields this lane stages: with P = 2 the first half g and a, the second b and e;
-    // with P = 4 one field each
-    const bool st_g = P == 1 || part == 0;
-    const bool st_a = P == 1 || (P == 2 ? part == 0 : part == 1);
-    const bool st_b = D1 && (P == 1 || (P == 2 ? part == 1 : part == 2));
-    const bool st_e = D2 && (P == 1 || (P == 2 ? part == 1 : part == 3));
-    const int ntiles = z1 > z0 ? (z1 - z0 + TZ - 1) / TZ : 0;
+    if (z1 <= z0) return;
     uint32_t vp[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) vp[i] = pack_lo16(v[i].x3, v[i].x2);
+    const SinCosCoefW kc = sincos_coef_w();
     struct Rec { uint4 g; double2 a, b, e; };
-    auto fetch = [&](int t) -> Rec {   // by value: see ctile_f64
-        const int j = z0 + t * TZ + zl;
+    auto load = [&](int j) -> Rec {
         Rec r;
-        r.g = make_uint4(0, 0, 0, 0);
-        r.a = r.b = r.e = make_double2(0.0, 0.0);
-        if (j < z1) {
-            if (st_g) r.g = zv.g[j];
-            if (st_a) r.a = zv.a[j];
-            if (st_b) r.b = zv.b[j];
-            if (st_e) r.e = zv.e[j];
-        }
+        r.g = zg[j];
+        r.a = za[j];
+        r.b = r.e = make_double2(0.0, 0.0);
+        if (D1) r.b = zb[j];
+        if (D2) r.e = ze[j];
         return r;
     };
-    Rec pf;
-    if (ntiles > 0) pf = fetch(0);
-    const SinCosCoefW kc = sincos_coef_w();
-    for (int t = 0; t < ntiles; ++t) {
-        if (t > 0) __syncthreads();   // every lane is done with the previous tile
-        if (st_g) {
-            sm.g[zl] = pf.g;
-            sm.gp[zl] = make_uint4(pack_lo16(pf.g.z, pf.g.w), 0u, 0u, 0u);
-        }
-        if (st_a) sm.a(zl) = pf.a;
-        if (st_b) sm.b(zl) = pf.b;
-        if (st_e) sm.e(zl) = pf.e;
-        __syncthreads();
-        if (t + 1 < ntiles) pf = fetch(t + 1);   // in flight while this tile is summed
-        const int zn = __builtin_amdgcn_readfirstlane(min(TZ, z1 - z0 - t * TZ));
-        for (int z = 0; z < zn; ++z) {
-            double2 bb = make_double2(0.0, 0.0), ee = bb;
-            if (D1) bb = sm.b(z);
-            if (D2) ee = sm.e(z);
-            ctw_zero<NC, FL>(sm.g[z], sm.gp[z].x, sm.a(z), bb, ee, v, vp, acc, tab, kc);
-        }
+    Rec cur = load(z0);
+    for (int j = z0; j < z1; ++j) {
+        const Rec nxt = load(min(j + 1, z1 - 1));   // in flight while zero j is summed
+        ctw_zero<NC, FL, true>(cur.g, pack_lo16_s(cur.g.z, cur.g.w), cur.a, cur.b, cur.e, v, vp,
+                               acc, tab, kc);
+        cur = nxt;
     }
-    __syncthreads();
 }
 
 #define K2_F32_GROUP 16   // zeros per compensated add in the FP32 C tile (divides K2_WG)
