@@ -214,7 +214,7 @@ int k2_screen_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, double
  * out[c*nz + j]).  mode 0 = evaluator path, 1 = schoolbook product, 2 = the exact-column phase
  * window: bits [128,174) of the product's columns 3..5, i.e. floor(W / 2^18) or up to 4 below
  * it (mod 2^46); 3 = the phase the FP32 kernels use (columns 4..5 exact, column 3 summed in
- * FP32 and rounded to a unit here), in the same units, within 2^12.6 of floor(W / 2^18); 4 = the
+ * FP32 and rounded to a unit here), in the same units, within 7.3e3 of floor(W / 2^18); 4 = the
  * one-launch FP64 kernel's remainder words reassembled into W with bit 0 (not used) clear. */
 int k2_phase_words(k2_table *t, uint32_t n_cand, const uint64_t *V, uint32_t z0, uint32_t nz,
                    int mode, uint64_t *out);

@@ -144,11 +144,16 @@ extern "C" int k2_table_create(k2_table** out, uint32_t n_zeros, const uint64_t*
         e = hipMemcpyAsync(t->gamma, gamma, (size_t)n_zeros * 8, hipMemcpyHostToDevice, t->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(t->w, w, (size_t)n_zeros * 16, hipMemcpyHostToDevice, t->stream);
+    // the limbs times 2 pi / 2^44 as floats (the FP32 window's column 3: with the candidate's
+    // limbs times 2^-34 the products are in radians), each rounded once from double
     std::vector<float4> gf(np, make_float4(0.f, 0.f, 0.f, 0.f));
+    const double gscale = 6.283185307179586 / 17592186044416.0;   // 2 pi / 2^44
     for (uint32_t j = 0; j < n_zeros; ++j) {
         const uint64_t lo = G[2 * j], hi = G[2 * j + 1];
-        gf[j] = make_float4((float)(uint32_t)lo, (float)(uint32_t)(lo >> 32), (float)(uint32_t)hi,
-                            (float)(uint32_t)(hi >> 32));
+        gf[j] = make_float4((float)((double)(uint32_t)lo * gscale),
+                            (float)((double)(uint32_t)(lo >> 32) * gscale),
+                            (float)((double)(uint32_t)hi * gscale),
+                            (float)((double)(uint32_t)(hi >> 32) * gscale));
     }
     if (e == hipSuccess)
         e = hipMemcpyAsync(t->gf, gf.data(), np * sizeof(float4), hipMemcpyHostToDevice, t->stream);

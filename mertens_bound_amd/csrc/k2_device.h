@@ -317,24 +317,35 @@ __device__ __forceinline__ void phase_window(Limbs4 g, Limbs4 v, uint32_t& hi, u
 
 // The FP32 kernels' window: columns 4..5 exact, column 3 in FP32 beside them.  Column 3 adds
 // S3 2^96 with S3 < 2^66, less than 2^-12 turn: it never wraps, so it needs no exact integer sum,
-// only precision well below the FP32 kernels' 2^-35-turn remainder.  gf holds g0..g3 as floats,
-// vf holds v0..v3 times 2^-34 as floats; four FP32 FMAs give t3 = S3 / 2^34 < 2^32 (units of
-// 2^130; worst-case error 2^-34 turn, rounding to nearest), which sincos_window_f adds to the
-// remainder in FP32.  hi:lo = columns 4..5 plus three quarters of a table step, 3 2^(172-TB): the
-// table index rounds there because t3 adds up to half a step, so that the remainder with t3
-// stays within three quarters of a step either side.  Column 5 is one packed dot product
-// (dot2_lo16 of gp, vp); g.x0 is not read (the FP32 C tile stages gp in its place).  Against the
-// exact column 3 this takes four FP32 FMAs (half-cost instructions) and one more in the
-// remainder, where that takes four 64-bit multiply-adds and five carry and register moves.
+// only precision well below the FP32 kernels' 2^-35-turn remainder.  gf holds g0..g3 times
+// 2 pi / 2^44 as floats (rounded once from double on the host), vf holds v0..v3 times 2^-34 as
+// floats (window_vf), so that four FP32 FMAs give column 3 straight in the remainder's units,
+// radians (S3 2^96 / 2^174 turn), and the sum starts from minus three quarters of a table step:
+// t3 = 2 pi S3 / 2^78 - 3/4 step, and sincos_window_f forms the remainder x = R step + t3 in one
+// FMA.  Error of t3: 2 2^-24 relative on S3 from the limbs' roundings (2.0e3 units of 2^-46
+// turn; each limb's own rounding, so no common bias), and the four FMAs' roundings at
+// |t| < 2^-8 rad, 2^-33 rad each (5.2e3 units): with columns 0-2, at most 7.3e3 units,
+// 6.5e-10 rad, ninety times below FP32 rounding of sin/cos.  hi:lo = columns 4..5 plus three quarters of a table step,
+// 3 2^(172-TB): the table index rounds there because column 3 adds up to half a step, so that
+// the remainder with it stays within three quarters of a step either side.  Column 5 is one
+// packed dot product (dot2_lo16 of gp, vp); g.x0 is not read (the FP32 C tile stages gp in its
+// place).  Against the exact column 3 this takes four FP32 FMAs (half-cost instructions), where
+// that takes four 64-bit multiply-adds and five carry and register moves.
+template <int TB>
+struct WindowF {
+    static constexpr float step = 6.28318530717958647692f / (float)(1ull << (32 + TB));
+    static constexpr float tq = step * 3221225472.0f;   // 3/4 of a table step
+};
+
 template <int TB, bool SB = false>
 __device__ __forceinline__ void phase_window_f(Limbs4 g, uint32_t gp, float4 gf, Limbs4 v,
                                                uint32_t vp, float4 vf, uint32_t& hi,
                                                uint32_t& lo, float& t3) {
     static_assert(TB >= 9 && TB <= 11, "table of 512..2048 points");
-    float t = gf.w * vf.x;          // g3 v0
-    t = fmaf(gf.z, vf.y, t);        // g2 v1
-    t = fmaf(gf.y, vf.z, t);        // g1 v2
-    t = fmaf(gf.x, vf.w, t);        // g0 v3
+    float t = fmaf(gf.w, vf.x, -WindowF<TB>::tq);   // g3 v0
+    t = fmaf(gf.z, vf.y, t);                        // g2 v1
+    t = fmaf(gf.y, vf.z, t);                        // g1 v2
+    t = fmaf(gf.x, vf.w, t);                        // g0 v3
     t3 = t;
     uint64_t s4 = mad64<SB>(g.x1, v.x3, 3ull << (44 - TB));
     s4 = mad64<SB>(g.x2, v.x2, s4);
@@ -347,6 +358,14 @@ __device__ __forceinline__ void phase_window_f(Limbs4 g, uint32_t gp, float4 gf,
 __device__ __forceinline__ float4 window_vf(Limbs4 v) {
     constexpr float s = 5.8207660913467407e-11f;   // 2^-34
     return make_float4((float)v.x0 * s, (float)v.x1 * s, (float)v.x2 * s, (float)v.x3 * s);
+}
+
+// a zero's limbs times 2 pi / 2^78 as floats, each rounded once (the screen's records: there
+// the zero takes the scaled side of phase_window_f and the candidate's limbs are plain floats)
+__device__ __forceinline__ float4 window_gx(Limbs4 g) {
+    constexpr double s = 6.283185307179586 / 302231454903657293676544.0;   // 2 pi / 2^78
+    return make_float4((float)((double)g.x0 * s), (float)((double)g.x1 * s),
+                       (float)((double)g.x2 * s), (float)((double)g.x3 * s));
 }
 
 // ---- FP64 sin/cos of 2 pi W / 2^64 ---------------------------------------------------------
@@ -528,10 +547,11 @@ __device__ __forceinline__ void sincos_window(uint32_t hi, uint32_t lo, const Ta
 }
 
 // sin/cos of the FP32 kernels' window (phase_window_f): as sincos_window, with the index rounded
-// at three quarters of a step and column 3 (t3, units of 2^130) added to the remainder:
-// x = 2 pi ((R + t3 2^(TB-12)) 2^-(32+TB) - 3/4 2^-TB), |x| <= 3/4 2 pi 2^-TB (2.3e-3 at TB = 11:
-// |x|^3/6 <= 2.0e-9 and x^4/24 <= 1.2e-12, against FP32 rounding of 6e-8; the x^3 error is odd
-// in x, and x is centred, so it averages out over the zeros).  x/2 is one multiply.
+// at three quarters of a step and column 3 (t3, radians, with the offset) added to the
+// remainder: x = R step + t3 = 2 pi ((R 2^-(32+TB) + S3 2^-78) - 3/4 2^-TB), one FMA,
+// |x| <= 3/4 2 pi 2^-TB (2.3e-3 at TB = 11: |x|^3/6 <= 2.0e-9 and x^4/24 <= 1.2e-12, against
+// FP32 rounding of 6e-8; the x^3 error is odd in x, and x is centred, so it averages out over the
+// zeros).  x/2 is one multiply.
 template <int TB>
 __device__ __forceinline__ void sincos_window_f(uint32_t hi, uint32_t lo, float t3,
                                                 const Tab32* tab, float& S, float& C) {
@@ -539,10 +559,7 @@ __device__ __forceinline__ void sincos_window_f(uint32_t hi, uint32_t lo, float 
     const uint32_t off = (TB == 11 ? hi : hi >> (11 - TB)) & mask;   // 8 * index
     const Tab32 t = *reinterpret_cast<const Tab32*>(reinterpret_cast<const char*>(tab) + off);
     const uint32_t R = __builtin_amdgcn_alignbit(hi, lo, 14 - TB);
-    constexpr float step = 6.28318530717958647692f / (float)(1ull << (32 + TB));
-    constexpr float tstep = step * (float)(1u << TB) / 4096.0f;   // step 2^(TB-12), exact
-    constexpr float tq = step * 3221225472.0f;                     // 3/4 of a table step
-    const float x = fmaf((float)R, step, fmaf(t3, tstep, -tq));
+    const float x = fmaf((float)R, WindowF<TB>::step, t3);
     const float h = 0.5f * x;
     C = fmaf(-x, fmaf(h, t.c, t.s), t.c);
     S = fmaf(x, fmaf(-h, t.s, t.c), t.s);

@@ -320,10 +320,13 @@ __global__ __launch_bounds__(256) void k2_phase_words_kernel(
         W = ((((uint64_t)hi << 32) | lo) - (1ull << (45 - K2_TAB32_BITS))) & ((1ull << 46) - 1);
     } else {
         uint32_t hi, lo;
-        float t3;
+        float t3x;
         phase_window_f<K2_TAB32_BITS>(g, pack_lo16(g.x2, g.x3), zgf[z0 + j], v,
-                                      pack_lo16(v.x3, v.x2), window_vf(v), hi, lo, t3);
-        const uint64_t c3 = (uint64_t)llrint(4.0 * (double)t3);   // units of 2^128
+                                       pack_lo16(v.x3, v.x2), window_vf(v), hi, lo, t3x);
+        // column 3 back in units of 2^128: (t3x + 3/4 step) / (2 pi 2^-46)
+        constexpr double unit = 6.283185307179586 / 70368744177664.0;   // 2 pi 2^-46 rad
+        const double t3 = ((double)t3x + (double)WindowF<K2_TAB32_BITS>::tq) / unit;
+        const uint64_t c3 = (uint64_t)llrint(t3);   // units of 2^128
         W = ((((uint64_t)hi << 32) | lo) - (3ull << (44 - K2_TAB32_BITS)) + c3) &
             ((1ull << 46) - 1);
     }

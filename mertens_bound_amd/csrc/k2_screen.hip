@@ -61,10 +61,10 @@ struct ScreenStages {
 };
 
 // one zero of the screen: g = (column 5's packed limbs pack_lo16(g3, g2), g1, g2, g3) and
-// gvf = g0..g3 times 2^-34 as floats, the operands phase_window_f takes on its candidate side
-// (the window's columns are symmetric in G and V, so the uniform G takes the scalar operand
-// slots and the lane's V the other side); r[m-1] = e^{i m g d} (r padded to an odd count:
-// 16-byte records)
+// gvf = g0..g3 times 2 pi / 2^78 as floats, the operands phase_window_f takes on its
+// candidate side (the window's columns are symmetric in G and V, so the uniform G takes the
+// scalar operand slots and the lane's V the other side); r[m-1] = e^{i m g d} (r padded to an
+// odd count: 16-byte records)
 template <int M>
 struct ScreenRec {
     uint4 g;
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k2_screen_records_kernel(
     ScreenRec<M> z;
     const uint4 gg = zg[j];
     z.g = make_uint4(pack_lo16(gg.w, gg.z), gg.y, gg.z, gg.w);
-    z.gvf = window_vf(Limbs4{gg.x, gg.y, gg.z, gg.w});
+    z.gvf = window_gx(Limbs4{gg.x, gg.y, gg.z, gg.w});
     z.a = make_float2((float)(f * ww.x), (float)(f * ww.y));
 #pragma unroll
     for (int m = 0; m < (M | 1); ++m) {
@@ -122,7 +122,7 @@ __device__ __forceinline__ void screen_term(const Limbs4& v, uint32_t vp, const 
     uint32_t hi, lo;
     float t3;
     phase_window_f<K2_TAB32_BITS, SB>(v, vp, vf, Limbs4{0u, z.g.y, z.g.z, z.g.w}, z.g.x, z.gvf,
-                                      hi, lo, t3);
+                                       hi, lo, t3);
     float S, C;
     sincos_window_f<K2_TAB32_BITS>(hi, lo, t3, s_tab, S, C);
     // q = a e^{i th}

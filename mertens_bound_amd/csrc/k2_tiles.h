@@ -545,7 +545,8 @@ __device__ __forceinline__ void ztile(const ZeroView& zv, int z0, int z1, const 
                         for (int c = 0; c < ZG; ++c) {
                             uint32_t hi, lo;
                             float t3;
-                            phase_window_f<TB, true>(g, gp, gf, v[c], vp[c], vf[c], hi, lo, t3);
+                            phase_window_f<TB, true>(g, gp, gf, v[c], vp[c], vf[c], hi, lo,
+                                                      t3);
                             float S, C;
                             sincos_window_f<TB>(hi, lo, t3, tab32, S, C);
                             add_terms<FL>(tp[c], S, C, a, b, e);

@@ -77,7 +77,7 @@ def test_phase_words_edges(oracle_lib):
     for mode in (0, 1):
         assert np.array_equal(t.phase_words(V, mode=mode), oracle_lib.phase_words(G, V))
     _check_window(t.phase_words(V, mode=2), oracle_lib.phase_words(G, V))
-    _check_window_f(t.phase_words(V, mode=3), oracle_lib.phase_words(G, V))
+    _check_window_f(t.phase_words(V, mode=3), oracle_lib.phase_words(G, V), check_mean=False)
 
 
 def _check_window(win, Wo):
@@ -148,18 +148,22 @@ def test_sincos_split_table_of_words():
             assert abs(c[i] - float(mpmath.cos(t))) <= 3.5e-16
 
 
-def _check_window_f(win, Wo):
-    """The FP32 kernels' window sums column 3 in FP32: four products of floats rounded to 24
-    bits, three FMAs, then truncation to units of 2^130.  Worst case |S3 - t 2^34| <= 6 2^-24 S3
-    with S3 < 2^66 (units of 2^96), i.e. 6 2^8 units of 2^130 = 6144 units of bit 128, plus 4
-    for the truncation and 4 for columns 0..2: |window - floor(W / 2^18)| <= 6152 (mod 2^46).
-    The rounding is to nearest, so the mean error stays near the truncation's -2.5 units."""
+def _check_window_f(win, Wo, check_mean=True):
+    """The FP32 kernels' window sums column 3 in FP32, in radians with the table offset folded
+    in (phase_window_f): the zero's limbs times 2 pi / 2^44 and the candidate's times 2^-34,
+    each rounded once to float, four FMAs starting from minus 3/4 of a step.  Worst case
+    2 2^-24 S3 with S3 < 2^66 (units of 2^96), 2048 units of bit 128, plus the FMAs' four
+    roundings at |t| < 2^-8 rad, 2^-33 rad each = 5218 units, plus 4 for columns 0..2 and 1 for
+    the conversion to units here: |window - floor(W / 2^18)| <= 7271 (mod 2^46).  The roundings
+    are to nearest and each limb's own, so the mean error stays within tens of units over random
+    words (not over a few dozen extreme ones)."""
     m46 = (1 << 46) - 1
     assert np.all(win <= np.uint64(m46))
     d = ((win.astype(object) - (Wo >> np.uint64(18)).astype(object)) & m46)
     d = np.array([x - (1 << 46) if x >= (1 << 45) else x for x in d.ravel()], dtype=np.int64)
-    assert np.abs(d).max() <= 6152
-    assert abs(d.mean()) <= 64
+    assert np.abs(d).max() <= 7271
+    if check_mean:
+        assert abs(d.mean()) <= 64
     return d
 
 

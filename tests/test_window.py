@@ -83,14 +83,28 @@ def _fma32(a, b, c):
     return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
 
 
+TWO_PI = 6.283185307179586476925286766559005768
+
+
+def _window_consts(tb=11):
+    """The kernels' FP32 constants: the remainder step 2 pi 2^-(32+tb) and three quarters of a
+    table step, as the device code forms them in float32."""
+    f32 = np.float32
+    step = f32(f32(TWO_PI) / f32(2.0 ** (32 + tb)))
+    return step, f32(step * f32(3221225472.0))
+
+
 def _window_f(G, V, tb=11):
     """The FP32 kernels' window (phase_window_f): columns 4..5 exact plus three quarters of a
     table step (column 5 mod 2^32, of which the kernels use the low 14 bits), and beside it
-    column 3 summed in float32 from the limbs as floats (v scaled by 2^-34): t3 = S3 / 2^34."""
+    column 3 summed in float32, in radians, from minus three quarters of a step: the zero's limbs
+    times 2 pi / 2^44 and the candidate's times 2^-34 as floats, t3 = 2 pi S3 / 2^78 - 3/4 step."""
     g, v = _limbs(G), _limbs(V)
-    gf = [np.float32(x) for x in g]
+    _, tq = _window_consts(tb)
+    gscale = TWO_PI / 2.0 ** 44
+    gf = [np.float32(float(x) * gscale) for x in g]
     vf = [np.float32(np.float32(x) * np.float32(2.0 ** -34)) for x in v]
-    t = np.float32(gf[3] * vf[0])
+    t = _fma32(gf[3], vf[0], -tq)
     t = _fma32(gf[2], vf[1], t)
     t = _fma32(gf[1], vf[2], t)
     t = _fma32(gf[0], vf[3], t)
@@ -100,46 +114,48 @@ def _window_f(G, V, tb=11):
 
 
 def test_window_f_column3_in_float32_within_bound():
-    """The phase the FP32 kernels use, window - offset + 4 t3 (units of bit 128): column 3 in
-    FP32 has 6 2^-24 relative error on S3 < 2^66 (units of 2^96), at most 6144 units of bit 128,
-    plus 4 for columns 0..2; rounding to nearest keeps the mean within a few units.  A dropped or
-    mis-scaled column-3 term misses by 2^32 units or more, a wrong offset by 2^33."""
+    """The phase the FP32 kernels use, window - offset + (t3 + 3/4 step) / (2 pi 2^-46) (units
+    of bit 128): column 3 in FP32 errs by at most 2 2^-24 S3 with S3 < 2^66 (units of 2^96),
+    2048 units of bit 128, plus four FMA roundings at |t| < 2^-8 rad (2^-33 rad = 1304.5 units
+    each) and 4 for columns 0..2: 7270 units.  Rounding to nearest keeps the mean within a few
+    units.  A dropped or mis-scaled column-3 term misses by 2^32 units or more, a wrong offset by
+    2^33."""
     from fractions import Fraction
     m46 = 1 << 46
+    _, tq = _window_consts()
+    unit = TWO_PI / 2.0 ** 46
     d = []
     for G, V in _cases(3000, 4):
         exact = ((G * V) >> 128) % m46
         w, t3 = _window_f(G, V)
-        eff = Fraction(w - (3 << 33)) + 4 * Fraction(float(t3))
+        eff = Fraction(w - (3 << 33)) + Fraction((float(t3) + float(tq)) / unit)
         x = (eff - exact) % m46
         d.append(float(x - m46 if x >= m46 // 2 else x))
     d = np.array(d)
-    assert np.abs(d).max() <= 6152, np.abs(d).max()
+    assert np.abs(d).max() <= 7270, np.abs(d).max()
     assert abs(d.mean()) <= 64, d.mean()
     assert np.abs(d).max() > 16          # FP32 rounding does move the phase
 
 
 def test_window_f_sincos_float32_within_rounding():
     """sincos_window_f: index = top 11 bits of the window (rounded at three quarters of a step),
-    x = (R + t3/2) 2 pi 2^-43 - 3/4 2 pi 2^-11 in FP32 (two FMAs), h = x/2, and the rotation;
-    compared with sin/cos of the exact phase.  |x| stays within 3/4 of a step, so sin x = x
-    errs by at most 2e-9: the total stays inside 2.5e-7, where a wrong offset, scale or index
-    misses by 1e-3 or more."""
+    x = R 2 pi 2^-43 + t3 in FP32 (one FMA; t3 carries column 3 and minus 3/4 of a step), h = x/2,
+    and the rotation; compared with sin/cos of the exact phase.  |x| stays within 3/4 of a step,
+    so sin x = x errs by at most 2e-9: the total stays inside 2.5e-7, where a wrong offset, scale
+    or index misses by 1e-3 or more."""
     f32 = np.float32
     two_pi = 2 * np.pi
     k = np.arange(2048)
     tab_c = np.cos(two_pi * k / 2048).astype(f32)
     tab_s = np.sin(two_pi * k / 2048).astype(f32)
-    step = f32(two_pi) / f32(2.0 ** 43)
-    tstep = f32(step * f32(0.5))
-    tq = f32(step * f32(3 * 2 ** 30))
+    step, _ = _window_consts()
     errs = []
     for G, V in _cases(3000, 5):
         w, t3 = _window_f(G, V)
         hi, lo = w >> 32, w & 0xFFFFFFFF
         idx = (hi >> 3) & 0x7FF
         R = ((hi << 32 | lo) >> 3) & 0xFFFFFFFF
-        x = _fma32(f32(R), step, _fma32(t3, tstep, -tq))
+        x = _fma32(f32(R), step, t3)
         h = f32(f32(0.5) * x)
         c, s = tab_c[idx], tab_s[idx]
         C = _fma32(-x, _fma32(h, c, s), c)
