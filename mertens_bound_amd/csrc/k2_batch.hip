@@ -60,51 +60,69 @@ __global__ void k2_counter_reset_kernel(int* counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
+// LDS of the persistent kernel: one 40 KiB region, laid out per kind of item --
+//   FP64 C items: the split 2,048-point table (sincos_tabw_at) at [0, 32 KiB + 8), records in
+//                 scalar registers (ctile_f64s);
+//   FP64 Z items: the 1,024-point FP64 table at [0, 16 KiB), the Z-tile scratch above it;
+//   FP32 items:   the FP32 table at [0, 16 KiB), the C-tile records or the Z-tile scratch above;
+// and the next item's index in its last word (inside the FP32 records' span, past the split
+// table's), written and read only between items: every item ends with a barrier.  The table is
+// reloaded when the kind of table changes (a few times per workgroup: microseconds against
+// milliseconds per item), which keeps four workgroups per CU (40 KiB with K2 and K2').
+constexpr size_t K2_BATCH_TAB_BYTES = sizeof(Tab64) * K2_TAB;   // 16 KiB
+template <int FL, int CZ>
+constexpr size_t batch_lds_bytes() {
+    constexpr size_t rec = sizeof(CTileSmem32<FL>) > sizeof(ZTileSmem<CZ>)
+                               ? sizeof(CTileSmem32<FL>) : sizeof(ZTileSmem<CZ>);
+    constexpr size_t low = K2_BATCH_TAB_BYTES + rec;
+    constexpr size_t wide = K2_TABW_DOUBLES * sizeof(double) + sizeof(int);
+    return ((low > wide ? low : wide) + 15) / 16 * 16;
+}
+
 template <int NC, int CZ, int FL, bool ALLOW32>
 __global__ __launch_bounds__(256, 4) void k2_batch_kernel(
     BatchViews views, const BatchItem* __restrict__ items, int n_items, int* counter,
     const uint4* __restrict__ cand, const uint32_t* __restrict__ cidx,
     double* __restrict__ partial, const Tab64* __restrict__ tab64,
     const Tab32* __restrict__ tab32) {
-    // one raw LDS block serves the C-tile records, the Z-tile scratch and the item index: an item
-    // is either kind, and the index is read into a register before the item's work begins
-    constexpr size_t RAW = ctile_smem_bytes<FL>() > sizeof(ZTileSmem<CZ>)
-                               ? ctile_smem_bytes<FL>() : sizeof(ZTileSmem<CZ>);
-    __shared__ __align__(16) unsigned char sm_raw[RAW];
-    auto& sm64 = *reinterpret_cast<CTileSmem64<FL>*>(sm_raw);
-    auto& sm32 = *reinterpret_cast<CTileSmem32<FL>*>(sm_raw);
-    auto& zsm = *reinterpret_cast<ZTileSmem<CZ>*>(sm_raw);
-    int& s_item = *reinterpret_cast<int*>(sm_raw);
-    // one 16 KiB table region holds the FP64 table or, while FP32 items run, the FP32 one; it is
-    // reloaded when the kind of item changes (a few times per workgroup: microseconds against
-    // milliseconds per item), which leaves room for four workgroups per CU
-    static_assert(sizeof(Tab32) * K2_TAB32 <= sizeof(Tab64) * K2_TAB, "FP32 table fits");
-    __shared__ Tab64 s_tab64[K2_TAB];
-    Tab32* s_tab32 = reinterpret_cast<Tab32*>(s_tab64);
+    constexpr size_t LDS = batch_lds_bytes<FL, CZ>();
+    static_assert(sizeof(Tab32) * K2_TAB32 <= K2_BATCH_TAB_BYTES, "FP32 table fits");
+    __shared__ __align__(16) unsigned char lds[LDS];
+    Tab64* s_tab64 = reinterpret_cast<Tab64*>(lds);
+    Tab32* s_tab32 = reinterpret_cast<Tab32*>(lds);
+    double* s_tabw = reinterpret_cast<double*>(lds);
+    auto& sm32 = *reinterpret_cast<CTileSmem32<FL>*>(lds + K2_BATCH_TAB_BYTES);
+    auto& zsm = *reinterpret_cast<ZTileSmem<CZ>*>(lds + K2_BATCH_TAB_BYTES);
+    int& s_item = *reinterpret_cast<int*>(lds + LDS - sizeof(int));
     const int tid = threadIdx.x;
-    int loaded = -1;   // kind of table in s_tab64: 0 FP64, 1 FP32 (uniform over the workgroup)
+    int loaded = -1;   // table in LDS: 0 FP64 1,024-point, 1 FP32, 2 FP64 split (uniform)
     constexpr int CW = K2_WG * NC;
     for (;;) {
         __syncthreads();   // every wave is done with the previous item's LDS before s_item lands
         if (tid == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = s_item;
+        const int it = __builtin_amdgcn_readfirstlane(s_item);   // uniform: scalar loads below
         __syncthreads();
         if (it >= n_items) break;
         const BatchItem I = items[it];
         const ZeroView zv = views.zv[(I.flags >> 8) & 3];
         const bool f32 = ALLOW32 && (I.flags & 2);
-        if ((int)f32 != loaded) {
+        const bool zt = (I.flags & 1) != 0;
+        const int kind = f32 ? 1 : (zt ? 0 : 2);
+        if (kind != loaded) {
             // every lookup of the previous item is done: tiles end with a barrier
-            if (f32) {
+            if (kind == 1) {
                 for (int i = tid; i < K2_TAB32; i += K2_WG) s_tab32[i] = tab32[i];
-            } else {
+            } else if (kind == 0) {
                 for (int i = tid; i < K2_TAB; i += K2_WG) s_tab64[i] = tab64[i];
+            } else {
+                const double* tw = reinterpret_cast<const double*>(tab64 + K2_TAB);
+                for (int i = tid; i < K2_TABW_DOUBLES; i += K2_WG) s_tabw[i] = tw[i];
             }
             __syncthreads();
-            loaded = (int)f32;
+            loaded = kind;
         }
-        if (I.flags & 1) {
+        if (zt) {
             double* out = partial + I.poff;
             if (f32) {
                 if constexpr (ALLOW32)
@@ -141,7 +159,7 @@ __global__ __launch_bounds__(256, 4) void k2_batch_kernel(
             Acc<false> acc[NC];
 #pragma unroll
             for (int i = 0; i < NC; ++i) acc[i].zero();
-            ctile_f64<NC, FL>(zv, I.z0, I.z1, v, acc, sm64, s_tab64);
+            ctile_f64s<NC, FL>(zv.g, zv.a, zv.b, zv.e, I.z0, I.z1, v, acc, s_tabw);
 #pragma unroll
             for (int i = 0; i < NC; ++i)
 #pragma unroll

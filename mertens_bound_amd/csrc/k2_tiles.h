@@ -281,6 +281,22 @@ __device__ __forceinline__ void ctile_f64(const ZeroView& zv, int z0, int z1, co
 //    one compare instead of NC compares), and only then does any lane take the fix-up branch
 //    (phase_carry_case_w: the words drop W bit 0, so bit 110 joins the test and halves the rate).
 
+// p[j] read through the constant address space: with j the same in every lane the load is a
+// scalar one (s_load) whatever else the kernel writes (a kernel that stores or uses atomics
+// through other pointers would otherwise read the records with vector loads into VGPRs).  The
+// data must not change while the kernel runs.
+__device__ __forceinline__ uint4 ld_const(const uint4* p, int j) {
+    typedef __attribute__((address_space(4))) const uint32_t CU;
+    const CU* q = (const CU*)(p + j);   // a C cast: reinterpret_cast cannot change the space
+    return make_uint4(q[0], q[1], q[2], q[3]);
+}
+
+__device__ __forceinline__ double2 ld_const(const double2* p, int j) {
+    typedef __attribute__((address_space(4))) const double CD;
+    const CD* q = (const CD*)(p + j);
+    return make_double2(q[0], q[1]);
+}
+
 // One zero's terms for the lane's NC candidates.  SG: the record sits in scalar registers.
 template <int NC, int FL, bool SG = false>
 __device__ __forceinline__ void ctw_zero(const uint4 gg, uint32_t gp, const double2 aa,
@@ -333,11 +349,11 @@ __device__ __forceinline__ void ctile_f64s(const uint4* __restrict__ zg,
     struct Rec { uint4 g; double2 a, b, e; };
     auto load = [&](int j) -> Rec {
         Rec r;
-        r.g = zg[j];
-        r.a = za[j];
+        r.g = ld_const(zg, j);
+        r.a = ld_const(za, j);
         r.b = r.e = make_double2(0.0, 0.0);
-        if (D1) r.b = zb[j];
-        if (D2) r.e = ze[j];
+        if (D1) r.b = ld_const(zb, j);
+        if (D2) r.e = ld_const(ze, j);
         return r;
     };
     Rec cur = load(z0);
