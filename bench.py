@@ -761,7 +761,7 @@ def run_weights(args, ctx=None):
     n_cand = args.n_cand if (args.n_cand and ctx is None) else c["n_cand"]
     V = synth.make_candidates(n_cand, c["v_lo"], c["v_hi"], c["seed"])
     t = ev.K2Table(tb)
-    t.set_tiling(target_blocks=3072)
+    t.set_tiling(target_blocks=49152)
     Vd = ev.v_tensor(V)
     names, wt = pipeline.reference_weight_vectors(tb.gamma)
     wtd = torch.tensor(wt, device="cuda")

@@ -16,10 +16,13 @@
 //   a_j     2 e^{-x} w (the factor 2 exact, one product per component)
 //                                                            |a - A| <= |A| (3.5 + 9 x) u
 //   b_j     g a (one product per component, g's 3.5 u)       |b - B| <= |B| (8 + 9 x) u
-//   phase   W = floor(G V / 2^110) mod 2^64 exact; the remainder drops W bits [0,3) and the
-//           fraction below W: < 2 pi 2^-61 rad = 2.8e-18 (0.03 u)
-//   sin/cos sincos_tab64_at: table rounding 0.5 u, remainder polynomial 1.2 u, combine 0.55 u,
-//           the dropped bits 0.03 u -> |S - sin|, |C - cos| <= 2.3 u (bounded here by 3 u)
+//   phase   W = floor(G V / 2^110) mod 2^64 exact; the remainder drops W bit 0 and the
+//           fraction below W: < 2 pi 2^-63 rad = 6.8e-19 (0.006 u)
+//   sin/cos sincos_tabw_at (2,048-point split table): table rounding 0.5 u, remainder
+//           polynomial 1.15 u (cos's rounding 0.5 u, sin's dropped x^5/120 <= 0.64 u at the
+//           interval edges), combine 0.55 u, the dropped bit 0.006 u -> |S - sin|, |C - cos|
+//           <= 2.3 u (bounded here by 3 u; checked over 2^20 words and both edges of every
+//           interval in tests/test_gpu_phase.py)
 //   term    the exact products a.x C, a.y S (K2) and b.x S, b.y C (K2') at the computed S, C:
 //           against a.x cos - a.y sin, sqrt(2) |a| 3 u = 4.25 |a| u
 //   group   the products of K2_ENC_GROUP = 8 consecutive zeros enter a group sum by FMAs, 16
@@ -140,18 +143,19 @@ __global__ __launch_bounds__(256) void k2_enclose_bounds_kernel(const double* __
 
 // C tiles over (split, candidate tile) workgroups with error-free accumulation; split s writes
 // rows [4 s, 4 s + 4) of partial: hi, lo of K2, then hi, lo of K2'
+// (the zero records come into scalar registers, ctile_f64s; LDS holds the split sin/cos table)
 template <int NC, int FL>
-__global__ __launch_bounds__(256, 2) void k2_enclose_kernel(
+__global__ __launch_bounds__(256, 4) void k2_enclose_kernel(
     ZeroView zv, int nz, int zps, const uint4* __restrict__ cand, int n_cand, int n_ctiles,
     double* __restrict__ partial, int n_cand_pad, const Tab64* __restrict__ tab64) {
-    __shared__ CTileSmem64<FL> sm;
-    __shared__ Tab64 s_tab[K2_TAB];
+    __shared__ double s_tab[K2_TABW_DOUBLES];
     const int L = xcd_linear_block();
     const int split = L / n_ctiles, ct = L - split * n_ctiles;
     const int z0 = split * zps;
     const int z1 = min(nz, z0 + zps);
     const int tid = threadIdx.x;
-    for (int i = tid; i < K2_TAB; i += K2_WG) s_tab[i] = tab64[i];
+    const double* tw = reinterpret_cast<const double*>(tab64 + K2_TAB);
+    for (int i = tid; i < K2_TABW_DOUBLES; i += K2_WG) s_tab[i] = tw[i];
     Limbs4 v[NC];
     AccGD acc[NC];
 #pragma unroll
@@ -162,7 +166,8 @@ __global__ __launch_bounds__(256, 2) void k2_enclose_kernel(
         v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
         acc[i].zero();
     }
-    ctile_f64<NC, FL>(zv, z0, z1, v, acc, sm, s_tab);
+    __syncthreads();   // the table
+    ctile_f64s<NC, FL>(zv.g, zv.a, zv.b, zv.e, z0, z1, v, acc, s_tab);
     double* prow = partial + (size_t)split * 4 * n_cand_pad;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {

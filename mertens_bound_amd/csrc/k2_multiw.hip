@@ -8,10 +8,12 @@
 // truncated vectors).  The exact phase and the sin/cos are computed once per candidate and zero
 // and shared by all vectors: each further vector costs one FMA per requested sum.
 //
-// C tiles of 256 lanes, NC candidates per lane (4 for up to 5 sums x vectors, 2 for up to 10, so
-// that each staged zero record and weight feeds several candidates); zero records of 128 zeros
-// staged in LDS (double buffered): G, w, g (derivatives only) and 2 wt_m; split partial rows
-// [split][q*M + m][cand].
+// C tiles of 256 lanes, NC candidates per lane (4 for up to 5 sums x vectors, 2 for up to 10);
+// every lane handles the same zero at a time, so the zero's record (G, w, g, the M weights) is
+// read with scalar loads into scalar registers (ld_const) and the VALU instructions take it as
+// their scalar operand, as in the one-launch FP64 C tile (ctile_f64s); LDS holds the split sin/cos
+// table.  The sums take wt_m as stored and are doubled when written (exact: a power of two).
+// Split partial rows [split][q*M + m][cand].
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -24,16 +26,7 @@
 
 using namespace k2dev;
 
-#define K2_MW_TZ 128     // zeros per LDS tile
 #define K2_MW_MAX 8      // weight vectors per call
-
-template <int M, bool DER>
-struct MwSmem {
-    uint4 g[2][K2_MW_TZ];
-    double2 w[2][K2_MW_TZ];
-    double gam[2][DER ? K2_MW_TZ : 1];
-    double wt[2][M][K2_MW_TZ];   // 2 wt_m[j]
-};
 
 template <int M, int FL, int NC>
 __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
@@ -45,14 +38,15 @@ __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
     constexpr bool D2 = (FL & K2_D2) != 0;
     constexpr bool DER = D1 || D2;
     constexpr int NQ = 1 + (D1 ? 1 : 0) + (D2 ? 1 : 0);
-    __shared__ MwSmem<M, DER> sm;
-    __shared__ Tab64 s_tab[K2_TAB];
+    __shared__ double s_tab[K2_TABW_DOUBLES];
     const int L = xcd_linear_block();
     const int split = L / n_ctiles, ct = L - split * n_ctiles;
     const int z0 = split * zps, z1 = min(nz, z0 + zps);
     const int tid = threadIdx.x;
-    for (int i = tid; i < K2_TAB; i += 256) s_tab[i] = tab64[i];
+    const double* tw = reinterpret_cast<const double*>(tab64 + K2_TAB);
+    for (int i = tid; i < K2_TABW_DOUBLES; i += 256) s_tab[i] = tw[i];
     Limbs4 v[NC];
+    uint32_t vp[NC];
     double acc[NC][NQ][M];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
@@ -60,77 +54,50 @@ __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
         uint4 cv = make_uint4(0, 0, 0, 0);
         if (c < n_cand) cv = cand[c];
         v[i] = Limbs4{cv.x, cv.y, cv.z, cv.w};
+        vp[i] = pack_lo16(cv.w, cv.z);
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
 #pragma unroll
             for (int m = 0; m < M; ++m) acc[i][q][m] = 0.0;
     }
-    // staging: lanes 0..127 carry G, w, g of one zero; lanes 128..255 its M weights
-    const int zl = tid & (K2_MW_TZ - 1);
-    const bool rec = tid < K2_MW_TZ;
-    uint4 rg = make_uint4(0, 0, 0, 0);
-    double2 rw = make_double2(0.0, 0.0);
-    double rgam = 0.0;
-    double rwt[M];
-    const int ntiles = z1 > z0 ? (z1 - z0 + K2_MW_TZ - 1) / K2_MW_TZ : 0;
-    auto fetch = [&](int t) {
-        const int j = z0 + t * K2_MW_TZ + zl;
-        const bool in = j < z1;
-        if (rec) {
-            rg = in ? zg[j] : make_uint4(0, 0, 0, 0);
-            rw = in ? zw[j] : make_double2(0.0, 0.0);
-            if (DER) rgam = in ? zgam[j] : 0.0;
-        } else {
+    __syncthreads();   // the table
+    const SinCosCoefW kc = sincos_coef_w();
+    struct Rec { uint4 g; double2 w; double gam; double wt[M]; };
+    auto load = [&](int j) -> Rec {
+        Rec r;
+        r.g = ld_const(zg, j);
+        r.w = ld_const(zw, j);
+        r.gam = DER ? ld_const(zgam, j) : 0.0;
 #pragma unroll
-            for (int m = 0; m < M; ++m) rwt[m] = in ? 2.0 * wt[(size_t)m * ld + j] : 0.0;
-        }
+        for (int m = 0; m < M; ++m) r.wt[m] = ld_const(wt + (size_t)m * ld, j);
+        return r;
     };
-    auto stage = [&](int buf) {
-        if (rec) {
-            sm.g[buf][zl] = rg;
-            sm.w[buf][zl] = rw;
-            if (DER) sm.gam[buf][zl] = rgam;
-        } else {
-#pragma unroll
-            for (int m = 0; m < M; ++m) sm.wt[buf][m][zl] = rwt[m];
-        }
-    };
-    if (ntiles > 0) { fetch(0); stage(0); }
-    __syncthreads();
-    const SinCosCoef kc = sincos_coef();
-    for (int t = 0; t < ntiles; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < ntiles) fetch(t + 1);
-        const int zn = min(K2_MW_TZ, z1 - z0 - t * K2_MW_TZ);   // zeros of this tile
-        for (int z = 0; z < zn; ++z) {
-            const uint4 gg = sm.g[buf][z];
-            const double2 ww = sm.w[buf][z];
-            double wz[M];
-#pragma unroll
-            for (int m = 0; m < M; ++m) wz[m] = sm.wt[buf][m][z];
-            const double g = DER ? sm.gam[buf][z] : 0.0;
+    if (z1 > z0) {
+        Rec cur = load(z0);
+        for (int j = z0; j < z1; ++j) {
+            const Rec nxt = load(min(j + 1, z1 - 1));   // in flight while zero j is summed
+            double S[NC], C[NC];
+            ctw_sincos<NC, true>(cur.g, pack_lo16_s(cur.g.z, cur.g.w), v, vp, s_tab, kc, S, C);
+            const double2 ww = cur.w;
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                double S, C;
-                sincos_tab64_at(phase_rem(Limbs4{gg.x, gg.y, gg.z, gg.w}, v[i]), s_tab, S, C, kc);
-                const double A = fma(ww.x, C, -ww.y * S);          // Re(w e^{i th})
+                const double A = fma(ww.x, C[i], -ww.y * S[i]);          // Re(w e^{i th})
 #pragma unroll
-                for (int m = 0; m < M; ++m) acc[i][0][m] = fma(wz[m], A, acc[i][0][m]);
+                for (int m = 0; m < M; ++m) acc[i][0][m] = fma(cur.wt[m], A, acc[i][0][m]);
                 if constexpr (D1) {
-                    const double gB = g * fma(ww.x, S, ww.y * C);   // g Im(w e^{i th})
+                    const double gB = cur.gam * fma(ww.x, S[i], ww.y * C[i]);   // g Im(w e^{i th})
 #pragma unroll
-                    for (int m = 0; m < M; ++m) acc[i][1][m] = fma(-wz[m], gB, acc[i][1][m]);
+                    for (int m = 0; m < M; ++m) acc[i][1][m] = fma(-cur.wt[m], gB, acc[i][1][m]);
                 }
                 if constexpr (D2) {
-                    const double ggA = g * g * A;
+                    const double ggA = cur.gam * cur.gam * A;
 #pragma unroll
                     for (int m = 0; m < M; ++m)
-                        acc[i][NQ - 1][m] = fma(-wz[m], ggA, acc[i][NQ - 1][m]);
+                        acc[i][NQ - 1][m] = fma(-cur.wt[m], ggA, acc[i][NQ - 1][m]);
                 }
             }
+            cur = nxt;
         }
-        if (t + 1 < ntiles) stage(buf ^ 1);
-        __syncthreads();
     }
     double* prow = partial + (size_t)split * NQ * M * n_cand_pad;
 #pragma unroll
@@ -139,7 +106,8 @@ __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
 #pragma unroll
-            for (int m = 0; m < M; ++m) prow[(size_t)(q * M + m) * n_cand_pad + c] = acc[i][q][m];
+            for (int m = 0; m < M; ++m)
+                prow[(size_t)(q * M + m) * n_cand_pad + c] = 2.0 * acc[i][q][m];
     }
 }
 
@@ -234,9 +202,13 @@ extern "C" int k2_eval_weights_device(k2_table* t, uint32_t n_cand, const uint64
     const int n_ctiles = (int)((n_cand + cw - 1) / cw);
     const int n_cand_pad = n_ctiles * cw;
     int n_splits = 0, zps = 0;
-    split_range((int)n_zeros, n_ctiles, t->tiling.target_blocks, &n_splits, &zps);
     const int nq = 1 + ((flags & K2_D1) ? 1 : 0) + ((flags & K2_D2) ? 1 : 0);
     const int rows = nq * (int)n_w;
+    // split rows of nq * n_w sums each: at most about 1 GiB of partial sums
+    const double row_bytes = (double)rows * n_cand_pad * sizeof(double);
+    const int target = (int)std::max(1.0, std::min((double)t->tiling.target_blocks,
+                                                   (double)(1u << 30) / row_bytes * n_ctiles));
+    split_range((int)n_zeros, n_ctiles, target, &n_splits, &zps);
     const size_t need = (size_t)std::max(1, n_splits) * rows * n_cand_pad * sizeof(double);
     if (e == hipSuccess && need > t->partial_bytes) {
         (void)hipFree(t->partial);

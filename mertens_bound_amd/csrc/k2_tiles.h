@@ -297,13 +297,18 @@ __device__ __forceinline__ double2 ld_const(const double2* p, int j) {
     return make_double2(q[0], q[1]);
 }
 
-// One zero's terms for the lane's NC candidates.  SG: the record sits in scalar registers.
-template <int NC, int FL, bool SG = false>
-__device__ __forceinline__ void ctw_zero(const uint4 gg, uint32_t gp, const double2 aa,
-                                         const double2 bb, const double2 ee,
-                                         const Limbs4 (&v)[NC], const uint32_t (&vp)[NC],
-                                         Acc<false> (&acc)[NC], const double* tab,
-                                         const SinCosCoefW& kc) {
+__device__ __forceinline__ double ld_const(const double* p, int j) {
+    typedef __attribute__((address_space(4))) const double CD;
+    return ((const CD*)p)[j];
+}
+
+// sin/cos of one zero's phase for the lane's NC candidates.  SG: the zero's limbs and gp sit in
+// scalar registers.
+template <int NC, bool SG = false>
+__device__ __forceinline__ void ctw_sincos(const uint4 gg, uint32_t gp, const Limbs4 (&v)[NC],
+                                           const uint32_t (&vp)[NC], const double* tab,
+                                           const SinCosCoefW& kc, double (&S)[NC],
+                                           double (&C)[NC]) {
     const Limbs4 g{gg.x, gg.y, gg.z, gg.w};
     uint32_t c3[NC], c4[NC], c5[NC], ct[NC];
 #pragma unroll
@@ -322,22 +327,32 @@ __device__ __forceinline__ void ctw_zero(const uint4 gg, uint32_t gp, const doub
                 r[i] = phase_rem_w_of(phase_word_fixed(g, v[i], c3[i], c4[i], c5[i], ct[i]));
     }
 #pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        double S, C;
-        sincos_tabw_at(r[i], tab, S, C, kc);
-        accumulate<FL>(acc[i], S, C, aa, bb, ee);
-    }
+    for (int i = 0; i < NC; ++i) sincos_tabw_at(r[i], tab, S[i], C[i], kc);
 }
 
-// Sums zeros [z0, z1) (uniform over the workgroup) into acc for the NC candidates of this lane;
-// tab: the split table in LDS, staged and made visible by the caller.  The next zero's record
-// is loaded while the current one is summed.
-template <int NC, int FL>
+// One zero's terms for the lane's NC candidates.  SG: the record sits in scalar registers.
+template <int NC, int FL, bool SG = false, class ACC = Acc<false>>
+__device__ __forceinline__ void ctw_zero(const uint4 gg, uint32_t gp, const double2 aa,
+                                         const double2 bb, const double2 ee,
+                                         const Limbs4 (&v)[NC], const uint32_t (&vp)[NC],
+                                         ACC (&acc)[NC], const double* tab,
+                                         const SinCosCoefW& kc) {
+    double S[NC], C[NC];
+    ctw_sincos<NC, SG>(gg, gp, v, vp, tab, kc, S, C);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) accumulate<FL>(acc[i], S[i], C[i], aa, bb, ee);
+}
+
+// Sums zeros [z0, z1) (uniform over the workgroup) into acc for the NC candidates of this lane
+// (ACC: Acc<false>, or AccGD in the enclosure mode: its group sums are flushed every
+// K2_ENC_GROUP zeros from z0 and at the end); tab: the split table in LDS, staged and made
+// visible by the caller.  The next zero's record is loaded while the current one is summed.
+template <int NC, int FL, class ACC = Acc<false>>
 __device__ __forceinline__ void ctile_f64s(const uint4* __restrict__ zg,
                                            const double2* __restrict__ za,
                                            const double2* __restrict__ zb,
                                            const double2* __restrict__ ze, int z0, int z1,
-                                           const Limbs4 (&v)[NC], Acc<false> (&acc)[NC],
+                                           const Limbs4 (&v)[NC], ACC (&acc)[NC],
                                            const double* tab) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
@@ -356,12 +371,23 @@ __device__ __forceinline__ void ctile_f64s(const uint4* __restrict__ zg,
         if (D2) r.e = ld_const(ze, j);
         return r;
     };
+    constexpr int GR = AccGroup<ACC>::value;
     Rec cur = load(z0);
     for (int j = z0; j < z1; ++j) {
         const Rec nxt = load(min(j + 1, z1 - 1));   // in flight while zero j is summed
         ctw_zero<NC, FL, true>(cur.g, pack_lo16_s(cur.g.z, cur.g.w), cur.a, cur.b, cur.e, v, vp,
                                acc, tab, kc);
+        if constexpr (GR > 0) {
+            if ((j - z0) % GR == GR - 1) {   // uniform
+#pragma unroll
+                for (int i = 0; i < NC; ++i) acc[i].template flush<FL>();
+            }
+        }
         cur = nxt;
+    }
+    if constexpr (GR > 0) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) acc[i].template flush<FL>();
     }
 }
 
