@@ -52,16 +52,12 @@ PUBLISHED_PEAKS = {"fp64_flops": 78.6e12, "fp32_flops": 157.3e12}
 # measured once on gfx950 (k2_valu_rates, lane operations/s); used only where the live
 # measurement fails
 RECORDED_RATES = {"int32_mad": 33.0e12, "trans": 9.8e12}
-# As-executed VALU issue model (not the roofline): full-rate slot-equivalents per evaluation,
-# one wave64 instruction per 4 cycles per SIMD = 39.3 T lane-slots/s on 256 CUs at 2.4 GHz;
-# FP32 instructions issue in half a slot.  plain: 15 INT32 multiply-adds + 21 FP64 FMAs; mixed
-# FP32 tail as built: 6 integer instructions + 15 FP32 at half a slot; enclosure: + 1.5 FP64
-# additions; screen: 8 integer + 22 FP32 instructions per zero.
-SLOTS = {"plain": 36.0, "plain_d2": 38.0, "mixed": 13.5, "enclose": 37.5, "screen": 19.0}
+# As-executed VALU issue model (not the roofline): SQ_INSTS_VALU per evaluation from the
+# committed counter summaries (wave instructions x 64 / evaluations), each wave64 VALU
+# instruction taking one 4-cycle issue slot of a SIMD: 39.3 T lane-instructions/s on 256 CUs at
+# 2.4 GHz.  The VALU dual-issues some pairs (SQ_ACTIVE_INST_VALU2), so a loop can pass 1.
 VALU_SLOT_PEAK = 39.3e12
-# SQ_INSTS_VALU per evaluation from the committed counter summaries (wave instructions x 64 /
-# evaluations)
-VALU_INSTS = {"plain": 41.7, "mixed": 22.7, "enclose": 44.8, "screen": 30.7}
+VALU_INSTS = {"plain": 39.0, "mixed": 21.7, "enclose": 41.5, "screen": 29.7}
 _RATES = {}
 
 
@@ -108,8 +104,8 @@ def roofline(form, evals, kern_ms, alg_bytes, workload, head_frac=0.0):
                        "unit": units[cl], "frac": ach / peaks[cl]}
     bind = max(classes, key=lambda c: classes[c]["frac"])
     b = classes[bind]
-    slots = SLOTS[form] if form != "mixed" else (head_frac * SLOTS["plain"]
-                                                 + (1.0 - head_frac) * SLOTS["mixed"])
+    insts = VALU_INSTS.get(form) if form != "mixed" else (
+        head_frac * VALU_INSTS["plain"] + (1.0 - head_frac) * VALU_INSTS["mixed"])
     r = {"bound": "valu", "binding_class": bind, "achieved": b["achieved"], "peak": b["peak"],
          "unit": b["unit"], "frac": b["frac"], "traffic": traffic_summary(workload),
          "classes": classes,
@@ -119,11 +115,12 @@ def roofline(form, evals, kern_ms, alg_bytes, workload, head_frac=0.0):
          "kernel_ms": kern_ms, "kernel_evals_per_s": rate,
          "algorithmic_bytes": alg_bytes,
          "hbm_GBps_algorithmic": alg_bytes / (kern_ms * 1e-3) / 1e9,
-         "valu_issue": {"slots_per_eval_as_built": slots,
-                        "valu_issue_frac": rate * slots / VALU_SLOT_PEAK,
-                        "sq_insts_valu_per_eval": VALU_INSTS.get(form),
-                        "note": "issue model, not the roofline: full-rate VALU slots per "
-                                "evaluation as compiled against 39.3 T slots/s"}}
+         "valu_issue": {"sq_insts_valu_per_eval": insts,
+                        "valu_issue_frac": (rate * insts / VALU_SLOT_PEAK
+                                            if insts is not None else None),
+                        "note": "issue model, not the roofline: VALU instructions per "
+                                "evaluation (committed counters) against one wave64 "
+                                "instruction per 4 cycles per SIMD, 39.3 T lane-instr/s"}}
     for k in ("measured_fp64_fma_lane_ops", "measured_fp32_fma_lane_ops"):
         if k in peaks:
             r.setdefault("measured_rates", {})[k] = peaks[k]

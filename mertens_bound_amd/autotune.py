@@ -145,8 +145,18 @@ def tune_batch(table, reqs, V, form, grid=BATCH_GRID, log=print):
         b.close()
         results.append({"tiling": ti, "ms": ms, "grid": info["grid"], "items": info["n_items"]})
         log(f"  batch {form} {ti}: {ms:.2f} ms (grid {info['grid']}, items {info['n_items']})")
+    return prefer_fewer_items(results)
+
+
+def prefer_fewer_items(results, tol=0.005):
+    """Sort by time, then move to the front the tiling with the fewest work items among those
+    within tol of the fastest: each C item reads its zero chunk once, so fewer items means less
+    memory traffic for the same time."""
     results.sort(key=lambda r: r["ms"])
-    return results
+    near = [r for r in results if r["ms"] <= results[0]["ms"] * (1.0 + tol)]
+    pick = min(near, key=lambda r: (r["items"], r["ms"]))
+    results.remove(pick)
+    return [pick] + results
 
 
 def tune_polish(table, V0, k, form, grid=POLISH_GRID, log=print):
