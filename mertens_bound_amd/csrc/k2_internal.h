@@ -113,7 +113,7 @@ struct k2_table {
     int device = 0;
     uint32_t n = 0, n_pad = 0;
     uint4* g = nullptr;
-    float4* gf = nullptr;   // the limbs of g as floats (column 3 of the FP32 window)
+    float4* gf = nullptr;   // the limbs of g times 2 pi / 2^44 as floats (column 3 of the FP32 window)
     double* gamma = nullptr;
     double* w = nullptr;
     k2dev::Tab64* tab64 = nullptr;   // sin/cos tables (sincos_tables)

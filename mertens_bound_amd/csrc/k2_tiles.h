@@ -29,7 +29,7 @@ struct ZeroView {
     const float2* af;
     const float2* bf;
     const float2* ef;
-    const float4* gf;   // g0..g3 as floats (the FP32 window's column 3)
+    const float4* gf;   // g0..g3 times 2 pi / 2^44 as floats (the FP32 window's column 3)
 };
 
 // ---- accumulators -------------------------------------------------------------------------
