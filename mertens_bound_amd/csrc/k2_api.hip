@@ -337,6 +337,8 @@ struct Plan {
     int n_ctiles = 0, n_cand_pad = 0;
     int sp_head = 0, zps_head = 0;   // FP64 part
     int sp_tail = 0, zps_tail = 0;   // FP32 part (mixed only)
+    int nl_head = 0, zl_head = 0;    // long splits leading the FP64 part (guided_splits)
+    int nl_tail = 0, zl_tail = 0;    // and the FP32 part
     int head_end = 0;
 };
 
@@ -355,6 +357,33 @@ void k2impl::split_range(int nz, int n_ctiles, int target, int* n_splits, int* z
     *n_splits = (nz + z - 1) / z;
 }
 
+// Guided splits: with short splits only, the grid that ends the launch evenly has many of them,
+// and each split writes a partial row per candidate (1,515 splits of 1,536 zeros, 1.59 GB of rows
+// for 65,536 candidates over the full table).  Long splits (factor times the short ones) first and
+// short ones last keep the even end with a fifth of the rows: the last factor rounds of the
+// resident workgroups (K2_SLOTS) run short splits, enough to fill every slot while the last long
+// split finishes.  The kernels deal the splits to the 8 XCDs in turn (block_split), each XCD
+// walking its own in order, and the long splits come in multiples of 8, so every XCD gets the
+// same share of long splits and at most one short split more than another.  The splits stay
+// contiguous in zero order and the reduction adds them in that order: a plan's results are the
+// same on every run (another plan's differ by rounding only).
+void k2impl::guided_splits(int nz, int n_ctiles, int target, int factor, SplitPlan* s) {
+    *s = SplitPlan{};
+    split_range(nz, n_ctiles, target, &s->n, &s->zps);
+    if (factor <= 1 || s->zps < 256 || s->n == 0) return;
+    const int64_t n_short = ((int64_t)factor * K2_SLOTS + n_ctiles - 1) / n_ctiles;
+    const int64_t zl = (int64_t)factor * s->zps;
+    const int64_t long_zeros = (int64_t)nz - n_short * s->zps;
+    if (long_zeros < 8 * zl) return;
+    // a multiple of 8 long splits, so that every XCD gets the same number of them
+    const int nl = (int)(long_zeros / zl) / 8 * 8;
+    const int rest = nz - nl * (int)zl;
+    s->n_long = nl;
+    s->zps_long = (int)zl;
+    s->n = nl + (rest + s->zps - 1) / s->zps;
+}
+
+
 static Plan make_plan(const k2_table* t, const WeightArrays* ws, int n_cand, int nz, bool mixed) {
     Plan p;
     const int cw = t->tiling.wg_threads * t->tiling.nc;   // candidates per C tile
@@ -371,10 +400,15 @@ static Plan make_plan(const k2_table* t, const WeightArrays* ws, int n_cand, int
         const int nt = nz - p.head_end;
         const double wh = 2.1 * p.head_end, wt = (double)nt;
         const int tb_head = std::max(p.n_ctiles, (int)std::lround(tb * wh / (wh + wt)));
-        split_range(p.head_end, p.n_ctiles, tb_head, &p.sp_head, &p.zps_head);
-        split_range(nt, p.n_ctiles, std::max(1, tb - tb_head), &p.sp_tail, &p.zps_tail);
+        SplitPlan h, tl;
+        guided_splits(p.head_end, p.n_ctiles, tb_head, K2_SPLIT_LONG, &h);
+        guided_splits(nt, p.n_ctiles, std::max(1, tb - tb_head), 1, &tl);
+        p.sp_head = h.n; p.zps_head = h.zps; p.nl_head = h.n_long; p.zl_head = h.zps_long;
+        p.sp_tail = tl.n; p.zps_tail = tl.zps; p.nl_tail = tl.n_long; p.zl_tail = tl.zps_long;
     } else {
-        split_range(nz, p.n_ctiles, tb, &p.sp_head, &p.zps_head);
+        SplitPlan h;
+        guided_splits(nz, p.n_ctiles, tb, K2_SPLIT_LONG, &h);
+        p.sp_head = h.n; p.zps_head = h.zps; p.nl_head = h.n_long; p.zl_head = h.zps_long;
         p.head_end = nz;
     }
     return p;
@@ -510,6 +544,7 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
                                       std::to_string(need * row_bytes) + " bytes of partial sums, "
                                       "more than the " + std::to_string(pb.cap) + "-byte scratch");
         if (p.sp_head + p.sp_tail > max_rows) {
+            p.nl_head = p.zl_head = p.nl_tail = p.zl_tail = 0;
             if (p.sp_tail > 0 && p.sp_head > 0) {
                 int rh = (int)std::max<int64_t>(1, max_rows * p.sp_head / (p.sp_head + p.sp_tail));
                 rh = (int)std::min<int64_t>(rh, max_rows - 1);
@@ -548,7 +583,8 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
         HIPCHK(hipEventRecord(pp.a, st));
     }
     const bool fused = p.sp_tail > 0 && p.sp_head > 0 && a.nt == 256 && a.nc <= 2 &&
-                       (double)n_cand * nz <= K2_FUSED_MIXED_EVALS;
+                       (double)n_cand * nz <= K2_FUSED_MIXED_EVALS && p.nl_head == 0 &&
+                       p.nl_tail == 0;
     if (fused) {
         // small mixed problem: head and tail splits in one launch on st
         a.z_begin = 0; a.z_end = nz; a.row_base = 0;
@@ -559,20 +595,24 @@ static int eval_core(k2_table* t, int n_cand, const uint4* cand, WeightArrays* w
         HIPCHK(hipEventRecord(t->fork, st));
         HIPCHK(hipStreamWaitEvent(t->aux, t->fork, 0));
         a.z_begin = 0; a.z_end = p.head_end; a.zps = p.zps_head; a.row_base = 0;
-        a.n_blocks = p.n_ctiles * p.sp_head;
+        a.n_long = p.nl_head; a.zps_long = p.zl_head;
+        a.n_blocks = ctile_blocks(p.n_ctiles, p.sp_head, p.nl_head);
         HIPCHK(launch_eval(a, false, t->aux));
         a.z_begin = p.head_end; a.z_end = nz; a.zps = p.zps_tail; a.row_base = p.sp_head;
-        a.n_blocks = p.n_ctiles * p.sp_tail;
+        a.n_long = p.nl_tail; a.zps_long = p.zl_tail;
+        a.n_blocks = ctile_blocks(p.n_ctiles, p.sp_tail, p.nl_tail);
         HIPCHK(launch_eval(a, true, st));
         HIPCHK(hipEventRecord(t->join, t->aux));
         HIPCHK(hipStreamWaitEvent(st, t->join, 0));
     } else if (p.sp_tail > 0) {
         a.z_begin = p.head_end; a.z_end = nz; a.zps = p.zps_tail; a.row_base = 0;
-        a.n_blocks = p.n_ctiles * p.sp_tail;
+        a.n_long = p.nl_tail; a.zps_long = p.zl_tail;
+        a.n_blocks = ctile_blocks(p.n_ctiles, p.sp_tail, p.nl_tail);
         HIPCHK(launch_eval(a, true, st));
     } else if (p.sp_head > 0) {
         a.z_begin = 0; a.z_end = p.head_end; a.zps = p.zps_head; a.row_base = 0;
-        a.n_blocks = p.n_ctiles * p.sp_head;
+        a.n_long = p.nl_head; a.zps_long = p.zl_head;
+        a.n_blocks = ctile_blocks(p.n_ctiles, p.sp_head, p.nl_head);
         HIPCHK(launch_eval(a, false, st));
     }
     if (prof) {

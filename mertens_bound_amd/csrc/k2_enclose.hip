@@ -141,18 +141,19 @@ __global__ __launch_bounds__(256) void k2_enclose_bounds_kernel(const double* __
     }
 }
 
-// C tiles over (split, candidate tile) workgroups with error-free accumulation; split s writes
-// rows [4 s, 4 s + 4) of partial: hi, lo of K2, then hi, lo of K2'
+// C tiles over (split, candidate tile) workgroups (uniform or guided splits, block_split) with
+// error-free accumulation; split s writes rows [4 s, 4 s + 4) of partial: hi, lo of K2, then
+// hi, lo of K2'
 // (the zero records come into scalar registers, ctile_f64s; LDS holds the split sin/cos table)
 template <int NC, int FL>
 __global__ __launch_bounds__(256, 4) void k2_enclose_kernel(
-    ZeroView zv, int nz, int zps, const uint4* __restrict__ cand, int n_cand, int n_ctiles,
-    double* __restrict__ partial, int n_cand_pad, const Tab64* __restrict__ tab64) {
+    ZeroView zv, int nz, int zps, int n_long, int zps_long, const uint4* __restrict__ cand,
+    int n_cand, int n_ctiles, double* __restrict__ partial, int n_cand_pad,
+    const Tab64* __restrict__ tab64) {
     __shared__ double s_tab[K2_TABW_DOUBLES];
-    const int L = xcd_linear_block();
-    const int split = L / n_ctiles, ct = L - split * n_ctiles;
-    const int z0 = split * zps;
-    const int z1 = min(nz, z0 + zps);
+    const BlockSplit bs = block_split(n_ctiles, 0, nz, zps, n_long, zps_long);
+    if (bs.z0 >= nz) return;   // a padding split: no row
+    const int split = bs.split, ct = bs.ct, z0 = bs.z0, z1 = bs.z1;
     const int tid = threadIdx.x;
     const double* tw = reinterpret_cast<const double*>(tab64 + K2_TAB);
     for (int i = tid; i < K2_TABW_DOUBLES; i += K2_WG) s_tab[i] = tw[i];
@@ -216,11 +217,11 @@ void free_enclose(k2_table* t) {
 }
 
 template <int NC, int FL>
-static hipError_t launch_enclose_t(dim3 grid, const ZeroView& zv, int nz, int zps,
+static hipError_t launch_enclose_t(dim3 grid, const ZeroView& zv, int nz, const SplitPlan& sp,
                                    const uint4* cand, int n_cand, int n_ctiles, double* partial,
                                    int n_cand_pad, const Tab64* tab64, hipStream_t st) {
-    hipLaunchKernelGGL((k2_enclose_kernel<NC, FL>), grid, dim3(K2_WG), 0, st, zv, nz, zps, cand,
-                       n_cand, n_ctiles, partial, n_cand_pad, tab64);
+    hipLaunchKernelGGL((k2_enclose_kernel<NC, FL>), grid, dim3(K2_WG), 0, st, zv, nz, sp.zps,
+                       sp.n_long, sp.zps_long, cand, n_cand, n_ctiles, partial, n_cand_pad, tab64);
     return hipGetLastError();
 }
 
@@ -268,8 +269,9 @@ extern "C" int k2_enclose_device(k2_table* t, uint32_t n_cand, const uint64_t* V
     const int nc = t->tiling.nc <= 1 ? 1 : 2;
     const int n_ctiles = std::max(1, (int)((n_cand + 256u * nc - 1) / (256u * nc)));
     const int n_cand_pad = n_ctiles * 256 * nc;
-    int n_splits = 0, zps = 0;
-    if (n_cand) split_range((int)n_zeros, n_ctiles, t->tiling.target_blocks, &n_splits, &zps);
+    SplitPlan sp;
+    if (n_cand) guided_splits((int)n_zeros, n_ctiles, t->tiling.target_blocks, K2_SPLIT_LONG, &sp);
+    const int n_splits = sp.n;
     const size_t need = (size_t)std::max(1, n_splits) * 4 * n_cand_pad * sizeof(double);
     if (n_cand && need > t->enc_partial_bytes) {
         (void)hipFree(t->enc_partial);
@@ -290,17 +292,17 @@ extern "C" int k2_enclose_device(k2_table* t, uint32_t n_cand, const uint64_t* V
     }
     if (e == hipSuccess && n_cand && n_splits > 0) {
         ZeroView zv{t->g, t->enc_a, t->enc_b, nullptr, nullptr, nullptr, nullptr, nullptr};
-        const dim3 grid(n_ctiles * n_splits);
+        const dim3 grid(ctile_blocks(n_ctiles, n_splits, sp.n_long));
         const uint4* cand = (const uint4*)V_dev;
         if (nc == 1)
-            e = d1 ? launch_enclose_t<1, K2_D1>(grid, zv, n_zeros, zps, cand, n_cand, n_ctiles,
+            e = d1 ? launch_enclose_t<1, K2_D1>(grid, zv, n_zeros, sp, cand, n_cand, n_ctiles,
                                                  t->enc_partial, n_cand_pad, t->tab64, st)
-                   : launch_enclose_t<1, 0>(grid, zv, n_zeros, zps, cand, n_cand, n_ctiles,
+                   : launch_enclose_t<1, 0>(grid, zv, n_zeros, sp, cand, n_cand, n_ctiles,
                                             t->enc_partial, n_cand_pad, t->tab64, st);
         else
-            e = d1 ? launch_enclose_t<2, K2_D1>(grid, zv, n_zeros, zps, cand, n_cand, n_ctiles,
+            e = d1 ? launch_enclose_t<2, K2_D1>(grid, zv, n_zeros, sp, cand, n_cand, n_ctiles,
                                                  t->enc_partial, n_cand_pad, t->tab64, st)
-                   : launch_enclose_t<2, 0>(grid, zv, n_zeros, zps, cand, n_cand, n_ctiles,
+                   : launch_enclose_t<2, 0>(grid, zv, n_zeros, sp, cand, n_cand, n_ctiles,
                                             t->enc_partial, n_cand_pad, t->tab64, st);
     }
     if (e == hipSuccess && n_cand) {

@@ -35,6 +35,13 @@ struct WeightArrays {
 #define K2_MIN_SPLIT 32     // fewest zeros per C-tile split (short splits for small problems)
 #define K2_FILL_BLOCKS 2048 // a C-tile grid below this many workgroups gets shorter splits
 #define K2_FUSED_MIXED_EVALS 1e9   // mixed-form problems up to this many evaluations: one launch
+#define K2_SLOTS 1024       // C-tile workgroups resident at once on MI355X (256 CUs x 4)
+// guided splits (guided_splits): long splits of this many short ones first, for the FP64 C-tile
+// launches (one-launch FP64 and the mixed form's head, enclosure); the FP32 tail and the
+// multi-weight kernel keep uniform splits (measured 0.6% and 1.2% slower with long ones)
+#ifndef K2_SPLIT_LONG
+#define K2_SPLIT_LONG 8
+#endif
 
 struct LaunchArgs {
     int nc = 4;
@@ -50,6 +57,7 @@ struct LaunchArgs {
     const float2* zef = nullptr;
     const float4* zgf = nullptr;
     int z_begin = 0, z_end = 0, zps = 0;   // zero range and zeros per split of this launch
+    int n_long = 0, zps_long = 0;          // the first n_long splits hold zps_long zeros each
     const uint4* cand = nullptr;
     int n_cand = 0, n_ctiles = 0;
     double* partial = nullptr;
@@ -175,6 +183,17 @@ hipError_t launch_eval_mixed(const LaunchArgs& a, int head_end, int zps_head, in
 // zeros per split (a multiple of 256) and split count for nz zeros, n_ctiles candidate tiles
 // and a target workgroup count
 void split_range(int nz, int n_ctiles, int target, int* n_splits, int* zps);
+// guided splits over nz zeros: n splits in all, the first n_long of zps_long zeros (a multiple
+// of zps), the rest of zps zeros (the last one shorter), zps as split_range gives it
+struct SplitPlan {
+    int n = 0, zps = 0, n_long = 0, zps_long = 0;
+};
+void guided_splits(int nz, int n_ctiles, int target, int factor, SplitPlan* sp);
+// workgroups of a C-tile grid over sp splits (guided splits pad the split count to a multiple
+// of 8: the kernels deal splits to the XCDs in turn)
+inline int ctile_blocks(int n_ctiles, int sp, int n_long) {
+    return n_ctiles * (n_long > 0 ? (sp + 7) / 8 * 8 : sp);
+}
 // releases the enclosure-mode buffers of a table
 void free_enclose(k2_table* t);
 hipError_t launch_weights(const double* gamma, const double* w, int n, int n_pad, double k,

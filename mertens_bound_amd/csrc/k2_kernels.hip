@@ -65,7 +65,8 @@ __global__ __launch_bounds__(256) void k2_abs_blocks_kernel(const double2* __res
 }
 
 // ---- one-launch evaluation: C tiles over (split, candidate tile) workgroups -----------------------
-// Split s covers zeros [z_begin + s*zps, ...) up to z_end and writes partial row row_base + s.
+// Splits s < n_long cover zeros [z_begin + s*zps_long, ...) (zps_long each), the later ones
+// zps each after those, up to z_end; split s writes partial row row_base + s.
 // F32 selects the precision form (phase window, FP32 sin/cos and weights, compensated FP32
 // sums); the mixed form launches an FP64 head and an FP32 tail side by side.
 // FP64: the zero records come into scalar registers (ctile_f64s) and LDS holds only the
@@ -77,19 +78,19 @@ __global__ __launch_bounds__(256) void k2_abs_blocks_kernel(const double2* __res
 template <int NC, int FL, bool F32, int NT>
 __global__ __launch_bounds__(NT, (NT == 512 ? ((FL & K2_D2) ? 5 : (F32 && NC == 2 ? 8 : 6))
                                             : (NC <= 2 ? 4 : 2))) void k2_eval_kernel(
-    ZeroView zv, int z_begin, int z_end, int zps, const uint4* __restrict__ cand, int n_cand,
-    int n_ctiles, double* __restrict__ partial, int row_base, int n_cand_pad,
-    const Tab64* __restrict__ tab64, const Tab32* __restrict__ tab32) {
+    ZeroView zv, int z_begin, int z_end, int zps, int n_long, int zps_long,
+    const uint4* __restrict__ cand, int n_cand, int n_ctiles, double* __restrict__ partial,
+    int row_base, int n_cand_pad, const Tab64* __restrict__ tab64,
+    const Tab32* __restrict__ tab32) {
     // FP32: the C tile's records and the 2,048-point FP32 table; FP64: only the split
     // 2,048-point table (after the K2_TAB interleaved entries of the FP64 table buffer) -- the
     // FP64 C tile reads its zero records into scalar registers
     __shared__ typename std::conditional<F32, CTileSmem32<FL>, char>::type sm;
     constexpr int NTAB = F32 ? K2_TAB32 : K2_TABW_DOUBLES;
     __shared__ typename std::conditional<F32, Tab32, double>::type s_tab[NTAB];
-    const int L = xcd_linear_block();
-    const int split = L / n_ctiles, ct = L - split * n_ctiles;
-    const int z0 = z_begin + split * zps;
-    const int z1 = min(z_end, z0 + zps);
+    const BlockSplit bs = block_split(n_ctiles, z_begin, z_end, zps, n_long, zps_long);
+    if (bs.z0 >= z_end) return;   // a padding split: no row
+    const int split = bs.split, ct = bs.ct, z0 = bs.z0, z1 = bs.z1;
     const int tid = threadIdx.x;
     for (int i = tid; i < NTAB; i += NT) {
         if constexpr (F32) s_tab[i] = tab32[i];
@@ -440,15 +441,16 @@ static hipError_t launch_eval_t(const LaunchArgs& a, hipStream_t st) {
     if constexpr (NC <= 2) {
         if (a.nt == 512) {
             hipLaunchKernelGGL((k2_eval_kernel<NC, FL, F32, 512>), dim3(a.n_blocks), dim3(512), 0,
-                               st, zv, a.z_begin, a.z_end, a.zps, a.cand, a.n_cand, a.n_ctiles,
-                               a.partial, a.row_base, a.n_cand_pad, a.tab64, a.tab32);
+                               st, zv, a.z_begin, a.z_end, a.zps, a.n_long, a.zps_long, a.cand,
+                               a.n_cand, a.n_ctiles, a.partial, a.row_base, a.n_cand_pad,
+                               a.tab64, a.tab32);
             return hipGetLastError();
         }
     }
     if (a.nt != 256) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k2_eval_kernel<NC, FL, F32, 256>), dim3(a.n_blocks), dim3(256), 0, st, zv,
-                       a.z_begin, a.z_end, a.zps, a.cand, a.n_cand, a.n_ctiles, a.partial,
-                       a.row_base, a.n_cand_pad, a.tab64, a.tab32);
+                       a.z_begin, a.z_end, a.zps, a.n_long, a.zps_long, a.cand, a.n_cand,
+                       a.n_ctiles, a.partial, a.row_base, a.n_cand_pad, a.tab64, a.tab32);
     return hipGetLastError();
 }
 

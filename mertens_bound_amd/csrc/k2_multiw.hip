@@ -31,17 +31,17 @@ using namespace k2dev;
 template <int M, int FL, int NC>
 __global__ __launch_bounds__(256, 3) void k2_multiw_kernel(
     const uint4* __restrict__ zg, const double2* __restrict__ zw, const double* __restrict__ zgam,
-    const double* __restrict__ wt, int ld, int nz, int zps, const uint4* __restrict__ cand,
-    int n_cand, int n_ctiles, double* __restrict__ partial, int n_cand_pad,
-    const Tab64* __restrict__ tab64) {
+    const double* __restrict__ wt, int ld, int nz, k2impl::SplitPlan sp,
+    const uint4* __restrict__ cand, int n_cand, int n_ctiles, double* __restrict__ partial,
+    int n_cand_pad, const Tab64* __restrict__ tab64) {
     constexpr bool D1 = (FL & K2_D1) != 0;
     constexpr bool D2 = (FL & K2_D2) != 0;
     constexpr bool DER = D1 || D2;
     constexpr int NQ = 1 + (D1 ? 1 : 0) + (D2 ? 1 : 0);
     __shared__ double s_tab[K2_TABW_DOUBLES];
-    const int L = xcd_linear_block();
-    const int split = L / n_ctiles, ct = L - split * n_ctiles;
-    const int z0 = split * zps, z1 = min(nz, z0 + zps);
+    const BlockSplit bs = block_split(n_ctiles, 0, nz, sp.zps, sp.n_long, sp.zps_long);
+    if (bs.z0 >= nz) return;   // a padding split: no row
+    const int split = bs.split, ct = bs.ct, z0 = bs.z0, z1 = bs.z1;
     const int tid = threadIdx.x;
     const double* tw = reinterpret_cast<const double*>(tab64 + K2_TAB);
     for (int i = tid; i < K2_TABW_DOUBLES; i += 256) s_tab[i] = tw[i];
@@ -137,33 +137,33 @@ static int mw_nc(int M, unsigned fl) {
 
 template <int M, int FL>
 static hipError_t launch_mw_t(dim3 grid, const k2_table* t, const double* wt, int ld, int nz,
-                              int zps, const uint4* cand, int n_cand, int n_ctiles,
+                              const SplitPlan& sp, const uint4* cand, int n_cand, int n_ctiles,
                               double* partial, int n_cand_pad, hipStream_t st) {
     constexpr int NQ = 1 + ((FL & K2_D1) ? 1 : 0) + ((FL & K2_D2) ? 1 : 0);
     constexpr int NC = NQ * M <= 5 ? 4 : (NQ * M <= 10 ? 2 : 1);
     hipLaunchKernelGGL((k2_multiw_kernel<M, FL, NC>), grid, dim3(256), 0, st, t->g,
-                       (const double2*)t->w, t->gamma, wt, ld, nz, zps, cand, n_cand, n_ctiles,
+                       (const double2*)t->w, t->gamma, wt, ld, nz, sp, cand, n_cand, n_ctiles,
                        partial, n_cand_pad, t->tab64);
     return hipGetLastError();
 }
 
 template <int M>
 static hipError_t launch_mw_fl(unsigned fl, dim3 grid, const k2_table* t, const double* wt,
-                               int ld, int nz, int zps, const uint4* cand, int n_cand,
+                               int ld, int nz, const SplitPlan& sp, const uint4* cand, int n_cand,
                                int n_ctiles, double* partial, int n_cand_pad, hipStream_t st) {
     switch (fl & (K2_D1 | K2_D2)) {
-        case 0: return launch_mw_t<M, 0>(grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
-        case K2_D1: return launch_mw_t<M, K2_D1>(grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
-        case K2_D2: return launch_mw_t<M, K2_D2>(grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
-        default: return launch_mw_t<M, K2_D1 | K2_D2>(grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+        case 0: return launch_mw_t<M, 0>(grid, t, wt, ld, nz, sp, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+        case K2_D1: return launch_mw_t<M, K2_D1>(grid, t, wt, ld, nz, sp, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+        case K2_D2: return launch_mw_t<M, K2_D2>(grid, t, wt, ld, nz, sp, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+        default: return launch_mw_t<M, K2_D1 | K2_D2>(grid, t, wt, ld, nz, sp, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
     }
 }
 
 static hipError_t launch_mw(int M, unsigned fl, dim3 grid, const k2_table* t, const double* wt,
-                            int ld, int nz, int zps, const uint4* cand, int n_cand, int n_ctiles,
-                            double* partial, int n_cand_pad, hipStream_t st) {
+                            int ld, int nz, const SplitPlan& sp, const uint4* cand, int n_cand,
+                            int n_ctiles, double* partial, int n_cand_pad, hipStream_t st) {
     switch (M) {
-#define K2_MW(X) case X: return launch_mw_fl<X>(fl, grid, t, wt, ld, nz, zps, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
+#define K2_MW(X) case X: return launch_mw_fl<X>(fl, grid, t, wt, ld, nz, sp, cand, n_cand, n_ctiles, partial, n_cand_pad, st);
         K2_MW(1) K2_MW(2) K2_MW(3) K2_MW(4) K2_MW(5) K2_MW(6) K2_MW(7) K2_MW(8)
 #undef K2_MW
         default: return hipErrorInvalidValue;
@@ -201,14 +201,15 @@ extern "C" int k2_eval_weights_device(k2_table* t, uint32_t n_cand, const uint64
     const int cw = 256 * mw_nc((int)n_w, flags);   // candidates per C tile
     const int n_ctiles = (int)((n_cand + cw - 1) / cw);
     const int n_cand_pad = n_ctiles * cw;
-    int n_splits = 0, zps = 0;
     const int nq = 1 + ((flags & K2_D1) ? 1 : 0) + ((flags & K2_D2) ? 1 : 0);
     const int rows = nq * (int)n_w;
     // split rows of nq * n_w sums each: at most about 1 GiB of partial sums
     const double row_bytes = (double)rows * n_cand_pad * sizeof(double);
     const int target = (int)std::max(1.0, std::min((double)t->tiling.target_blocks,
                                                    (double)(1u << 30) / row_bytes * n_ctiles));
-    split_range((int)n_zeros, n_ctiles, target, &n_splits, &zps);
+    SplitPlan sp;
+    guided_splits((int)n_zeros, n_ctiles, target, 1, &sp);
+    const int n_splits = sp.n;
     const size_t need = (size_t)std::max(1, n_splits) * rows * n_cand_pad * sizeof(double);
     if (e == hipSuccess && need > t->partial_bytes) {
         (void)hipFree(t->partial);
@@ -218,8 +219,8 @@ extern "C" int k2_eval_weights_device(k2_table* t, uint32_t n_cand, const uint64
         if (e == hipSuccess) t->partial_bytes = need;
     }
     if (e == hipSuccess && n_splits > 0)
-        e = launch_mw((int)n_w, flags, dim3(n_ctiles * n_splits), t, wt_dev, (int)ld,
-                      (int)n_zeros, zps, (const uint4*)V_dev, (int)n_cand, n_ctiles, t->partial,
+        e = launch_mw((int)n_w, flags, dim3(ctile_blocks(n_ctiles, n_splits, sp.n_long)), t,
+                      wt_dev, (int)ld, (int)n_zeros, sp, (const uint4*)V_dev, (int)n_cand, n_ctiles, t->partial,
                       n_cand_pad, st);
     if (e == hipSuccess) {
         const long long cnt = (long long)rows * n_cand;

@@ -20,6 +20,34 @@ __device__ __forceinline__ int xcd_linear_block() {
     return (nb % 8 == 0) ? ((b & 7) * (nb >> 3) + (b >> 3)) : b;
 }
 
+// The split, candidate tile and zero range [z0, z1) of this workgroup of a C-tile grid over
+// (split, candidate tile).  Uniform splits (n_long = 0) of zps zeros from z_begin, in
+// xcd_linear_block's order.  Guided splits (the host's guided_splits: the first n_long of
+// zps_long zeros, then zps each, up to z_end; the grid a multiple of 8 splits): XCD x, which gets
+// every 8th block, walks splits x, x + 8, ... in order, each over all candidate tiles, so that
+// every XCD runs its share of the long splits first and of the short ones last.  z0 >= z_end
+// marks a padding split past the end, which writes no row.
+struct BlockSplit {
+    int split, ct, z0, z1;
+};
+__device__ __forceinline__ BlockSplit block_split(int n_ctiles, int z_begin, int z_end, int zps,
+                                                  int n_long, int zps_long) {
+    BlockSplit r;
+    if (n_long > 0) {
+        const int b = blockIdx.x, k = b >> 3, j = k / n_ctiles;
+        r.ct = k - j * n_ctiles;
+        r.split = 8 * j + (b & 7);
+    } else {
+        const int L = xcd_linear_block();
+        r.split = L / n_ctiles;
+        r.ct = L - r.split * n_ctiles;
+    }
+    const bool lng = r.split < n_long;
+    r.z0 = z_begin + (lng ? r.split * zps_long : n_long * zps_long + (r.split - n_long) * zps);
+    r.z1 = min(z_end, r.z0 + (lng ? zps_long : zps));
+    return r;
+}
+
 // one k's weighted zero arrays (padded with zero weights past the table end)
 struct ZeroView {
     const uint4* g;
