@@ -67,7 +67,9 @@ typedef struct k2_request {
 /* evaluation tiling (chosen by the autotuner; zero fields mean "library default") */
 typedef struct k2_tiling {
     int32_t nc;            /* candidates per lane: 1, 2, 4 or 8 */
-    int32_t target_blocks; /* workgroups per launch the zero splits aim at */
+    int32_t target_blocks; /* workgroups per launch the short zero splits aim at (FP64
+                              launches put long splits of 8 short ones first, so they run
+                              about a fifth of this many) */
     int32_t mixed_head;    /* mixed form: zeros evaluated in FP64 before the FP32 form takes over;
                               0 = the shortest multiple of 4096 zeros after which the weight
                               magnitudes sum_j 2 e^{-k g_j^2} |w_j| add up to <= mixed_tail */
