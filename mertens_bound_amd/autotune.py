@@ -124,8 +124,8 @@ def tune_eval(table, V, k, form, grid=EVAL_GRID, log=print):
         results.append({"tiling": {"nc": nc, "target_blocks": tb, "wg_threads": wg}, "ms": ms})
         log(f"  eval {form} n={n} nc={nc} wg={wg} target_blocks={tb}: {ms:.3f} ms")
     table.set_tiling(nc=2, target_blocks=3072, wg_threads=256)
-    results.sort(key=lambda r: r["ms"])
-    return results
+    # fewer workgroups, fewer zero splits: fewer partial rows written and reduced
+    return prefer_fewest(results, lambda r: r["tiling"]["target_blocks"], 0.003)
 
 
 def tune_batch(table, reqs, V, form, grid=BATCH_GRID, log=print):
@@ -145,16 +145,16 @@ def tune_batch(table, reqs, V, form, grid=BATCH_GRID, log=print):
         b.close()
         results.append({"tiling": ti, "ms": ms, "grid": info["grid"], "items": info["n_items"]})
         log(f"  batch {form} {ti}: {ms:.2f} ms (grid {info['grid']}, items {info['n_items']})")
-    return prefer_fewer_items(results)
+    # each C item reads its zero chunk once: fewer items, less memory traffic
+    return prefer_fewest(results, lambda r: r["items"], 0.005)
 
 
-def prefer_fewer_items(results, tol=0.005):
-    """Sort by time, then move to the front the tiling with the fewest work items among those
-    within tol of the fastest: each C item reads its zero chunk once, so fewer items means less
-    memory traffic for the same time."""
+def prefer_fewest(results, key, tol):
+    """Sort by time, then move to the front the tiling with the smallest key among those within
+    tol of the fastest (the key counts what costs memory traffic but hardly time)."""
     results.sort(key=lambda r: r["ms"])
     near = [r for r in results if r["ms"] <= results[0]["ms"] * (1.0 + tol)]
-    pick = min(near, key=lambda r: (r["items"], r["ms"]))
+    pick = min(near, key=lambda r: (key(r), r["ms"]))
     results.remove(pick)
     return [pick] + results
 
@@ -172,8 +172,8 @@ def tune_polish(table, V0, k, form, grid=POLISH_GRID, log=print):
         results.append({"tiling": {"nc": nc, "target_blocks": tb, "wg_threads": wg}, "ms": ms})
         log(f"  polish {form} nc={nc} wg={wg} target_blocks={tb}: {ms:.3f} ms")
     table.set_tiling(nc=2, target_blocks=3072, wg_threads=256)
-    results.sort(key=lambda r: r["ms"])
-    return results
+    # fewer workgroups, fewer zero splits: fewer partial rows written and reduced
+    return prefer_fewest(results, lambda r: r["tiling"]["target_blocks"], 0.003)
 
 
 def main():
