@@ -402,7 +402,7 @@ static Plan make_plan(const k2_table* t, const WeightArrays* ws, int n_cand, int
         const int tb_head = std::max(p.n_ctiles, (int)std::lround(tb * wh / (wh + wt)));
         SplitPlan h, tl;
         guided_splits(p.head_end, p.n_ctiles, tb_head, K2_SPLIT_LONG, &h);
-        guided_splits(nt, p.n_ctiles, std::max(1, tb - tb_head), 1, &tl);
+        guided_splits(nt, p.n_ctiles, std::max(1, tb - tb_head), K2_SPLIT_LONG, &tl);
         p.sp_head = h.n; p.zps_head = h.zps; p.nl_head = h.n_long; p.zl_head = h.zps_long;
         p.sp_tail = tl.n; p.zps_tail = tl.zps; p.nl_tail = tl.n_long; p.zl_tail = tl.zps_long;
     } else {

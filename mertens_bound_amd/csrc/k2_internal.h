@@ -36,11 +36,14 @@ struct WeightArrays {
 #define K2_FILL_BLOCKS 2048 // a C-tile grid below this many workgroups gets shorter splits
 #define K2_FUSED_MIXED_EVALS 1e9   // mixed-form problems up to this many evaluations: one launch
 #define K2_SLOTS 1024       // C-tile workgroups resident at once on MI355X (256 CUs x 4)
-// guided splits (guided_splits): long splits of this many short ones first, for the FP64 C-tile
-// launches (one-launch FP64 and the mixed form's head, enclosure); the FP32 tail and the
-// multi-weight kernel keep uniform splits (measured 0.6% and 1.2% slower with long ones)
+// guided splits (guided_splits): long splits of this many short ones first, in the one-launch
+// kernels (FP64, the mixed form's head and FP32 tail) and the enclosure; the multi-weight kernel
+// (three workgroups per CU) measured 0.7% slower with them and keeps uniform splits
 #ifndef K2_SPLIT_LONG
 #define K2_SPLIT_LONG 8
+#endif
+#ifndef K2_SPLIT_LONG_MW
+#define K2_SPLIT_LONG_MW 1
 #endif
 
 struct LaunchArgs {

@@ -208,7 +208,7 @@ extern "C" int k2_eval_weights_device(k2_table* t, uint32_t n_cand, const uint64
     const int target = (int)std::max(1.0, std::min((double)t->tiling.target_blocks,
                                                    (double)(1u << 30) / row_bytes * n_ctiles));
     SplitPlan sp;
-    guided_splits((int)n_zeros, n_ctiles, target, 1, &sp);
+    guided_splits((int)n_zeros, n_ctiles, target, K2_SPLIT_LONG_MW, &sp);
     const int n_splits = sp.n;
     const size_t need = (size_t)std::max(1, n_splits) * rows * n_cand_pad * sizeof(double);
     if (e == hipSuccess && need > t->partial_bytes) {
