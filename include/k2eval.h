@@ -194,6 +194,12 @@ int k2_enclose_device(k2_table *t, uint32_t n_cand, const uint64_t *V_dev, doubl
 /* debug: the enclosure mode's e^{-x} (same operations as on the device), for checking its
  * error bound on the host */
 int k2_exp_neg_host(uint32_t n, const double *x, double *out);
+/* debug (host only): the zero split plan of a C-tile launch over nz zeros with n_ctiles
+ * candidate tiles, target workgroups and long-split factor (1 = uniform splits): out[0] splits
+ * in all, out[1] zeros per short split, out[2] long splits (first), out[3] zeros per long split,
+ * out[4] workgroups of the launch (the split count padded to a multiple of 8 when there are long
+ * splits).  Splits are contiguous in zero order and the last one may be shorter. */
+int k2_split_plan(int32_t nz, int32_t n_ctiles, int32_t target, int32_t factor, int32_t *out);
 
 /* The FP64 head (zeros) the mixed form uses for Gaussian constant k over the first n_zeros
  * zeros under the table's tiling (computes and caches k's weights on first use). */

@@ -831,6 +831,20 @@ extern "C" int k2_mixed_head(k2_table* t, double k, uint32_t n_zeros, uint32_t* 
     return K2_OK;
 }
 
+extern "C" int k2_split_plan(int32_t nz, int32_t n_ctiles, int32_t target, int32_t factor,
+                             int32_t* out) {
+    if (!out || nz < 0 || n_ctiles < 1 || target < 1 || factor < 1)
+        return fail(K2_E_ARG, "k2_split_plan: nz >= 0, n_ctiles, target and factor >= 1 required");
+    SplitPlan sp;
+    guided_splits(nz, n_ctiles, target, factor, &sp);
+    out[0] = sp.n;
+    out[1] = sp.zps;
+    out[2] = sp.n_long;
+    out[3] = sp.zps_long;
+    out[4] = ctile_blocks(n_ctiles, sp.n, sp.n_long);
+    return K2_OK;
+}
+
 extern "C" int k2_phase_words(k2_table* t, uint32_t n_cand, const uint64_t* V, uint32_t z0,
                               uint32_t nz, int mode, uint64_t* out) {
     if (!t || (n_cand && nz && (!V || !out))) return fail(K2_E_ARG, "k2_phase_words: null pointer");

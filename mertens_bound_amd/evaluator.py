@@ -88,6 +88,7 @@ def _declare(L):
     L.k2_enclose_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_double, ctypes.c_uint32,
                                     ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]
     L.k2_exp_neg_host.argtypes = [ctypes.c_uint32, _f64p, _f64p]
+    L.k2_split_plan.argtypes = [ctypes.c_int32] * 4 + [ctypes.POINTER(ctypes.c_int32)]
     L.k2_eval_weights_device.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp,
                                          _vp, _vp, _vp]
@@ -361,6 +362,14 @@ def exp_neg_host(x):
     _check(load_library().k2_exp_neg_host(x.size, x.ctypes.data_as(_f64p),
                                           out.ctypes.data_as(_f64p)), "k2_exp_neg_host")
     return out
+
+
+def split_plan(nz, n_ctiles, target, factor):
+    """The zero split plan of a C-tile launch (host arithmetic only): dict with the split count,
+    zeros per short split, long splits and zeros per long split, and the launch's workgroups."""
+    out = (ctypes.c_int32 * 5)()
+    _check(load_library().k2_split_plan(nz, n_ctiles, target, factor, out), "k2_split_plan")
+    return {"n": out[0], "zps": out[1], "n_long": out[2], "zps_long": out[3], "blocks": out[4]}
 
 
 def _requests(requests):
