@@ -35,8 +35,8 @@ def setup():
     import torch
     tb = synth.make_zero_table(100_000, 41)
     t = ev.K2Table(tb)
-    # the metric's tiling: 192 splits x 64 candidate tiles would need 302 MB of partial sums
-    # for 65,536 candidates, so the allocation-free call cuts the plan to fit 64 MiB
+    # at this tiling 131 splits x 64 candidate tiles would need 206 MB of partial sums for
+    # 65,536 candidates, so the allocation-free call cuts the plan to fit 64 MiB
     t.set_tiling(nc=4, target_blocks=12288)
     return tb, t, torch
 
